@@ -1,0 +1,354 @@
+"""a2c_amd: MI355X free-entropy engine for Gaussian phase packets (Pd-H EAM).
+
+Python host mirror over the C-ABI of liba2c_fe.so (include/a2c_fe.h), loaded
+with ctypes.  The method names follow the reference's C++ interface:
+
+    FreeEntropy.formFunctionAndGradient  Minimizer::formFunctionAndGradient
+                                         (minimizer.cpp:83-369)
+    FreeEntropy.findNeighbors            findNeighbors (neighbors.cpp:8-113) +
+                                         Minimizer::findLocalNeighbors
+                                         (minimizer.cpp:833-1144)
+    FreeEntropy.calculateChemicalPotential  minimizer.cpp:588-802
+    FreeEntropy.integrateDiffusion       integrate.cpp:9-87
+    FreeEntropy.minimizeFreeEntropy      Minimizer::minimizeFreeEntropy
+                                         (minimizer.cpp:548-586, TAOBLMVM)
+
+Errors are raised as A2CError carrying the library status code and message
+(the reference exits the process instead).  There is no CPU fallback: if the
+HIP library is missing, importing this package raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liba2c_fe.so")
+
+A2C_OK = 0
+A2C_EINVAL = -1
+A2C_ENOMEM = -2
+A2C_EHIP = -3
+A2C_ENBR_OVERFLOW = -4
+A2C_ENONFINITE = -5
+A2C_ESTATE = -6
+A2C_ENOTCONV = -7
+
+DERIV = {"analytic": 0, "fd": 1}
+PREC = {"fp64": 0, "mixed": 1}
+LIST_IDS = {"PdPd": 0, "PdH": 1, "HPd": 2, "HH": 3, "HH1": 4}
+
+
+class A2CError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("a2c error %d: %s" % (code, msg))
+        self.code = code
+
+
+class FeDesc(C.Structure):
+    _fields_ = [
+        ("n_pd", C.c_int32), ("n_h", C.c_int32),
+        ("box", C.c_double * 3), ("periodic", C.c_int32 * 3),
+        ("rc", C.c_double), ("a_pd", C.c_double), ("temperature", C.c_double),
+        ("quad_rule", C.c_int32), ("deriv_mode", C.c_int32), ("fd_eps", C.c_double),
+        ("precision", C.c_int32), ("device", C.c_int32), ("max_neighbors", C.c_int32),
+    ]
+
+
+class NbrStats(C.Structure):
+    _fields_ = [("max_neib", C.c_int32), ("width", C.c_int32 * 5), ("total", C.c_int64 * 5)]
+
+
+class LmvmOpts(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32), ("max_it", C.c_int32), ("max_funcs", C.c_int32),
+        ("gatol", C.c_double), ("grtol", C.c_double), ("gttol", C.c_double),
+        ("sigma_lb", C.c_double), ("mixed_switch", C.c_int32), ("mixed_gtol", C.c_double),
+    ]
+
+
+class LmvmReport(C.Structure):
+    _fields_ = [
+        ("reason", C.c_int32), ("iterations", C.c_int32), ("evals", C.c_int32),
+        ("evals_fp64", C.c_int32), ("f", C.c_double), ("gnorm", C.c_double),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("n_kernels", C.c_int32), ("names", (C.c_char * 32) * 8),
+        ("avg_ms", C.c_double * 8), ("total_ms", C.c_double),
+    ]
+
+
+class LatticeDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("a", C.c_double),
+        ("seed", C.c_uint64), ("lattice_id", C.c_uint32), ("disp_std", C.c_double),
+        ("sigma_lo", C.c_double), ("sigma_hi", C.c_double),
+        ("x_lo", C.c_double), ("x_hi", C.c_double),
+        ("x_clip_lo", C.c_double), ("x_clip_hi", C.c_double),
+        ("slab_cells", C.c_int32), ("slab_x", C.c_double),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("liba2c_fe.so not built (run a2c_amd/build.py): %s" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    P, I, D, VP = C.POINTER, C.c_int, C.c_double, C.c_void_p
+    dp = P(C.c_double)
+    ip = P(C.c_int32)
+    sig = {
+        "a2c_fe_desc_default": (None, [P(FeDesc)]),
+        "a2c_fe_create": (I, [P(FeDesc), P(VP)]),
+        "a2c_fe_destroy": (None, [VP]),
+        "a2c_fe_last_error": (C.c_char_p, [VP]),
+        "a2c_fe_set_precision": (I, [VP, C.c_int32]),
+        "a2c_fe_set_occupancy": (I, [VP, dp]),
+        "a2c_fe_build_neighbors": (I, [VP, dp, P(NbrStats)]),
+        "a2c_fe_get_neighbors": (I, [VP, C.c_int32, ip, ip]),
+        "a2c_fe_get_pdpdnum": (I, [VP, ip]),
+        "a2c_fe_density": (I, [VP, dp, dp]),
+        "a2c_fe_eval": (I, [VP, dp, dp, dp]),
+        "a2c_fe_eval_device": (I, [VP, VP, VP, VP]),
+        "a2c_tao_objgrad": (I, [VP, dp, dp, dp, VP]),
+        "a2c_fe_chem_potential": (I, [VP, dp, dp, dp, D, dp, ip]),
+        "a2c_fe_diffuse": (I, [VP, D, D, D, D, D, dp, ip, dp]),
+        "a2c_lmvm_opts_default": (None, [P(LmvmOpts)]),
+        "a2c_fe_minimize": (I, [VP, dp, P(LmvmOpts), dp, P(LmvmReport)]),
+        "a2c_dev_alloc": (I, [VP, C.c_size_t, P(VP)]),
+        "a2c_dev_free": (I, [VP, VP]),
+        "a2c_dev_upload": (I, [VP, VP, VP, C.c_size_t]),
+        "a2c_dev_download": (I, [VP, VP, VP, C.c_size_t]),
+        "a2c_fe_sync": (I, [VP]),
+        "a2c_fe_time_eval": (I, [VP, VP, VP, VP, C.c_int32, P(Timing)]),
+        "a2c_lattice_desc_default": (None, [P(LatticeDesc)]),
+        "a2c_lattice_fcc_pdh": (I, [P(LatticeDesc), dp, dp, dp]),
+        "a2c_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d doubles, got %d" % (n, a.size))
+    return a
+
+
+# --------------------------------------------------------------------------
+# specimens
+# --------------------------------------------------------------------------
+def lattice(n, a=3.89, seed=1, lattice_id=0, disp_std=0.02, sigma=(0.10, 0.10),
+            x=(0.10, 0.10), x_clip=(1e-3, 1.0 - 1e-16), slab_cells=0, slab_x=0.0):
+    """Seeded FCC Pd + octahedral-H specimen (include/a2c_lattice.h).
+
+    n: cells per axis (int or 3-tuple).  Returns (X[4N], x[nH], box[3], n_pd, n_h).
+    """
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    d = LatticeDesc()
+    lib.a2c_lattice_desc_default(C.byref(d))
+    d.nx, d.ny, d.nz, d.a = nx, ny, nz, a
+    d.seed, d.lattice_id, d.disp_std = seed, lattice_id, disp_std
+    d.sigma_lo, d.sigma_hi = sigma
+    d.x_lo, d.x_hi = x
+    d.x_clip_lo, d.x_clip_hi = x_clip
+    d.slab_cells, d.slab_x = slab_cells, slab_x
+    nper = 4 * nx * ny * nz
+    X = np.empty(8 * nper, dtype=np.float64)
+    xo = np.empty(nper, dtype=np.float64)
+    box = np.empty(3, dtype=np.float64)
+    rc = lib.a2c_lattice_fcc_pdh(C.byref(d), _dptr(X), _dptr(xo), _dptr(box))
+    if rc != 0:
+        raise A2CError(rc, "bad lattice descriptor")
+    return X, xo, box, nper, nper
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.a2c_philox4x32_10(c, k, o)
+    return list(o)
+
+
+# --------------------------------------------------------------------------
+# engine
+# --------------------------------------------------------------------------
+class FreeEntropy:
+    """One lattice on one GPU (a2c_fe_ctx)."""
+
+    def __init__(self, n_pd, n_h, box=(0.0, 0.0, 0.0), periodic=(0, 0, 0), rc=5.35, a_pd=3.89,
+                 T=300.0, deriv="analytic", eps=1e-5, precision="fp64", device=0, max_neighbors=0):
+        d = FeDesc()
+        lib.a2c_fe_desc_default(C.byref(d))
+        d.n_pd, d.n_h = n_pd, n_h
+        for c in range(3):
+            d.box[c] = float(box[c])
+            d.periodic[c] = int(bool(periodic[c]))
+        d.rc, d.a_pd, d.temperature = rc, a_pd, T
+        d.deriv_mode, d.fd_eps = DERIV[deriv], eps
+        d.precision, d.device, d.max_neighbors = PREC[precision], device, max_neighbors
+        self.desc = d
+        self.n_pd, self.n_h, self.N = n_pd, n_h, n_pd + n_h
+        self._ctx = C.c_void_p()
+        rc_ = lib.a2c_fe_create(C.byref(d), C.byref(self._ctx))
+        if rc_ != 0:
+            raise A2CError(rc_, "a2c_fe_create failed")
+
+    # -- plumbing --
+    def close(self):
+        if self._ctx:
+            lib.a2c_fe_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib.a2c_fe_last_error(self._ctx)
+            raise A2CError(rc, msg.decode() if msg else "")
+
+    def set_precision(self, precision):
+        self._check(lib.a2c_fe_set_precision(self._ctx, PREC[precision]))
+
+    # -- reference-named API --
+    def set_occupancy(self, x):
+        x = _f64(x, self.n_h)
+        self._x = x
+        self._check(lib.a2c_fe_set_occupancy(self._ctx, _dptr(x)))
+
+    def findNeighbors(self, X):
+        X = _f64(X, 4 * self.N)
+        st = NbrStats()
+        self._check(lib.a2c_fe_build_neighbors(self._ctx, _dptr(X), C.byref(st)))
+        return {"max_neib": st.max_neib, "width": list(st.width), "total": list(st.total)}
+
+    build_neighbors = findNeighbors
+
+    def neighbors(self, which):
+        """CSR (offsets, indices) of one list: 'PdPd', 'PdH', 'HPd', 'HH' or 'HH1'."""
+        w = LIST_IDS[which]
+        rows = self.n_pd if w < 2 else self.n_h
+        off = np.zeros(rows + 1, dtype=np.int32)
+        self._check(lib.a2c_fe_get_neighbors(self._ctx, w, _iptr(off), None))
+        idx = np.zeros(max(int(off[-1]), 1), dtype=np.int32)
+        self._check(lib.a2c_fe_get_neighbors(self._ctx, w, _iptr(off), _iptr(idx)))
+        return off, idx[: off[-1]]
+
+    def pdpdnum(self):
+        out = np.zeros((max(self.n_pd, 1), 3), dtype=np.int32)
+        self._check(lib.a2c_fe_get_pdpdnum(self._ctx, _iptr(out)))
+        return out[: self.n_pd]
+
+    def density(self, X):
+        X = _f64(X, 4 * self.N)
+        rho = np.zeros(max(self.N, 1))
+        self._check(lib.a2c_fe_density(self._ctx, _dptr(X), _dptr(rho)))
+        return rho[: self.N]
+
+    def formFunctionAndGradient(self, X):
+        X = _f64(X, 4 * self.N)
+        G = np.zeros(max(4 * self.N, 1))
+        f = C.c_double()
+        self._check(lib.a2c_fe_eval(self._ctx, _dptr(X), C.byref(f), _dptr(G)))
+        return f.value, G[: 4 * self.N]
+
+    eval = formFunctionAndGradient
+
+    def calculateChemicalPotential(self, X, coe_pd, coe_h, gamma_bd, gamma, full_h):
+        """Updates gamma (float64[n_h]) and full_h (int32[n_h]) in place."""
+        X = _f64(X, 4 * self.N)
+        cp, ch = _f64(coe_pd, 4), _f64(coe_h, 4)
+        assert gamma.dtype == np.float64 and full_h.dtype == np.int32
+        self._check(lib.a2c_fe_chem_potential(self._ctx, _dptr(X), _dptr(cp), _dptr(ch), gamma_bd,
+                                              _dptr(gamma), _iptr(full_h)))
+        return gamma, full_h
+
+    def integrateDiffusion(self, dt, nu, Qm, xLb, xUb, gamma, full_h):
+        """One diffusion step from the context occupancy; returns the new x."""
+        g = _f64(gamma, self.n_h)
+        fh = np.ascontiguousarray(full_h, dtype=np.int32)
+        x = np.array(self._x, dtype=np.float64)
+        self._check(lib.a2c_fe_diffuse(self._ctx, dt, nu, Qm, xLb, xUb, _dptr(g), _iptr(fh), _dptr(x)))
+        self._x = x
+        return x
+
+    def minimizeFreeEntropy(self, X0, **opts):
+        """Bounded LMVM relaxation; returns (X, report dict)."""
+        o = LmvmOpts()
+        lib.a2c_lmvm_opts_default(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        X0 = _f64(X0, 4 * self.N)
+        Xo = np.zeros_like(X0)
+        rep = LmvmReport()
+        self._check(lib.a2c_fe_minimize(self._ctx, _dptr(X0), C.byref(o), _dptr(Xo), C.byref(rep)))
+        return Xo, {k: getattr(rep, k) for k, _ in LmvmReport._fields_}
+
+    # -- device-resident helpers --
+    def dev_alloc(self, nbytes):
+        p = C.c_void_p()
+        self._check(lib.a2c_dev_alloc(self._ctx, nbytes, C.byref(p)))
+        return p.value
+
+    def dev_free(self, p):
+        self._check(lib.a2c_dev_free(self._ctx, C.c_void_p(p)))
+
+    def upload(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        self._check(lib.a2c_dev_upload(self._ctx, C.c_void_p(dptr), arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+
+    def download(self, arr, dptr):
+        self._check(lib.a2c_dev_download(self._ctx, arr.ctypes.data_as(C.c_void_p), C.c_void_p(dptr), arr.nbytes))
+        return arr
+
+    def eval_device(self, dX, dG, df):
+        self._check(lib.a2c_fe_eval_device(self._ctx, C.c_void_p(dX), C.c_void_p(dG), C.c_void_p(df)))
+
+    def sync(self):
+        self._check(lib.a2c_fe_sync(self._ctx))
+
+    def time_eval(self, dX, dG, df, iters):
+        t = Timing()
+        self._check(lib.a2c_fe_time_eval(self._ctx, C.c_void_p(dX), C.c_void_p(dG), C.c_void_p(df), iters,
+                                         C.byref(t)))
+        kern = {t.names[k].value.decode(): t.avg_ms[k] for k in range(t.n_kernels)}
+        return {"kernels_ms": kern, "total_ms": t.total_ms}
+
+
+def exported_symbols():
+    """Symbols include/*.h declares (checked by the CPU test suite)."""
+    import re
+    root = os.path.dirname(_HERE)
+    names = []
+    for h in ("a2c_fe.h", "a2c_lattice.h"):
+        txt = open(os.path.join(root, "include", h)).read()
+        names += re.findall(r"\b(a2c_\w+)\s*\(", txt)
+    return sorted(set(names))

@@ -1,0 +1,306 @@
+// FP64 objective + gradient: the plain single-instance kernel of
+// Minimizer::formFunctionAndGradient (minimizer.cpp:83-369).
+//
+//   k_density64  phase I  (:140-157; electrondensity.cpp:12-56): rho per
+//                site, then F(rho), F'(rho) once per site.
+//   k_grad64     phase II (:162-351): pair terms and gradient per site, the
+//                per-site objective share, and a fixed-shape block reduction.
+//   k_final_sum  deterministic sum of the block partials -> f.
+//
+// One lane per site over ELL rows: a wave holds 64 consecutive sites of one
+// species, reads slot t of their rows as one coalesced load, and keeps the
+// per-site sums in registers (no atomics, so results are bitwise
+// reproducible).  The 12-point rule of quadrature.cpp:43-75 is used in its
+// structured form: point (c, +-) of site i displaces r along axis c by
+// +-sqrt(6) sigma_i, point (c, +-) of site j by -+sqrt(6) sigma_j.
+#include <cmath>
+
+#include "eam64.hpp"
+#include "fe_internal.hpp"
+
+namespace a2c {
+namespace {
+
+constexpr int TB = 256;       // threads per block (reduction shape is fixed)
+constexpr double QW = 1.0 / 12.0;  // 0.5/dim, quadrature.cpp:39
+constexpr double SQ6 = 2.449489742783178098;  // sqrt(2)*sqrt(dim/2)
+
+struct Site {
+  double q[3];
+  double s;
+};
+
+__device__ __forceinline__ Site load_site(const double* __restrict__ X, int s) {
+  const double4 v = *reinterpret_cast<const double4*>(X + 4 * (size_t)s);
+  return {{v.x, v.y, v.z}, v.w};
+}
+
+// d = q_i - image(q_j): periodic axes shift q_j by L*rint((q_j - q_i)/L).
+__device__ __forceinline__ void pair_vec(const DevSys& S, const Site& a, const Site& b, double d[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double qj = b.q[c];
+    if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) / S.box[c]);
+    d[c] = a.q[c] - qj;
+  }
+}
+
+template <int DM>
+__device__ __forceinline__ double dF(bool pd, double rho, double eps) {
+  if (DM == A2C_DERIV_FD) {
+    return pd ? (eam::F_Pd(rho + eps) - eam::F_Pd(rho - eps)) / (2.0 * eps)
+              : (eam::F_H(rho + eps) - eam::F_H(rho - eps)) / (2.0 * eps);
+  }
+  return pd ? eam::F_Pd_vd(rho).d : eam::F_H_vd(rho).d;
+}
+
+// Point p in 0..11: axis c = (p>>1)%3, sign = p&1 ? -1 : +1, owner = p<6 ? i : j.
+__device__ __forceinline__ void point_r(int p, const double d[3], double ai, double aj, double r[3]) {
+  const int c = (p >> 1) % 3;
+  const double sg = (p & 1) ? -1.0 : 1.0;
+  const double disp = (p < 6) ? sg * ai : -sg * aj;
+  r[0] = d[0]; r[1] = d[1]; r[2] = d[2];
+  r[c] += disp;
+}
+
+// ---------------------------------------------------------------- density
+template <int DM>
+__global__ __launch_bounds__(TB) void k_density64(DevSys S, const double* __restrict__ X,
+                                                  const double* __restrict__ xo, double* __restrict__ rho_out,
+                                                  double* __restrict__ Fp, double* __restrict__ emb, double eps) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  const Ell LP = pd ? S.pdpd : S.hpd;
+  const Ell LH = pd ? S.pdh : S.hh;
+  const Site a = load_site(X, s);
+  const double ai = SQ6 * a.s;
+  double rho = 0.0;
+  const int np = LP.cnt[row];
+  for (int t = 0; t < np; ++t) {
+    const int j = LP.idx[(size_t)t * LP.rows + row];
+    const Site b = load_site(X, j);
+    double d[3];
+    pair_vec(S, a, b, d);
+    const double aj = SQ6 * b.s;
+    double acc = 0.0;
+#pragma unroll
+    for (int p = 0; p < 12; ++p) {
+      double r[3];
+      point_r(p, d, ai, aj, r);
+      acc += eam::f_Pd(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
+    }
+    rho += acc * QW;
+  }
+  const int nhn = LH.cnt[row];
+  for (int t = 0; t < nhn; ++t) {
+    const int h = LH.idx[(size_t)t * LH.rows + row];
+    const Site b = load_site(X, S.npd + h);
+    double d[3];
+    pair_vec(S, a, b, d);
+    const double aj = SQ6 * b.s;
+    double acc = 0.0;
+#pragma unroll
+    for (int p = 0; p < 12; ++p) {
+      double r[3];
+      point_r(p, d, ai, aj, r);
+      acc += eam::f_H(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
+    }
+    rho += xo[h] * acc * QW;
+  }
+  rho_out[s] = rho;
+  double F = pd ? eam::F_Pd(rho) : eam::F_H(rho);
+  Fp[s] = dF<DM>(pd, rho, eps);
+  emb[s] = pd ? F : xo[row] * F;
+}
+
+// ---------------------------------------------------------------- gradient
+// Pair kinds: 0 Pd-Pd, 1 Pd-H (site Pd), 2 H-Pd (site H), 3 H-H.
+template <int DM, int KIND>
+__device__ __forceinline__ void pair_point(double r, double Fi, double Fj, double eps, double& phi, double& c) {
+  if (DM == A2C_DERIV_FD) {
+    auto fd = [&](auto g) { return (g(r + eps) - g(r - eps)) / (2.0 * eps); };
+    if (KIND == 0) {
+      const double fpd = fd([](double x) { return eam::f_Pd(x); });
+      phi = eam::phi_Pd(r);
+      c = (Fi + Fj) * fpd + fd([](double x) { return eam::phi_Pd(x); });
+    } else if (KIND == 1 || KIND == 2) {
+      const double fpd = fd([](double x) { return eam::f_Pd(x); });
+      const double fh = fd([](double x) { return eam::f_H(x); });
+      phi = eam::phi_morse<0>(r);
+      const double dph = fd([](double x) { return eam::phi_morse<0>(x); });
+      c = (KIND == 1) ? Fi * fh + Fj * fpd + dph : Fi * fpd + Fj * fh + dph;
+    } else {
+      const double fh = fd([](double x) { return eam::f_H(x); });
+      phi = eam::phi_morse<1>(r);
+      c = (Fi + Fj) * fh + fd([](double x) { return eam::phi_morse<1>(x); });
+    }
+  } else {
+    if (KIND == 0) {
+      const eam::VD f = eam::f_Pd_vd(r);
+      const eam::VD ph = eam::phi_Pd_vd(r, f);
+      phi = ph.v;
+      c = (Fi + Fj) * f.d + ph.d;
+    } else if (KIND == 1 || KIND == 2) {
+      const eam::VD f = eam::f_Pd_vd(r);
+      const eam::VD h = eam::f_H_vd(r);
+      const eam::VD ph = eam::phi_morse_vd<0>(r);
+      phi = ph.v;
+      c = (KIND == 1) ? Fi * h.d + Fj * f.d + ph.d : Fi * f.d + Fj * h.d + ph.d;
+    } else {
+      const eam::VD h = eam::f_H_vd(r);
+      const eam::VD ph = eam::phi_morse_vd<1>(r);
+      phi = ph.v;
+      c = (Fi + Fj) * h.d + ph.d;
+    }
+  }
+}
+
+// Accumulate one neighbor pair: energy (phi summed over points), q-gradient
+// and sigma-gradient (r . xi1 is nonzero only on the six points of site i).
+template <int DM, int KIND>
+__device__ __forceinline__ void pair_accum(const double d[3], double ai, double aj, double Fi, double Fj,
+                                           double eps, double& e, double g[3], double& gs) {
+  double pe = 0.0, px = 0.0, py = 0.0, pz = 0.0, ps = 0.0;
+#pragma unroll
+  for (int p = 0; p < 12; ++p) {
+    double r[3];
+    point_r(p, d, ai, aj, r);
+    const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double phi, c;
+    pair_point<DM, KIND>(rn, Fi, Fj, eps, phi, c);
+    pe += phi;
+    const double u = c / rn;
+    px += u * r[0];
+    py += u * r[1];
+    pz += u * r[2];
+    if (p < 6) {
+      const int ax = (p >> 1) % 3;
+      ps += ((p & 1) ? -u : u) * r[ax];  // (c/|r|) r . xi1 / sqrt(3)
+    }
+  }
+  e += pe;
+  g[0] += px; g[1] += py; g[2] += pz;
+  gs += ps;
+}
+
+template <int DM>
+__global__ __launch_bounds__(TB) void k_grad64(DevSys S, const double* __restrict__ X,
+                                               const double* __restrict__ xo, const double* __restrict__ Fp,
+                                               const double* __restrict__ emb, double* __restrict__ G,
+                                               double* __restrict__ part, double eps) {
+  __shared__ double red[TB];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = S.npd + S.nh;
+  double fs = 0.0;
+  if (s < N) {
+    const bool pd = s < S.npd;
+    const int row = pd ? s : s - S.npd;
+    const Ell LP = pd ? S.pdpd : S.hpd;
+    const Ell LH = pd ? S.pdh : S.hh;
+    const Site a = load_site(X, s);
+    const double ai = SQ6 * a.s;
+    const double Fi = Fp[s];
+    const double xi = pd ? 1.0 : xo[row];
+    double ePd = 0.0, eH = 0.0, g[3] = {0.0, 0.0, 0.0}, gs = 0.0;
+    const int np = LP.cnt[row];
+    for (int t = 0; t < np; ++t) {
+      const int j = LP.idx[(size_t)t * LP.rows + row];
+      const Site b = load_site(X, j);
+      double d[3];
+      pair_vec(S, a, b, d);
+      if (pd) pair_accum<DM, 0>(d, ai, SQ6 * b.s, Fi, Fp[j], eps, ePd, g, gs);
+      else pair_accum<DM, 2>(d, ai, SQ6 * b.s, Fi, Fp[j], eps, ePd, g, gs);
+    }
+    const int nhn = LH.cnt[row];
+    for (int t = 0; t < nhn; ++t) {
+      const int h = LH.idx[(size_t)t * LH.rows + row];
+      const Site b = load_site(X, S.npd + h);
+      double d[3];
+      pair_vec(S, a, b, d);
+      const double xh = xo[h];
+      double e1 = 0.0, g1[3] = {0.0, 0.0, 0.0}, gs1 = 0.0;
+      if (pd) pair_accum<DM, 1>(d, ai, SQ6 * b.s, Fi, Fp[S.npd + h], eps, e1, g1, gs1);
+      else pair_accum<DM, 3>(d, ai, SQ6 * b.s, Fi, Fp[S.npd + h], eps, e1, g1, gs1);
+      eH += xh * e1;
+      g[0] += xh * g1[0]; g[1] += xh * g1[1]; g[2] += xh * g1[2];
+      gs += xh * gs1;
+    }
+    // V share of site s: embedding + half of every pair (minimizer.cpp:169-286)
+    const double V = emb[s] + 0.5 * QW * xi * (ePd + eH);
+    const double T = S.T;
+    const double sc = QW * xi / T;
+    G[4 * (size_t)s + 0] = sc * g[0];
+    G[4 * (size_t)s + 1] = sc * g[1];
+    G[4 * (size_t)s + 2] = sc * g[2];
+    G[4 * (size_t)s + 3] = SQ6 * sc * gs - 3.0 * KB * xi / a.s;
+    fs = V / T - 3.0 * KB * xi * log(a.s);
+  }
+  red[threadIdx.x] = fs;
+  __syncthreads();
+#pragma unroll
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Single block: fixed-order sum of the block partials.
+__global__ __launch_bounds__(1024) void k_final_sum(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) acc += part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+}  // namespace
+
+int launch_density64(a2c_fe_ctx* c, const double* dX) {
+  const int N = c->N;
+  if (N == 0) return A2C_OK;
+  const DevSys S = c->sys();
+  dim3 grid((N + TB - 1) / TB);
+  if (c->desc.deriv_mode == A2C_DERIV_FD)
+    hipLaunchKernelGGL(k_density64<A2C_DERIV_FD>, grid, dim3(TB), 0, c->stream, S, dX, c->dx, c->drho, c->dFp,
+                       c->demb, c->desc.fd_eps);
+  else
+    hipLaunchKernelGGL(k_density64<A2C_DERIV_ANALYTIC>, grid, dim3(TB), 0, c->stream, S, dX, c->dx, c->drho,
+                       c->dFp, c->demb, c->desc.fd_eps);
+  A2C_HIP(c, hipGetLastError());
+  return A2C_OK;
+}
+
+int launch_eval64(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
+  const int N = c->N;
+  const DevSys S = c->sys();
+  const int nb = (N + TB - 1) / TB;
+  if (timed) tm->mark(c->stream, "density64");
+  int rc = launch_density64(c, dX);
+  if (rc) return rc;
+  if (timed) tm->mark(c->stream, "grad64");
+  if (N > 0) {
+    if (c->desc.deriv_mode == A2C_DERIV_FD)
+      hipLaunchKernelGGL(k_grad64<A2C_DERIV_FD>, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, c->dFp, c->demb,
+                         dG, c->dpart, c->desc.fd_eps);
+    else
+      hipLaunchKernelGGL(k_grad64<A2C_DERIV_ANALYTIC>, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, c->dFp,
+                         c->demb, dG, c->dpart, c->desc.fd_eps);
+    A2C_HIP(c, hipGetLastError());
+  }
+  if (timed) tm->mark(c->stream, "final_sum");
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(1024), 0, c->stream, c->dpart, nb, dF);
+  A2C_HIP(c, hipGetLastError());
+  if (timed) tm->mark(c->stream, nullptr);
+  return A2C_OK;
+}
+
+}  // namespace a2c

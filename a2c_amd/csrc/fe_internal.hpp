@@ -1,0 +1,128 @@
+// Internal declarations shared by the HIP sources of liba2c_fe.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "a2c_fe.h"
+
+namespace a2c {
+
+constexpr double KB = 8.6173324e-5;  // Boltzmann constant, eV/K (minimizer.cpp:92)
+constexpr int NBR_CAP = 256;          // per-site neighbor cap of the build kernel
+
+// One neighbor list in ELL layout: entry t of row r is idx[t * rows + r], so
+// lane r of a wave reading slot t touches consecutive words (coalesced).
+// Indices are species-local (Pd index or H index), rows sorted ascending.
+struct Ell {
+  int32_t rows = 0;
+  int32_t width = 0;
+  int32_t* idx = nullptr;  // width * rows
+  int32_t* cnt = nullptr;  // rows
+};
+
+// Device view of one lattice: what every eval kernel needs.
+struct DevSys {
+  int32_t npd, nh;
+  double box[3];
+  int32_t periodic[3];
+  double T;
+  Ell pdpd, pdh, hpd, hh, hh1;
+};
+
+// Tables of the mixed-precision path: piecewise cubic in r on a uniform grid.
+struct TabDesc {
+  float r0, inv_h;  // grid origin and inverse spacing
+  int32_t n;        // intervals
+  int32_t pad;
+};
+
+struct Timing;
+
+}  // namespace a2c
+
+struct a2c_fe_ctx {
+  a2c_fe_desc desc;
+  int32_t N = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // state
+  double* dX = nullptr;      // 4N
+  double* dG = nullptr;      // 4N
+  double* dx = nullptr;      // nH occupancy
+  double* drho = nullptr;    // N
+  double* dFp = nullptr;     // N  F'(rho)
+  double* demb = nullptr;    // N  embedding energy per site (x-weighted for H)
+  double* dpart = nullptr;   // block partial sums
+  double* dF = nullptr;      // 1 objective (device)
+  int32_t nblk_red = 0;
+  bool have_x = false;
+  bool have_nbr = false;
+
+  // neighbor lists
+  a2c::Ell ell[5];           // PdPd, PdH, HPd, HH, HH1
+  int32_t* dpdpdnum = nullptr;  // nPd*3
+  a2c_nbr_stats stats{};
+
+  // mixed-precision tables + FP32 mirrors
+  float* dtab = nullptr;     // coefficient tables
+  a2c::TabDesc tab{};
+  float4* dq32 = nullptr;    // unused slot kept for layout experiments
+
+  // host staging (pinned)
+  double* hX = nullptr;
+  double* hG = nullptr;
+
+  a2c::DevSys sys() const {
+    a2c::DevSys s;
+    s.npd = desc.n_pd;
+    s.nh = desc.n_h;
+    for (int c = 0; c < 3; ++c) { s.box[c] = desc.box[c]; s.periodic[c] = desc.periodic[c]; }
+    s.T = desc.temperature;
+    s.pdpd = ell[0]; s.pdh = ell[1]; s.hpd = ell[2]; s.hh = ell[3]; s.hh1 = ell[4];
+    return s;
+  }
+};
+
+namespace a2c {
+
+// error helpers -------------------------------------------------------------
+int set_err(a2c_fe_ctx* c, int code, const std::string& msg);
+#define A2C_HIP(ctx, call)                                                       \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return ::a2c::set_err((ctx), A2C_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// kernels (launchers) ---------------------------------------------------------
+int nbr_build(a2c_fe_ctx* c);                           // nbr.hip
+int launch_eval64(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm);
+int launch_density64(a2c_fe_ctx* c, const double* dX);  // fills drho, dFp, demb
+int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm);
+int build_tables(a2c_fe_ctx* c);                        // eval32.hip
+int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h,
+                double* dgam_raw);                      // occupancy.hip
+int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb,
+                   const double* dgam, const int32_t* dfull, double* dxnew);
+
+// Event timing of individual kernels on the context stream: mark() records
+// the next preallocated event; durations are differences of consecutive marks.
+struct Timing {
+  hipEvent_t* ev = nullptr;
+  int cap = 0;
+  int n = 0;
+  const char* name[8] = {};
+  bool on = false;
+  void mark(hipStream_t s, const char* nm) {
+    if (!on || n >= cap) return;
+    if (n < 8) name[n] = nm;
+    hipEventRecord(ev[n], s);
+    ++n;
+  }
+};
+
+}  // namespace a2c

@@ -1,0 +1,296 @@
+// Cell-list neighbor build: replaces findNeighbors (neighbors.cpp:8-113) and
+// Minimizer::findLocalNeighbors (minimizer.cpp:833-1144).
+//
+// Set semantics (bit-exact with the oracle): j != i and
+// ((dx*dx + dy*dy) + dz*dz) <= rc*rc, d = q_j - q_i in FP64, evaluated with
+// contraction off so no FMA changes a borderline pair (neighbors.cpp:79-82).
+// Periodic axes use d -= L*rint(d/L).  Shells: PdPdnum counts d2 < 0.75a^2,
+// 0.75a^2 < d2 < 1.25a^2, else (minimizer.cpp:935-947); HH1 = H neighbors with
+// d2 < 0.75a^2 (:999-1004).
+//
+// Kernels: k_bin (cell id + histogram), host scan, k_fill (scatter into cell
+// order), k_nbr<false> (row lengths -> ELL widths), k_nbr<true> (sorted ELL
+// rows).  One 64-lane wave per site: lanes test consecutive members of a cell,
+// a ballot compacts hits into LDS, and each hit's slot is its rank among the
+// hits (an O(n) count per lane over LDS broadcasts), so rows come out sorted
+// whatever order the atomics binned the sites in.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fe_internal.hpp"
+
+namespace a2c {
+namespace {
+
+struct Grid {
+  int nc[3];
+  double lo[3], w[3];  // origin and cell width per axis
+  int periodic[3];
+  double box[3];
+};
+
+__device__ __forceinline__ int cell_axis(const Grid& g, double x, int c) {
+  if (g.periodic[c]) x -= g.box[c] * floor(x / g.box[c]);
+  int k = (int)floor((x - g.lo[c]) / g.w[c]);
+  return min(max(k, 0), g.nc[c] - 1);
+}
+
+__global__ void k_bin(const double* __restrict__ X, int N, Grid g, int* __restrict__ cid,
+                      int* __restrict__ ccount) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int cx = cell_axis(g, X[4 * s], 0), cy = cell_axis(g, X[4 * s + 1], 1), cz = cell_axis(g, X[4 * s + 2], 2);
+  int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+  cid[s] = c;
+  atomicAdd(&ccount[c], 1);
+}
+
+__global__ void k_fill(int N, const int* __restrict__ cid, int* __restrict__ cursor, int* __restrict__ order) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int pos = atomicAdd(&cursor[cid[s]], 1);
+  order[pos] = s;
+}
+
+struct NbrArgs {
+  const double* X;
+  int npd, N;
+  Grid g;
+  const int* cid;
+  const int* cstart;
+  const int* order;
+  double rc2, sh1, sh2;
+  int cap;
+  int32_t* cnt[5];    // row lengths: PdPd, PdH (Pd rows); HPd, HH, HH1 (H rows)
+  int32_t* idx[5];    // ELL arrays (fill pass)
+  int32_t width[5];   // ELL widths (fill pass)
+  int32_t* pdpdnum;   // nPd*3 (fill pass)
+  int32_t* maxw;      // 6 ints: max row length per list, max total (count pass)
+  int32_t* overflow;  // 1 if a site exceeds cap
+};
+
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_nbr(NbrArgs a) {
+  __shared__ int32_t sj[NBR_CAP];
+  __shared__ double sd[NBR_CAP];
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (s >= a.N) return;  // uniform over the block (one site per block)
+  const double qi0 = a.X[4 * s], qi1 = a.X[4 * s + 1], qi2 = a.X[4 * s + 2];
+  const int c = a.cid[s];
+  const int ci[3] = {c % a.g.nc[0], (c / a.g.nc[0]) % a.g.nc[1], c / (a.g.nc[0] * a.g.nc[1])};
+  int lst[3][3], cnt[3];
+  for (int ax = 0; ax < 3; ++ax) {  // distinct stencil cells per axis
+    cnt[ax] = 0;
+    for (int o = -1; o <= 1; ++o) {
+      int k = ci[ax] + o;
+      if (a.g.periodic[ax]) k = (k + a.g.nc[ax]) % a.g.nc[ax];
+      else if (k < 0 || k >= a.g.nc[ax]) continue;
+      bool dup = false;
+      for (int t = 0; t < cnt[ax]; ++t) dup |= (lst[ax][t] == k);
+      if (!dup) lst[ax][cnt[ax]++] = k;
+    }
+  }
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int n = 0, npn = 0, nh1 = 0;
+  for (int zz = 0; zz < cnt[2]; ++zz)
+    for (int yy = 0; yy < cnt[1]; ++yy)
+      for (int xx = 0; xx < cnt[0]; ++xx) {
+        const int cc = (lst[2][zz] * a.g.nc[1] + lst[1][yy]) * a.g.nc[0] + lst[0][xx];
+        const int beg = a.cstart[cc], end = a.cstart[cc + 1];
+        for (int base = beg; base < end; base += 64) {
+          const int t = base + lane;
+          bool ok = false;
+          int j = -1;
+          double d2 = 0.0;
+          if (t < end) {
+            j = a.order[t];
+            double d0 = a.X[4 * j] - qi0, d1 = a.X[4 * j + 1] - qi1, dz = a.X[4 * j + 2] - qi2;
+            if (a.g.periodic[0]) d0 = d0 - a.g.box[0] * rint(d0 / a.g.box[0]);
+            if (a.g.periodic[1]) d1 = d1 - a.g.box[1] * rint(d1 / a.g.box[1]);
+            if (a.g.periodic[2]) dz = dz - a.g.box[2] * rint(dz / a.g.box[2]);
+            d2 = d0 * d0 + d1 * d1 + dz * dz;
+            ok = (j != s) && (d2 <= a.rc2);
+          }
+          const unsigned long long m = __ballot(ok);
+          if (FILL && ok) {
+            const int pos = n + __popcll(m & lt);
+            if (pos < NBR_CAP) { sj[pos] = j; sd[pos] = d2; }
+          }
+          n += __popcll(m);
+          npn += __popcll(__ballot(ok && j < a.npd));
+          nh1 += __popcll(__ballot(ok && j >= a.npd && d2 < a.sh1));
+        }
+      }
+  const bool pd = s < a.npd;
+  const int row = pd ? s : s - a.npd;
+  if (n > a.cap) {
+    if (lane == 0) atomicOr(a.overflow, 1);
+    return;
+  }
+  const int nhn = n - npn;
+  if (!FILL) {
+    if (lane == 0) {
+      if (pd) {
+        a.cnt[0][row] = npn; a.cnt[1][row] = nhn;
+        atomicMax(&a.maxw[0], npn); atomicMax(&a.maxw[1], nhn);
+      } else {
+        a.cnt[2][row] = npn; a.cnt[3][row] = nhn; a.cnt[4][row] = nh1;
+        atomicMax(&a.maxw[2], npn); atomicMax(&a.maxw[3], nhn); atomicMax(&a.maxw[4], nh1);
+      }
+      atomicMax(&a.maxw[5], n);
+    }
+    return;
+  }
+  __syncthreads();  // LDS hits visible to every lane of the (single-wave) block
+  const int rowsP = pd ? a.npd : a.N - a.npd;  // rows of this site's lists
+  int32_t* outP = pd ? a.idx[0] : a.idx[2];
+  int32_t* outH = pd ? a.idx[1] : a.idx[3];
+  int sh0 = 0, sh1n = 0, sh2n = 0;
+  for (int e0 = 0; e0 < n; e0 += 64) {
+    const int e = e0 + lane;
+    bool isp = false, s0 = false, s1 = false;
+    if (e < n) {
+      const int key = sj[e];
+      const double d2 = sd[e];
+      isp = key < a.npd;
+      int rank = 0, rank1 = 0;
+      for (int m = 0; m < n; ++m) {
+        const int o = sj[m];
+        const bool less = o < key;
+        rank += less;
+        rank1 += less && (o >= a.npd) && (sd[m] < a.sh1);
+      }
+      if (isp) {
+        outP[(size_t)rank * rowsP + row] = key;  // Pd neighbors occupy ranks 0..npn-1
+        s0 = d2 < a.sh1;
+        s1 = d2 > a.sh1 && d2 < a.sh2;
+      } else {
+        const int h = key - a.npd;
+        outH[(size_t)(rank - npn) * rowsP + row] = h;
+        if (!pd && d2 < a.sh1) a.idx[4][(size_t)rank1 * rowsP + row] = h;
+      }
+    }
+    sh0 += __popcll(__ballot(isp && s0));
+    sh1n += __popcll(__ballot(isp && s1));
+    sh2n += __popcll(__ballot(isp && !s0 && !s1));
+  }
+  if (pd && lane == 0) {
+    a.pdpdnum[3 * row] = sh0;
+    a.pdpdnum[3 * row + 1] = sh1n;
+    a.pdpdnum[3 * row + 2] = sh2n;
+  }
+}
+
+}  // namespace
+
+// Host driver: grid from the positions, bin, scan, count, allocate, fill.
+int nbr_build(a2c_fe_ctx* c) {
+  const int N = c->N, npd = c->desc.n_pd, nh = c->desc.n_h;
+  const double rc = c->desc.rc;
+  Grid g;
+  for (int ax = 0; ax < 3; ++ax) {
+    g.periodic[ax] = c->desc.periodic[ax];
+    g.box[ax] = c->desc.box[ax];
+    double lo, hi;
+    if (g.periodic[ax]) {
+      lo = 0.0; hi = g.box[ax];
+    } else {
+      lo = 1e300; hi = -1e300;
+      for (int s = 0; s < N; ++s) { lo = std::min(lo, c->hX[4 * s + ax]); hi = std::max(hi, c->hX[4 * s + ax]); }
+      if (N == 0) { lo = 0.0; hi = 1.0; }
+    }
+    int nc = std::max(1, (int)std::floor((hi - lo) / rc));
+    if (g.periodic[ax] && nc < 3) nc = 1;
+    g.nc[ax] = nc;
+    g.lo[ax] = lo;
+    g.w[ax] = (hi - lo) / nc;
+    if (!(g.w[ax] > 0)) g.w[ax] = 1.0;
+  }
+  const long long ncell = (long long)g.nc[0] * g.nc[1] * g.nc[2];
+  if (ncell > (1LL << 28)) return set_err(c, A2C_EINVAL, "neighbor grid too large");
+  int *dcid = nullptr, *dcc = nullptr, *dorder = nullptr, *dmisc = nullptr;
+  A2C_HIP(c, hipMalloc(&dcid, sizeof(int) * std::max(N, 1)));
+  A2C_HIP(c, hipMalloc(&dcc, sizeof(int) * (ncell + 1)));
+  A2C_HIP(c, hipMalloc(&dorder, sizeof(int) * std::max(N, 1)));
+  A2C_HIP(c, hipMalloc(&dmisc, sizeof(int) * 8));
+  struct Free {
+    int *a, *b, *d, *e;
+    ~Free() { hipFree(a); hipFree(b); hipFree(d); hipFree(e); }
+  } fr{dcid, dcc, dorder, dmisc};
+  A2C_HIP(c, hipMemsetAsync(dcc, 0, sizeof(int) * (ncell + 1), c->stream));
+  A2C_HIP(c, hipMemsetAsync(dmisc, 0, sizeof(int) * 8, c->stream));
+  const int TB = 256;
+  if (N > 0) {
+    hipLaunchKernelGGL(k_bin, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, c->dX, N, g, dcid, dcc);
+    A2C_HIP(c, hipGetLastError());
+  }
+  std::vector<int> hc(ncell + 1);
+  A2C_HIP(c, hipMemcpyAsync(hc.data(), dcc, sizeof(int) * ncell, hipMemcpyDeviceToHost, c->stream));
+  A2C_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<int> start(ncell + 1, 0);
+  for (long long k = 0; k < ncell; ++k) start[k + 1] = start[k] + hc[k];
+  A2C_HIP(c, hipMemcpyAsync(dcc, start.data(), sizeof(int) * (ncell + 1), hipMemcpyHostToDevice, c->stream));
+  int* dcursor = nullptr;
+  A2C_HIP(c, hipMalloc(&dcursor, sizeof(int) * (ncell + 1)));
+  A2C_HIP(c, hipMemcpyAsync(dcursor, start.data(), sizeof(int) * (ncell + 1), hipMemcpyHostToDevice, c->stream));
+  if (N > 0) {
+    hipLaunchKernelGGL(k_fill, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, N, dcid, dcursor, dorder);
+    A2C_HIP(c, hipGetLastError());
+  }
+  // row-length arrays (kept in the ELL structs)
+  for (int w = 0; w < 5; ++w) {
+    hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt);
+    c->ell[w] = Ell{};
+    c->ell[w].rows = (w < 2) ? npd : nh;
+    A2C_HIP(c, hipMalloc(&c->ell[w].cnt, sizeof(int32_t) * std::max(c->ell[w].rows, 1)));
+    A2C_HIP(c, hipMemsetAsync(c->ell[w].cnt, 0, sizeof(int32_t) * std::max(c->ell[w].rows, 1), c->stream));
+  }
+  NbrArgs a{};
+  a.X = c->dX; a.npd = npd; a.N = N; a.g = g; a.cid = dcid; a.cstart = dcc; a.order = dorder;
+  a.rc2 = rc * rc;
+  a.sh1 = c->desc.a_pd * c->desc.a_pd * 0.75;
+  a.sh2 = c->desc.a_pd * c->desc.a_pd * 1.25;
+  a.cap = std::min(c->desc.max_neighbors > 0 ? c->desc.max_neighbors : NBR_CAP, NBR_CAP);
+  for (int w = 0; w < 5; ++w) a.cnt[w] = c->ell[w].cnt;
+  a.maxw = dmisc;
+  a.overflow = dmisc + 6;
+  if (N > 0) {
+    hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, c->stream, a);
+    A2C_HIP(c, hipGetLastError());
+  }
+  int hm[8];
+  A2C_HIP(c, hipMemcpyAsync(hm, dmisc, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+  A2C_HIP(c, hipStreamSynchronize(c->stream));
+  hipFree(dcursor);
+  if (hm[6]) {
+    c->have_nbr = false;
+    return set_err(c, A2C_ENBR_OVERFLOW, "a site has more neighbors than max_neighbors");
+  }
+  for (int w = 0; w < 5; ++w) {
+    c->ell[w].width = hm[w];
+    size_t n = (size_t)std::max(hm[w], 1) * std::max(c->ell[w].rows, 1);
+    A2C_HIP(c, hipMalloc(&c->ell[w].idx, sizeof(int32_t) * n));
+    A2C_HIP(c, hipMemsetAsync(c->ell[w].idx, 0xff, sizeof(int32_t) * n, c->stream));
+    a.idx[w] = c->ell[w].idx;
+    a.width[w] = hm[w];
+  }
+  hipFree(c->dpdpdnum);
+  c->dpdpdnum = nullptr;
+  A2C_HIP(c, hipMalloc(&c->dpdpdnum, sizeof(int32_t) * 3 * std::max(npd, 1)));
+  a.pdpdnum = c->dpdpdnum;
+  if (N > 0) {
+    hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), 0, c->stream, a);
+    A2C_HIP(c, hipGetLastError());
+  }
+  A2C_HIP(c, hipStreamSynchronize(c->stream));
+  c->stats.max_neib = hm[5];
+  for (int w = 0; w < 5; ++w) c->stats.width[w] = hm[w];
+  c->have_nbr = true;
+  return A2C_OK;
+}
+
+}  // namespace a2c

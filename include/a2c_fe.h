@@ -1,0 +1,179 @@
+/*
+ * a2c_fe.h -- C-ABI of the MI355X free-entropy engine for Gaussian phase
+ * packets (Pd-H EAM).  Plain pointers and sizes; no C++ or torch types.
+ *
+ * Host data layout (identical to the reference's TAO vector):
+ *   X, G : 4*N doubles, site-interleaved [qx, qy, qz, sigma], all Pd sites
+ *          first, then all H sites (minimizer.cpp:398-418, 464-473).
+ *   x    : n_h doubles, H site occupancy (minimizer.h:18).
+ * N = n_pd + n_h.  Pd index i is site i; H index a is site n_pd + a.
+ *
+ * Status codes: 0 = OK, negative = error (A2C_E*); the message of the last
+ * error on a context is returned by a2c_fe_last_error().  No entry point exits
+ * the process (the reference exits or aborts: electrondensity.cpp:50-53,
+ * neighbors.cpp:52-56, minimizer.cpp:420-423).
+ *
+ * Threading: one context per host thread.  Calls are synchronous on the
+ * context's HIP stream unless named *_device / *_async.
+ */
+#ifndef A2C_FE_H
+#define A2C_FE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define A2C_OK 0
+#define A2C_EINVAL (-1)         /* bad argument or descriptor */
+#define A2C_ENOMEM (-2)         /* device or host allocation failed */
+#define A2C_EHIP (-3)           /* HIP runtime error */
+#define A2C_ENBR_OVERFLOW (-4)  /* a site has more neighbors than max_neighbors */
+#define A2C_ENONFINITE (-5)     /* NaN/Inf objective or gradient */
+#define A2C_ESTATE (-6)         /* call order: e.g. eval before build_neighbors */
+#define A2C_ENOTCONV (-7)       /* optimizer stopped without converging */
+
+enum a2c_deriv_mode { A2C_DERIV_ANALYTIC = 0, A2C_DERIV_FD = 1 };
+enum a2c_precision { A2C_PREC_FP64 = 0, A2C_PREC_MIXED = 1 };
+enum a2c_quad_rule { A2C_QUAD_R12 = 0 };
+
+typedef struct a2c_fe_desc {
+  int32_t n_pd, n_h;
+  double box[3];          /* box edge lengths; used on periodic axes */
+  int32_t periodic[3];    /* 1 = minimum image on that axis (needs box >= 2 rc) */
+  double rc;              /* neighbor cutoff, Angstrom (input.file.rc; 5.35) */
+  double a_pd;            /* Pd lattice constant for shell classification (input.file.a_Pd) */
+  double temperature;     /* T (input.file.T0), K */
+  int32_t quad_rule;      /* A2C_QUAD_R12: quadrature.cpp:43-75, weight 1/12 */
+  int32_t deriv_mode;     /* A2C_DERIV_ANALYTIC or A2C_DERIV_FD (reference) */
+  double fd_eps;          /* FD step for A2C_DERIV_FD (input.file.eps; 1e-5) */
+  int32_t precision;      /* A2C_PREC_FP64 or A2C_PREC_MIXED */
+  int32_t device;         /* HIP device ordinal */
+  int32_t max_neighbors;  /* per-site cap over all lists (0: 256) */
+} a2c_fe_desc;
+
+typedef struct a2c_fe_ctx a2c_fe_ctx;
+
+typedef struct a2c_nbr_stats {
+  int32_t max_neib;       /* max neighbors of any site (findNeighbors' return) */
+  int32_t width[5];       /* max row length of PdPd, PdH, HPd, HH, HH1 */
+  int64_t total[5];       /* entries in PdPd, PdH, HPd, HH, HH1 */
+} a2c_nbr_stats;
+
+/* Fill a descriptor with the reference defaults (rc 5.35, eps 1e-5, R12,
+ * analytic derivatives, FP64, device 0, non-periodic). */
+void a2c_fe_desc_default(a2c_fe_desc* d);
+
+/* Replaces Minimizer::initialize (minimizer.cpp:484-546): sizes the problem
+ * and allocates device state. */
+int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out);
+void a2c_fe_destroy(a2c_fe_ctx* ctx);
+const char* a2c_fe_last_error(const a2c_fe_ctx* ctx);
+int a2c_fe_set_precision(a2c_fe_ctx* ctx, int32_t precision);
+
+/* Occupancy x0 held fixed during minimization (AppCtx::x0, minimizer.h:18;
+ * updateApplicationContext, minimizer.cpp:372-378). */
+int a2c_fe_set_occupancy(a2c_fe_ctx* ctx, const double* x);
+
+/* Replaces findNeighbors (neighbors.cpp:8-113) + Minimizer::findLocalNeighbors
+ * (minimizer.cpp:833-1144): neighbor sets at rc from the positions in X
+ * (sigma ignored), PdPdnum shell counts and the first H shell HH1.  Lists are
+ * frozen until the next call. */
+int a2c_fe_build_neighbors(a2c_fe_ctx* ctx, const double* X, a2c_nbr_stats* stats);
+
+/* CSR export of one list (0 PdPd, 1 PdH, 2 HPd, 3 HH, 4 HH1), rows sorted by
+ * neighbor index.  offsets: rows+1 entries; indices: total[which] entries.
+ * Indices are species-local (Pd index or H index) like the reference. */
+int a2c_fe_get_neighbors(a2c_fe_ctx* ctx, int32_t which, int32_t* offsets, int32_t* indices);
+/* PdPdnum: n_pd x 3 shell counts (minimizer.cpp:935-947). */
+int a2c_fe_get_pdpdnum(a2c_fe_ctx* ctx, int32_t* pdpdnum);
+
+/* Per-site averaged electron density rho (N doubles, Pd then H):
+ * electronDensity (electrondensity.cpp:12-56) for every site. */
+int a2c_fe_density(a2c_fe_ctx* ctx, const double* X, double* rho);
+
+/* THE HOT PATH.  Replaces Minimizer::formFunctionAndGradient
+ * (minimizer.cpp:83-369): f = V/T - 3 kB sum x ln sigma and G = df/dX. */
+int a2c_fe_eval(a2c_fe_ctx* ctx, const double* X, double* f, double* G);
+
+/* Same on device-resident buffers (HBM), asynchronous on the context stream:
+ * dX, dG are device pointers of 4N doubles, df a device pointer to 1 double. */
+int a2c_fe_eval_device(a2c_fe_ctx* ctx, const double* dX, double* dG, double* df);
+
+/* TAO-callback-shaped adapter, PetscErrorCode (*)(Tao, Vec X, PetscReal* f,
+ * Vec G, void* ctx) (minimizer.h:77-78): solver is ignored, ctx is an
+ * a2c_fe_ctx*, X and G are host arrays of 4N doubles. */
+int a2c_tao_objgrad(void* solver, const double* X, double* f, double* G, void* ctx);
+
+/* Replaces Minimizer::calculateChemicalPotential (minimizer.cpp:588-802):
+ * gamma_a = ln(x/(1-x)) + dV/dx_a/(kB T) + corrections + 1.5, capped at
+ * gamma_bd; full_h set when capped and x > 0.9999.  Sites with full_h == 1 on
+ * entry keep gamma.  x0 is the context occupancy.  coe_pd/coe_h: 4 each. */
+int a2c_fe_chem_potential(a2c_fe_ctx* ctx, const double* X, const double* coe_pd,
+                          const double* coe_h, double gamma_bd, double* gamma,
+                          int32_t* full_h);
+
+/* Replaces integrateDiffusion (integrate.cpp:9-87): one explicit step of the
+ * mean-field master equation on HH1 with rate nu*exp(-Qm/(kB T)); full_h sites
+ * keep x; result clamped to [xLb, xUb].  x0 is the context occupancy; the new
+ * occupancy is written to x (n_h) and becomes the context occupancy. */
+int a2c_fe_diffuse(a2c_fe_ctx* ctx, double dt, double nu, double Qm, double xLb,
+                   double xUb, const double* gamma, const int32_t* full_h, double* x);
+
+/* ---- bounded limited-memory variable metric (replaces TAOBLMVM, -------
+ * ---- minimizer.cpp:535, TaoSolve :569, bounds :804-830) ---------------- */
+typedef struct a2c_lmvm_opts {
+  int32_t m;            /* stored pairs (-tao_lmm_vectors; 5) */
+  int32_t max_it;       /* -tao_max_it (2000) */
+  int32_t max_funcs;    /* -tao_max_funcs (4000) */
+  double gatol;         /* -tao_gatol: ||proj g||_2 <= gatol (1e-8) */
+  double grtol;         /* -tao_grtol: ||proj g||/|f| <= grtol (1e-8) */
+  double gttol;         /* -tao_gttol: ||proj g||/||proj g0|| <= gttol (0) */
+  double sigma_lb;      /* lower bound on sigma (1e-6); q unbounded */
+  int32_t mixed_switch; /* MIXED precision: switch to FP64 evals once
+                           ||g|| <= mixed_gtol (1) */
+  double mixed_gtol;    /* (1e-5) */
+} a2c_lmvm_opts;
+
+typedef struct a2c_lmvm_report {
+  int32_t reason;       /* >0 converged (1 gatol, 2 grtol, 3 gttol); <0 stopped */
+  int32_t iterations;
+  int32_t evals;        /* total objective+gradient evaluations */
+  int32_t evals_fp64;   /* of which in FP64 */
+  double f;             /* final objective */
+  double gnorm;         /* final projected-gradient 2-norm */
+} a2c_lmvm_report;
+
+void a2c_lmvm_opts_default(a2c_lmvm_opts* o);
+
+/* Replaces Minimizer::minimizeFreeEntropy (minimizer.cpp:548-586): minimizes
+ * f over X (sigma >= sigma_lb), starting from X0; neighbor lists and x0 stay
+ * frozen.  X_out may alias X0 (host arrays of 4N doubles). */
+int a2c_fe_minimize(a2c_fe_ctx* ctx, const double* X0, const a2c_lmvm_opts* opts,
+                    double* X_out, a2c_lmvm_report* report);
+
+/* ---- device memory helpers (so callers need no other HIP runtime) ------- */
+int a2c_dev_alloc(a2c_fe_ctx* ctx, size_t bytes, void** ptr);
+int a2c_dev_free(a2c_fe_ctx* ctx, void* ptr);
+int a2c_dev_upload(a2c_fe_ctx* ctx, void* dst, const void* src, size_t bytes);
+int a2c_dev_download(a2c_fe_ctx* ctx, void* dst, const void* src, size_t bytes);
+int a2c_fe_sync(a2c_fe_ctx* ctx);
+
+/* ---- measurement ------------------------------------------------------- */
+typedef struct a2c_timing {
+  int32_t n_kernels;
+  char names[8][32];    /* kernel names in launch order */
+  double avg_ms[8];     /* average duration per launch (HIP events, ctx stream) */
+  double total_ms;      /* wall of the whole timed region (HIP events) */
+} a2c_timing;
+
+/* Runs `iters` device-resident evaluations of dX on the context stream,
+ * timing each kernel with HIP events recorded on that stream. */
+int a2c_fe_time_eval(a2c_fe_ctx* ctx, const double* dX, double* dG, double* df,
+                     int32_t iters, a2c_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

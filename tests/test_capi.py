@@ -1,0 +1,75 @@
+"""CPU-side checks of the product library: it loads, exports every symbol the
+public headers declare, validates descriptors before touching a GPU, and its
+specimen generator is the documented one."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import a2c_amd as A
+
+
+def test_library_exports_every_declared_symbol():
+    raw = C.CDLL(A.LIB_PATH)
+    names = A.exported_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(raw, n)]
+    assert not missing, missing
+
+
+def test_philox_known_answers():
+    """Random123 known-answer vectors for Philox4x32-10."""
+    assert A.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    f = 0xFFFFFFFF
+    assert A.philox4x32_10([f, f, f, f], [f, f]) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert A.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_specimen_layout():
+    X, x, box, npd, nh = A.lattice((2, 3, 4), disp_std=0.0, sigma=(0.1, 0.1), x=(0.2, 0.2))
+    assert npd == nh == 4 * 24 and X.size == 4 * (npd + nh) and x.size == nh
+    assert np.allclose(box, [2 * 3.89, 3 * 3.89, 4 * 3.89])
+    S = X.reshape(-1, 4)
+    # site = 4*cell + basis, cell = (iz*ny + iy)*nx + ix; Pd block first
+    assert np.allclose(S[1, :3], [0.5 * 3.89, 0.5 * 3.89, 0.0])
+    assert np.allclose(S[4, :3], [3.89, 0, 0])           # cell 1 = ix 1
+    assert np.allclose(S[8, :3], [0, 3.89, 0])           # cell 2 = iy 1
+    assert np.allclose(S[npd + 0, :3], [0.5 * 3.89, 0, 0])  # first H: octahedral (1/2,0,0)
+    assert np.allclose(S[:, 3], 0.1) and np.allclose(x, 0.2)
+
+
+def test_specimen_statistics_and_determinism():
+    X1, x1, *_ = A.lattice(8, seed=9, disp_std=0.03, sigma=(0.08, 0.15), x=(0.0, 0.6), x_clip=(0.0, 1.0))
+    X2, x2, *_ = A.lattice(8, seed=9, disp_std=0.03, sigma=(0.08, 0.15), x=(0.0, 0.6), x_clip=(0.0, 1.0))
+    assert np.array_equal(X1, X2) and np.array_equal(x1, x2)
+    X0, *_ = A.lattice(8, seed=9, disp_std=0.0)
+    disp = (X1 - X0).reshape(-1, 4)[:, :3].ravel()
+    assert abs(disp.std() - 0.03) < 0.001 and abs(disp.mean()) < 0.001
+    s = X1.reshape(-1, 4)[:, 3]
+    assert s.min() >= 0.08 and s.max() < 0.15 and abs(s.mean() - 0.115) < 0.002
+    assert x1.min() >= 0 and x1.max() < 0.6 and abs(x1.mean() - 0.3) < 0.01
+    X3, *_ = A.lattice(8, seed=10, disp_std=0.03)
+    assert not np.array_equal(X3, X1)
+
+
+def test_slab_occupancy():
+    X, x, box, npd, nh = A.lattice((16, 2, 2), x=(1e-3, 1e-3), slab_cells=8, slab_x=0.6)
+    ix = (np.arange(nh) // 4) % 16
+    assert np.all(x[ix < 8] == 0.6) and np.all(x[ix >= 8] == 1e-3)
+
+
+def test_descriptor_validation_happens_before_device_use():
+    d = A.FeDesc()
+    A.lib.a2c_fe_desc_default(C.byref(d))
+    assert d.rc == 5.35 and d.fd_eps == 1e-5 and d.max_neighbors == 256
+    ctx = C.c_void_p()
+    d.n_pd, d.n_h = 10, 10
+    d.periodic[0], d.box[0] = 1, 5.0  # < 2 rc
+    assert A.lib.a2c_fe_create(C.byref(d), C.byref(ctx)) == A.A2C_EINVAL
+    d.periodic[0] = 0
+    d.temperature = -1.0
+    assert A.lib.a2c_fe_create(C.byref(d), C.byref(ctx)) == A.A2C_EINVAL
+    d.temperature = 300.0
+    d.max_neighbors = 100000
+    assert A.lib.a2c_fe_create(C.byref(d), C.byref(ctx)) == A.A2C_EINVAL

@@ -1,0 +1,140 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C-ABI.
+
+Bars: neighbor sets bit-exact; FP64 densities and objective to 1e-12
+relative; FP64 gradient to 1e-10 of its max component; FD-derivative mode to
+the FD round-off of the reference's f_Pd form (1e-5 of ||G||)."""
+import numpy as np
+import pytest
+
+import a2c_amd as A
+import oracle as O
+from helpers import csr_rows, specimen
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (cells, periodic, displacement, sigma, x, T, seed)
+    (3, True, 0.02, (0.10, 0.10), (0.10, 0.10), 300.0, 1),
+    (4, True, 0.15, (0.08, 0.15), (0.0, 0.6), 500.0, 2),
+    (3, False, 0.05, (0.08, 0.15), (0.01, 0.6), 400.0, 3),
+    (6, True, 0.02, (0.10, 0.10), (0.10, 0.10), 300.0, 1),  # the plain 6^3 workload
+]
+
+
+def pair(case, deriv="analytic", precision="fp64"):
+    n, per, disp, sig, xr, T, seed = case
+    X, x, box, per3, npd, nh = specimen(n, seed=seed, disp=disp, sigma=sig, x=xr, periodic=per)
+    fe = A.FreeEntropy(npd, nh, box, per3, T=T, deriv=deriv, precision=precision)
+    fe.set_occupancy(x)
+    st = fe.findNeighbors(X)
+    orc = O.Oracle(npd, nh, box, per3, T=T, deriv=deriv, threads=8)
+    mx = orc.build_neighbors(X)
+    return fe, orc, X, x, st, mx
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_neighbor_sets_bit_exact(case):
+    fe, orc, X, x, st, mx = pair(case)
+    assert st["max_neib"] == mx
+    for k in ["PdPd", "PdH", "HPd", "HH", "HH1"]:
+        go, gi = fe.neighbors(k)
+        oo, oi = orc.neighbors(k)
+        assert np.array_equal(go, oo), k
+        assert np.array_equal(gi, oi), k
+    assert np.array_equal(fe.pdpdnum(), orc.pdpdnum())
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_density_fp64(case):
+    fe, orc, X, x, *_ = pair(case)
+    rg = fe.density(X)
+    ro = orc.density(X, x)
+    assert np.max(np.abs(rg - ro) / np.abs(ro)) <= 1e-12
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_objective_gradient_fp64(case):
+    fe, orc, X, x, *_ = pair(case)
+    fg, Gg = fe.formFunctionAndGradient(X)
+    fo, Go = orc.eval(X, x)
+    assert abs(fg - fo) <= 1e-12 * abs(fo)
+    assert np.max(np.abs(Gg - Go)) <= 1e-10 * np.max(np.abs(Go))
+
+
+@pytest.mark.parametrize("case", CASES[:2])
+def test_objective_gradient_fd_mode(case):
+    fe, orc, X, x, *_ = pair(case, deriv="fd")
+    fg, Gg = fe.formFunctionAndGradient(X)
+    fo, Go = orc.eval(X, x)
+    assert abs(fg - fo) <= 1e-12 * abs(fo)
+    assert np.linalg.norm(Gg - Go) <= 1e-5 * np.linalg.norm(Go)
+
+
+def test_eval_is_deterministic():
+    fe, orc, X, x, *_ = pair(CASES[1])
+    f1, G1 = fe.eval(X)
+    f2, G2 = fe.eval(X)
+    assert f1 == f2 and np.array_equal(G1, G2)
+
+
+def test_tao_adapter_and_device_entry_point():
+    fe, orc, X, x, *_ = pair(CASES[0])
+    f, G = fe.eval(X)
+    import ctypes as C
+    f2 = C.c_double()
+    G2 = np.zeros_like(G)
+    rc = A.lib.a2c_tao_objgrad(None, A._dptr(X), C.byref(f2), A._dptr(G2), fe._ctx)
+    assert rc == 0 and f2.value == f and np.array_equal(G2, G)
+    dX, dG, df = fe.dev_alloc(X.nbytes), fe.dev_alloc(X.nbytes), fe.dev_alloc(8)
+    try:
+        fe.upload(dX, X)
+        fe.eval_device(dX, dG, df)
+        fe.sync()
+        G3 = fe.download(np.zeros_like(G), dG)
+        f3 = fe.download(np.zeros(1), df)[0]
+        assert f3 == f and np.array_equal(G3, G)
+    finally:
+        for p in (dX, dG, df):
+            fe.dev_free(p)
+
+
+def test_no_hydrogen_occupancy_and_pd_only():
+    n, per, disp, sig, xr, T, seed = CASES[0]
+    X, x, box, per3, npd, nh = specimen(n, seed=seed, disp=disp)
+    x0 = np.zeros(nh)
+    fe = A.FreeEntropy(npd, nh, box, per3)
+    fe.set_occupancy(x0)
+    fe.findNeighbors(X)
+    orc = O.Oracle(npd, nh, box, per3)
+    orc.build_neighbors(X)
+    fg, Gg = fe.eval(X)
+    fo, Go = orc.eval(X, x0)
+    assert abs(fg - fo) <= 1e-12 * abs(fo)
+    assert np.max(np.abs(Gg - Go)) <= 1e-10 * np.max(np.abs(Go))
+    # Pd-only lattice (n_h = 0)
+    Xp = X[: 4 * npd]
+    fe2 = A.FreeEntropy(npd, 0, box, per3)
+    fe2.set_occupancy(np.zeros(0))
+    fe2.findNeighbors(Xp)
+    orc2 = O.Oracle(npd, 0, box, per3)
+    orc2.build_neighbors(Xp)
+    f2, G2 = fe2.eval(Xp)
+    fo2, Go2 = orc2.eval(Xp, np.zeros(0))
+    assert abs(f2 - fo2) <= 1e-12 * abs(fo2)
+
+
+def test_eval_before_neighbors_is_an_error():
+    X, x, box, per3, npd, nh = specimen(3)
+    fe = A.FreeEntropy(npd, nh, box, per3)
+    fe.set_occupancy(x)
+    with pytest.raises(A.A2CError) as e:
+        fe.eval(X)
+    assert e.value.code == A.A2C_ESTATE
+
+
+def test_neighbor_overflow_is_an_error():
+    X, x, box, per3, npd, nh = specimen(3)
+    fe = A.FreeEntropy(npd, nh, box, per3, max_neighbors=40)
+    fe.set_occupancy(x)
+    with pytest.raises(A.A2CError) as e:
+        fe.findNeighbors(X)
+    assert e.value.code == A.A2C_ENBR_OVERFLOW
