@@ -126,6 +126,7 @@ def _load():
         "a2c_lattice_desc_default": (None, [P(LatticeDesc)]),
         "a2c_lattice_fcc_pdh": (I, [P(LatticeDesc), dp, dp, dp]),
         "a2c_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+        "a2c_eam_tables": (I, [P(C.c_float), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -178,6 +179,15 @@ def lattice(n, a=3.89, seed=1, lattice_id=0, disp_std=0.02, sigma=(0.10, 0.10),
     if rc != 0:
         raise A2CError(rc, "bad lattice descriptor")
     return X, xo, box, nper, nper
+
+
+def eam_tables():
+    """FP32 cubic tables of the mixed path: array (5, 500, 4); table order
+    f_Pd, f_H, phi_Pd, phi_PdH, phi_H; interval k covers [1+0.01k, 1+0.01(k+1))."""
+    n = lib.a2c_eam_tables(None, 0)
+    out = np.zeros(n, dtype=np.float32)
+    lib.a2c_eam_tables(out.ctypes.data_as(C.POINTER(C.c_float)), n)
+    return out.reshape(5, 500, 4)
 
 
 def philox4x32_10(ctr, key):

@@ -101,7 +101,7 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   hipFree(c->dX); hipFree(c->dG); hipFree(c->dx); hipFree(c->drho); hipFree(c->dFp);
-  hipFree(c->demb); hipFree(c->dpart); hipFree(c->dF); hipFree(c->dpdpdnum); hipFree(c->dtab);
+  hipFree(c->demb); hipFree(c->dpart); hipFree(c->dF); hipFree(c->dpdpdnum); hipFree(c->dtab); hipFree(c->daux);
   for (int w = 0; w < 5; ++w) { hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt); }
   if (c->hX) hipHostFree(c->hX);
   if (c->hG) hipHostFree(c->hG);

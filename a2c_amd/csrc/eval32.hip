@@ -1,10 +1,369 @@
-// Mixed-precision objective + gradient (FP32 bulk, FP64 per-site and global
-// reductions).  Placeholder until the table kernels land.
+// Mixed-precision objective + gradient (Minimizer::formFunctionAndGradient,
+// minimizer.cpp:83-369): FP32 bulk quadrature, FP64 everywhere the error
+// would not average out.
+//
+//   FP64: positions, the pair vector d = q_i - q_j (then rounded to FP32, so
+//         the rounding is relative to |d| <= rc, not to the box size),
+//         per-site accumulation of the per-pair sums, rho, F(rho), F'(rho),
+//         the per-site objective share and the global reduction.
+//   FP32: the 12 quadrature points of a pair: |r|^2 = |d|^2 + a^2 +- 2 a d_c
+//         (each point displaces d along one axis by a = sqrt(6) sigma), 1/|r|
+//         by v_rsq_f32, and the EAM functions.
+//
+// The EAM functions are piecewise cubic Hermite polynomials on a uniform grid
+// in r (h = 0.01 A over [1, 6] A, switch radii 5.0, 5.05, 5.35 on grid
+// nodes), built on the host in 80-bit long double from the closed forms and
+// their analytic derivatives, stored as FP32 coefficients in the local
+// coordinate u in [0,1), and staged in LDS (5 tables x 500 intervals x 16 B =
+// 40 KB per workgroup).  Beyond 6 A every function is the exponential tail
+// of potential.cpp, evaluated directly.  Interpolation error is below the FP32
+// rounding of the table values (~1e-7 relative, mean bias ~1e-10).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "eam64.hpp"
 #include "fe_internal.hpp"
 
 namespace a2c {
-int build_tables(a2c_fe_ctx* c) { return set_err(c, A2C_ESTATE, "mixed precision not available yet"); }
-int launch_eval32(a2c_fe_ctx* c, const double*, double*, double*, bool, Timing*) {
-  return set_err(c, A2C_ESTATE, "mixed precision not available yet");
+namespace {
+
+constexpr int TB = 256;
+constexpr int NT = 500;               // intervals per table
+constexpr float R0 = 1.0f, R1 = 6.0f, INVH = 100.0f;
+constexpr double H64 = 0.01;
+constexpr double QW = 1.0 / 12.0;     // 0.5/dim, quadrature.cpp:39
+constexpr double SQ6 = 2.449489742783178098;
+constexpr float SQ6F = 2.449489742783178098f;
+enum { T_FPD = 0, T_FH = 1, T_PHPD = 2, T_PHPDH = 3, T_PHH = 4, NTAB = 5 };
+
+// Exponential tails beyond the switch radii: funRc * exp(k (r - rs)).
+__constant__ float TAIL[NTAB][3] = {
+    {0.001510569378297f, (float)(-0.077976566126292 / 0.001510569378297), 5.0f},
+    {0.009986990526578f, (float)(-0.013075667086408 / 0.009986990526578), 5.35f},
+    {-0.016958780164980f, (float)(0.089019679316328 / -0.016958780164980), 5.05f},
+    {-3.353001113951400e-04f, (float)(7.147062196813336e-04 / -3.353001113951400e-04), 5.35f},
+    {-0.003702722457147f, (float)(0.005465945471933 / -0.003702722457147), 5.35f}};
+
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// value and derivative of table w at distance r
+__device__ __forceinline__ void tab_vd(const float4* __restrict__ tab, int w, float r, float& v, float& d) {
+  if (r < R1) {
+    const float xx = fmaxf((r - R0) * INVH, 0.0f);
+    const int k = min((int)xx, NT - 1);
+    const float u = xx - (float)k;
+    const float4 c = tab[w * NT + k];
+    v = fmaf(u, fmaf(u, fmaf(u, c.w, c.z), c.y), c.x);
+    d = fmaf(u, fmaf(u, 3.0f * c.w, 2.0f * c.z), c.y) * INVH;
+  } else {
+    v = TAIL[w][0] * __expf(TAIL[w][1] * (r - TAIL[w][2]));
+    d = TAIL[w][1] * v;
+  }
 }
+__device__ __forceinline__ float tab_v(const float4* __restrict__ tab, int w, float r) {
+  if (r < R1) {
+    const float xx = fmaxf((r - R0) * INVH, 0.0f);
+    const int k = min((int)xx, NT - 1);
+    const float u = xx - (float)k;
+    const float4 c = tab[w * NT + k];
+    return fmaf(u, fmaf(u, fmaf(u, c.w, c.z), c.y), c.x);
+  }
+  return TAIL[w][0] * __expf(TAIL[w][1] * (r - TAIL[w][2]));
+}
+
+struct Site {
+  double q[3];
+  double s;
+};
+__device__ __forceinline__ Site load_site(const double* __restrict__ X, int s) {
+  const double4 v = *reinterpret_cast<const double4*>(X + 4 * (size_t)s);
+  return {{v.x, v.y, v.z}, v.w};
+}
+// d = q_i - image(q_j) in FP64, rounded to FP32
+__device__ __forceinline__ void pair_vec32(const DevSys& S, const Site& a, const Site& b, float d[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double qj = b.q[c];
+    if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) / S.box[c]);
+    d[c] = (float)(a.q[c] - qj);
+  }
+}
+
+__device__ __forceinline__ void stage_tables(float4* lds, const float4* __restrict__ g, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- density
+// Sum of the table-w density over the 12 points of one pair.
+__device__ __forceinline__ float pair_density(const float4* tab, int w, const float d[3], float ai, float aj) {
+  const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float bi = D2 + ai * ai, bj = D2 + aj * aj;
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float ti = 2.0f * ai * d[c], tj = 2.0f * aj * d[c];
+    const float r2[4] = {bi + ti, bi - ti, bj - tj, bj + tj};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = fmaxf(r2[q], 1e-12f);
+      acc += tab_v(tab, w, x * rsq(x));
+    }
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(TB) void k_density32(DevSys S, const double* __restrict__ X,
+                                                  const double* __restrict__ xo, const float4* __restrict__ gtab,
+                                                  double* __restrict__ rho_out, float2* __restrict__ aux,
+                                                  double* __restrict__ emb) {
+  __shared__ float4 tab[2 * NT];  // f_Pd, f_H
+  stage_tables(tab, gtab, 2 * NT);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  const Ell LP = pd ? S.pdpd : S.hpd;
+  const Ell LH = pd ? S.pdh : S.hh;
+  const Site a = load_site(X, s);
+  const float ai = SQ6F * (float)a.s;
+  double rho = 0.0;
+  const int np = LP.cnt[row];
+  for (int t = 0; t < np; ++t) {
+    const int j = LP.idx[(size_t)t * LP.rows + row];
+    const Site b = load_site(X, j);
+    float d[3];
+    pair_vec32(S, a, b, d);
+    rho += (double)pair_density(tab, T_FPD, d, ai, SQ6F * (float)b.s);
+  }
+  const int nhn = LH.cnt[row];
+  for (int t = 0; t < nhn; ++t) {
+    const int h = LH.idx[(size_t)t * LH.rows + row];
+    const Site b = load_site(X, S.npd + h);
+    float d[3];
+    pair_vec32(S, a, b, d);
+    rho += xo[h] * (double)pair_density(tab, T_FH, d, ai, SQ6F * (float)b.s);
+  }
+  rho *= QW;
+  rho_out[s] = rho;
+  const double xs = pd ? 1.0 : xo[row];
+  const eam::VD F = pd ? eam::F_Pd_vd(rho) : eam::F_H_vd(rho);
+  aux[s] = make_float2((float)F.d, (float)xs);
+  emb[s] = xs * F.v;
+}
+
+// ---------------------------------------------------------------- gradient
+// Pair kinds: 0 Pd-Pd, 1 Pd-H (site Pd), 2 H-Pd (site H), 3 H-H.
+template <int KIND>
+__device__ __forceinline__ void point_coef(const float4* tab, float r, float Fi, float Fj, float& phi, float& c) {
+  float v, dv;
+  if (KIND == 0) {
+    tab_vd(tab, T_FPD, r, v, dv);
+    float dphi;
+    tab_vd(tab, T_PHPD, r, phi, dphi);
+    c = fmaf(Fi + Fj, dv, dphi);
+  } else if (KIND == 3) {
+    tab_vd(tab, T_FH, r, v, dv);
+    float dphi;
+    tab_vd(tab, T_PHH, r, phi, dphi);
+    c = fmaf(Fi + Fj, dv, dphi);
+  } else {
+    float dpd, dh, dphi;
+    tab_vd(tab, T_FPD, r, v, dpd);
+    tab_vd(tab, T_FH, r, v, dh);
+    tab_vd(tab, T_PHPDH, r, phi, dphi);
+    c = (KIND == 1) ? fmaf(Fi, dh, fmaf(Fj, dpd, dphi)) : fmaf(Fi, dpd, fmaf(Fj, dh, dphi));
+  }
+}
+
+struct PairSums {
+  float pe;     // sum of phi over the 12 points
+  float g[3];   // sum of (c/|r|) r
+  float gs;     // sum over site-i points of (c/|r|) r . xi1 / sqrt(3)
+};
+
+template <int KIND>
+__device__ __forceinline__ PairSums pair_grad(const float4* tab, const float d[3], float ai, float aj, float Fi,
+                                              float Fj) {
+  const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float bi = D2 + ai * ai, bj = D2 + aj * aj;
+  float su = 0.0f, ui = 0.0f, pe = 0.0f, ti[3], tj[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float t1 = 2.0f * ai * d[c], t2 = 2.0f * aj * d[c];
+    // site-i points: r = d +- ai e_c ; site-j points: r = d -+ aj e_c
+    const float r2[4] = {bi + t1, bi - t1, bj - t2, bj + t2};
+    float u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = fmaxf(r2[q], 1e-12f);
+      const float rinv = rsq(x);
+      float phi, cc;
+      point_coef<KIND>(tab, x * rinv, Fi, Fj, phi, cc);
+      pe += phi;
+      u[q] = cc * rinv;
+    }
+    ti[c] = u[0] - u[1];
+    tj[c] = u[2] - u[3];
+    ui += u[0] + u[1];
+    su += (u[0] + u[1]) + (u[2] + u[3]);
+  }
+  PairSums p;
+  p.pe = pe;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) p.g[c] = fmaf(d[c], su, ai * ti[c] - aj * tj[c]);
+  p.gs = fmaf(d[0], ti[0], fmaf(d[1], ti[1], fmaf(d[2], ti[2], ai * ui)));
+  return p;
+}
+
+__global__ __launch_bounds__(TB) void k_grad32(DevSys S, const double* __restrict__ X, const double* __restrict__ xo,
+                                               const float4* __restrict__ gtab, const float2* __restrict__ aux,
+                                               const double* __restrict__ emb,
+                                               double* __restrict__ G, double* __restrict__ part) {
+  __shared__ float4 tab[NTAB * NT];
+  stage_tables(tab, gtab, NTAB * NT);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = S.npd + S.nh;
+  double fs = 0.0;
+  if (s < N) {
+    const bool pd = s < S.npd;
+    const int row = pd ? s : s - S.npd;
+    const Ell LP = pd ? S.pdpd : S.hpd;
+    const Ell LH = pd ? S.pdh : S.hh;
+    const Site a = load_site(X, s);
+    const float ai = SQ6F * (float)a.s;
+    const float Fi = aux[s].x;
+    const double xi = pd ? 1.0 : xo[row];
+    double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, gs = 0.0;
+    const int np = LP.cnt[row];
+    for (int t = 0; t < np; ++t) {
+      const int j = LP.idx[(size_t)t * LP.rows + row];
+      const Site b = load_site(X, j);
+      const float2 nb = aux[j];
+      float d[3];
+      pair_vec32(S, a, b, d);
+      const float aj = SQ6F * (float)b.s;
+      const PairSums p = pd ? pair_grad<0>(tab, d, ai, aj, Fi, nb.x) : pair_grad<2>(tab, d, ai, aj, Fi, nb.x);
+      e += p.pe; g0 += p.g[0]; g1 += p.g[1]; g2 += p.g[2]; gs += p.gs;
+    }
+    const int nhn = LH.cnt[row];
+    for (int t = 0; t < nhn; ++t) {
+      const int h = LH.idx[(size_t)t * LH.rows + row];
+      const Site b = load_site(X, S.npd + h);
+      const float2 nb = aux[S.npd + h];
+      float d[3];
+      pair_vec32(S, a, b, d);
+      const float aj = SQ6F * (float)b.s;
+      const PairSums p = pd ? pair_grad<1>(tab, d, ai, aj, Fi, nb.x) : pair_grad<3>(tab, d, ai, aj, Fi, nb.x);
+      const double w = xo[h];  // occupancy of the H neighbor
+      e += w * p.pe; g0 += w * p.g[0]; g1 += w * p.g[1]; g2 += w * p.g[2]; gs += w * p.gs;
+    }
+    const double T = S.T;
+    const double V = emb[s] + 0.5 * QW * xi * e;
+    const double sc = QW * xi / T;
+    G[4 * (size_t)s + 0] = sc * g0;
+    G[4 * (size_t)s + 1] = sc * g1;
+    G[4 * (size_t)s + 2] = sc * g2;
+    G[4 * (size_t)s + 3] = SQ6 * sc * gs - 3.0 * KB * xi / a.s;
+    fs = V / T - 3.0 * KB * xi * log(a.s);
+  }
+  // block reduction in the (now free) table memory
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(tab);
+  red[threadIdx.x] = fs;
+  __syncthreads();
+#pragma unroll
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_final_sum32(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) acc += part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// ---------------------------------------------------------------- tables
+template <class Fn>
+void fill_table(float4* out, Fn g) {
+  const long double h = H64, eps = 1e-13L;
+  for (int i = 0; i < NT; ++i) {
+    const long double ra = (long double)R0 + i * h + eps, rb = (long double)R0 + (i + 1) * h - eps;
+    const eam::VDT<long double> A = g(ra), B = g(rb);
+    const long double c0 = A.v, c1 = h * A.d;
+    const long double c2 = 3 * (B.v - A.v) - h * (2 * A.d + B.d);
+    const long double c3 = 2 * (A.v - B.v) + h * (A.d + B.d);
+    out[i] = make_float4((float)c0, (float)c1, (float)c2, (float)c3);
+  }
+}
+
+}  // namespace
+
+void host_tables(std::vector<float4>& t) {
+  using LD = long double;
+  t.assign((size_t)NTAB * NT, make_float4(0.f, 0.f, 0.f, 0.f));
+  fill_table(&t[T_FPD * NT], [](LD r) { return eam::f_Pd_vd<LD>(r); });
+  fill_table(&t[T_FH * NT], [](LD r) { return eam::f_H_vd<LD>(r); });
+  fill_table(&t[T_PHPD * NT], [](LD r) { return eam::phi_Pd_vd<LD>(r, eam::f_Pd_vd<LD>(r)); });
+  fill_table(&t[T_PHPDH * NT], [](LD r) { return eam::phi_morse_vd<0, LD>(r); });
+  fill_table(&t[T_PHH * NT], [](LD r) { return eam::phi_morse_vd<1, LD>(r); });
+}
+
+int build_tables(a2c_fe_ctx* c) {
+  std::vector<float4> t;
+  host_tables(t);
+  if (!c->dtab) A2C_HIP(c, hipMalloc(&c->dtab, sizeof(float4) * t.size()));
+  if (!c->daux) A2C_HIP(c, hipMalloc(&c->daux, sizeof(float2) * std::max(c->N, 1)));
+  A2C_HIP(c, hipMemcpy(c->dtab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice));
+  return A2C_OK;
+}
+
+int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
+  if (!c->dtab) return set_err(c, A2C_ESTATE, "tables not built");
+  const int N = c->N;
+  const DevSys S = c->sys();
+  const int nb = (N + TB - 1) / TB;
+  const float4* tab = c->dtab;
+  if (timed) tm->mark(c->stream, "density32");
+  if (N > 0) {
+    hipLaunchKernelGGL(k_density32, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, tab, c->drho, c->daux, c->demb);
+    A2C_HIP(c, hipGetLastError());
+  }
+  if (timed) tm->mark(c->stream, "grad32");
+  if (N > 0) {
+    hipLaunchKernelGGL(k_grad32, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, tab, c->daux, c->demb, dG,
+                       c->dpart);
+    A2C_HIP(c, hipGetLastError());
+  }
+  if (timed) tm->mark(c->stream, "final_sum");
+  hipLaunchKernelGGL(k_final_sum32, dim3(1), dim3(1024), 0, c->stream, c->dpart, nb, dF);
+  A2C_HIP(c, hipGetLastError());
+  if (timed) tm->mark(c->stream, nullptr);
+  return A2C_OK;
+}
+
 }  // namespace a2c
+
+// Host-only: the FP32 tables of the mixed path (no GPU needed).  Layout: 5
+// tables (f_Pd, f_H, phi_Pd, phi_PdH, phi_H) x 500 intervals x {c0,c1,c2,c3},
+// interval k covers r in [1 + 0.01 k, 1 + 0.01 (k+1)), value
+// c0 + u(c1 + u(c2 + u c3)) with u = (r - 1)/0.01 - k.
+extern "C" int a2c_eam_tables(float* out, int32_t cap) {
+  std::vector<float4> t;
+  a2c::host_tables(t);
+  const int n = (int)t.size() * 4;
+  if (out && cap >= n) std::memcpy(out, t.data(), sizeof(float) * n);
+  return n;
+}

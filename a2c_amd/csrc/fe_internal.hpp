@@ -32,13 +32,6 @@ struct DevSys {
   Ell pdpd, pdh, hpd, hh, hh1;
 };
 
-// Tables of the mixed-precision path: piecewise cubic in r on a uniform grid.
-struct TabDesc {
-  float r0, inv_h;  // grid origin and inverse spacing
-  int32_t n;        // intervals
-  int32_t pad;
-};
-
 struct Timing;
 
 }  // namespace a2c
@@ -68,9 +61,8 @@ struct a2c_fe_ctx {
   a2c_nbr_stats stats{};
 
   // mixed-precision tables + FP32 mirrors
-  float* dtab = nullptr;     // coefficient tables
-  a2c::TabDesc tab{};
-  float4* dq32 = nullptr;    // unused slot kept for layout experiments
+  float4* dtab = nullptr;    // cubic coefficient tables (5 x 500 intervals)
+  float2* daux = nullptr;    // per site {F'(rho), x} in FP32
 
   // host staging (pinned)
   double* hX = nullptr;
@@ -104,10 +96,11 @@ int launch_eval64(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool 
 int launch_density64(a2c_fe_ctx* c, const double* dX);  // fills drho, dFp, demb
 int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm);
 int build_tables(a2c_fe_ctx* c);                        // eval32.hip
+int tables_download(a2c_fe_ctx* c, float* out);         // 5*500 float4
 int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h,
-                double* dgam_raw);                      // occupancy.hip
+                double gamma_bd, double* dgam, int32_t* dfull, double* dcorr);  // occupancy.hip
 int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb,
-                   const double* dgam, const int32_t* dfull, double* dxnew);
+                   const double* dgam, const int32_t* dfull, double* df, double* dxnew);
 
 // Event timing of individual kernels on the context stream: mark() records
 // the next preallocated event; durations are differences of consecutive marks.

@@ -1,0 +1,555 @@
+// Bounded limited-memory variable metric minimizer with device-resident FP64
+// state: replaces TaoSolve with TAOBLMVM (minimizer.cpp:535, :569) over the
+// objective of a2c_fe_eval, with the bounds of formVariableBounds
+// (minimizer.cpp:804-830: sigma >= 1e-6, q free).
+//
+// Algorithm (PETSc TAO is not vendored, so its exact BLMVM is not pinned;
+// this follows its published structure):
+//   * L-BFGS two-loop recursion over the last m (s, y) pairs, H0 = (s'y/y'y) I;
+//     with no pairs, a scaled steepest-descent step.
+//   * bound handling by an active set: direction components that would push
+//     a sigma at its bound further down are zeroed, and the line search step
+//     is capped where the first free sigma would reach its bound.
+//   * Moré-Thuente line search (ftol 1e-4, gtol 0.9, xtol 1e-10, 30 trials;
+//     Moré & Thuente, ACM TOMS 20 (1994) 286-307), initial step 1.
+//   * on line-search failure: drop the pairs and retry with steepest descent;
+//     a second failure stops with reason -6.
+//   * stop on ||P g||_2 <= gatol, ||P g||/|f| <= grtol, ||P g||/||P g0|| <=
+//     gttol, max_it or max_funcs (TAO's default tests).
+//   * mixed precision: evaluations use the context precision until
+//     ||P g|| <= mixed_gtol or a mixed-precision line search fails; from then
+//     on they are FP64 (the FP32 gradient has a noise floor near 1e-6
+//     relative, above typical gatol).
+// Dot products are fixed-shape block reductions, so a run is bitwise
+// reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fe_internal.hpp"
+
+namespace a2c {
+int eval_dispatch(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm);
+
+namespace {
+
+constexpr int TB = 256;
+constexpr int NBD = 512;  // blocks of the dot kernels (fixed: determinism)
+constexpr int MAXDOT = 4;
+
+__global__ void k_axpby(size_t n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                        double* __restrict__ z) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    z[i] = a * x[i] + b * y[i];
+}
+
+struct DotArgs {
+  const double* a[MAXDOT];
+  const double* b[MAXDOT];
+  int k;
+};
+
+__global__ __launch_bounds__(TB) void k_dots(size_t n, DotArgs d, double* __restrict__ part) {
+  __shared__ double red[MAXDOT][TB];
+  double acc[MAXDOT] = {0, 0, 0, 0};
+  for (size_t i = blockIdx.x * (size_t)TB + threadIdx.x; i < n; i += (size_t)NBD * TB)
+    for (int k = 0; k < d.k; ++k) acc[k] += d.a[k][i] * d.b[k][i];
+  for (int k = 0; k < MAXDOT; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < MAXDOT; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < MAXDOT) part[threadIdx.x * NBD + blockIdx.x] = red[threadIdx.x][0];
+}
+
+// out[k] = sum of part[k*NBD ...] in a fixed order; one block per k
+__global__ __launch_bounds__(NBD) void k_dots_final(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[NBD];
+  red[threadIdx.x] = part[blockIdx.x * NBD + threadIdx.x];
+  __syncthreads();
+  for (int w = NBD / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// sigma components (index % 4 == 3): projection, projected gradient, active
+// set of a direction
+__global__ void k_project(size_t nsite, double lb, double* __restrict__ x) {
+  const size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (s < nsite) x[4 * s + 3] = fmax(x[4 * s + 3], lb);
+}
+__global__ void k_projgrad(size_t nsite, double lb, const double* __restrict__ x, const double* __restrict__ g,
+                           double* __restrict__ gp) {
+  const size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (s >= nsite) return;
+  gp[4 * s] = g[4 * s];
+  gp[4 * s + 1] = g[4 * s + 1];
+  gp[4 * s + 2] = g[4 * s + 2];
+  const double gs = g[4 * s + 3];
+  gp[4 * s + 3] = (x[4 * s + 3] <= lb && gs > 0.0) ? 0.0 : gs;
+}
+__global__ void k_zero_active(size_t nsite, double lb, const double* __restrict__ x, double* __restrict__ d) {
+  const size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (s < nsite && x[4 * s + 3] <= lb && d[4 * s + 3] < 0.0) d[4 * s + 3] = 0.0;
+}
+// largest step before a free sigma reaches the bound (min over blocks)
+__global__ __launch_bounds__(TB) void k_stepmax(size_t nsite, double lb, const double* __restrict__ x,
+                                                const double* __restrict__ d, double* __restrict__ part) {
+  __shared__ double red[TB];
+  double m = 1e300;
+  for (size_t s = blockIdx.x * (size_t)TB + threadIdx.x; s < nsite; s += (size_t)NBD * TB) {
+    const double ds = d[4 * s + 3];
+    if (ds < 0.0) m = fmin(m, (x[4 * s + 3] - lb) / (-ds));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Moré-Thuente line search: the step-selection rule (dcstep) and the driver
+// (dcsrch) of the published algorithm, reverse-communication style.
+struct MoreThuente {
+  double ftol = 1e-4, gtol = 0.9, xtol = 1e-10, stpmin = 1e-20, stpmax = 1e20;
+  bool brackt = false;
+  int stage = 1;
+  double finit = 0, ginit = 0, gtest = 0, width = 0, width1 = 0;
+  double stx = 0, fx = 0, gx = 0, sty = 0, fy = 0, gy = 0, stmin = 0, stmax = 0;
+
+  static void cstep(double& stx, double& fx, double& dx, double& sty, double& fy, double& dy, double& stp,
+                    double fp, double dp, bool& brackt, double stpmin, double stpmax) {
+    const double sgnd = dp * (dx / std::fabs(dx));
+    double stpf;
+    if (fp > fx) {
+      const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+      const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
+      double gamma = s * std::sqrt((theta / s) * (theta / s) - (dx / s) * (dp / s));
+      if (stp < stx) gamma = -gamma;
+      const double p = (gamma - dx) + theta, q = ((gamma - dx) + gamma) + dp, r = p / q;
+      const double stpc = stx + r * (stp - stx);
+      const double stpq = stx + ((dx / ((fx - fp) / (stp - stx) + dx)) / 2.0) * (stp - stx);
+      stpf = (std::fabs(stpc - stx) < std::fabs(stpq - stx)) ? stpc : stpc + (stpq - stpc) / 2.0;
+      brackt = true;
+    } else if (sgnd < 0.0) {
+      const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+      const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
+      double gamma = s * std::sqrt((theta / s) * (theta / s) - (dx / s) * (dp / s));
+      if (stp > stx) gamma = -gamma;
+      const double p = (gamma - dp) + theta, q = ((gamma - dp) + gamma) + dx, r = p / q;
+      const double stpc = stp + r * (stx - stp);
+      const double stpq = stp + (dp / (dp - dx)) * (stx - stp);
+      stpf = (std::fabs(stpc - stp) > std::fabs(stpq - stp)) ? stpc : stpq;
+      brackt = true;
+    } else if (std::fabs(dp) < std::fabs(dx)) {
+      const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+      const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
+      double gamma = s * std::sqrt(std::max(0.0, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
+      if (stp > stx) gamma = -gamma;
+      const double p = (gamma - dp) + theta, q = (gamma + (dx - dp)) + gamma, r = p / q;
+      double stpc;
+      if (r < 0.0 && gamma != 0.0) stpc = stp + r * (stx - stp);
+      else if (stp > stx) stpc = stpmax;
+      else stpc = stpmin;
+      const double stpq = stp + (dp / (dp - dx)) * (stx - stp);
+      if (brackt) {
+        stpf = (std::fabs(stpc - stp) < std::fabs(stpq - stp)) ? stpc : stpq;
+        if (stp > stx) stpf = std::min(stp + 0.66 * (sty - stp), stpf);
+        else stpf = std::max(stp + 0.66 * (sty - stp), stpf);
+      } else {
+        stpf = (std::fabs(stpc - stp) > std::fabs(stpq - stp)) ? stpc : stpq;
+        stpf = std::min(stpmax, stpf);
+        stpf = std::max(stpmin, stpf);
+      }
+    } else {
+      if (brackt) {
+        const double theta = 3.0 * (fp - fy) / (sty - stp) + dy + dp;
+        const double s = std::max({std::fabs(theta), std::fabs(dy), std::fabs(dp)});
+        double gamma = s * std::sqrt((theta / s) * (theta / s) - (dy / s) * (dp / s));
+        if (stp > sty) gamma = -gamma;
+        const double p = (gamma - dp) + theta, q = ((gamma - dp) + gamma) + dy, r = p / q;
+        stpf = stp + r * (sty - stp);
+      } else if (stp > stx) {
+        stpf = stpmax;
+      } else {
+        stpf = stpmin;
+      }
+    }
+    if (fp > fx) {
+      sty = stp; fy = fp; dy = dp;
+    } else {
+      if (sgnd < 0.0) { sty = stx; fy = fx; dy = dx; }
+      stx = stp; fx = fp; dx = dp;
+    }
+    stp = stpf;
+  }
+
+  void start(double f, double g, double stp) {
+    brackt = false;
+    stage = 1;
+    finit = f;
+    ginit = g;
+    gtest = ftol * ginit;
+    width = stpmax - stpmin;
+    width1 = 2.0 * width;
+    stx = 0.0; fx = finit; gx = ginit;
+    sty = 0.0; fy = finit; gy = ginit;
+    stmin = 0.0;
+    stmax = stp + 4.0 * stp;
+  }
+
+  // returns 0: evaluate at the new stp; 1: converged; 2: warning (stop, use stp)
+  int iterate(double f, double g, double& stp) {
+    const double ftest = finit + stp * gtest;
+    if (stage == 1 && f <= ftest && g >= 0.0) stage = 2;
+    if (brackt && (stp <= stmin || stp >= stmax)) return 2;
+    if (brackt && stmax - stmin <= xtol * stmax) return 2;
+    if (stp == stpmax && f <= ftest && g <= gtest) return 2;
+    if (stp == stpmin && (f > ftest || g >= gtest)) return 2;
+    if (f <= ftest && std::fabs(g) <= gtol * (-ginit)) return 1;
+    if (stage == 1 && f <= fx && f > ftest) {
+      double fm = f - stp * gtest, fxm = fx - stx * gtest, fym = fy - sty * gtest;
+      double gm = g - gtest, gxm = gx - gtest, gym = gy - gtest;
+      cstep(stx, fxm, gxm, sty, fym, gym, stp, fm, gm, brackt, stmin, stmax);
+      fx = fxm + stx * gtest; fy = fym + sty * gtest;
+      gx = gxm + gtest; gy = gym + gtest;
+    } else {
+      cstep(stx, fx, gx, sty, fy, gy, stp, f, g, brackt, stmin, stmax);
+    }
+    if (brackt) {
+      if (std::fabs(sty - stx) >= 0.66 * width1) stp = stx + 0.5 * (sty - stx);
+      width1 = width;
+      width = std::fabs(sty - stx);
+    }
+    if (brackt) {
+      stmin = std::min(stx, sty);
+      stmax = std::max(stx, sty);
+    } else {
+      stmin = stp + 1.1 * (stp - stx);
+      stmax = stp + 4.0 * (stp - stx);
+    }
+    stp = std::max(stp, stpmin);
+    stp = std::min(stp, stpmax);
+    if ((brackt && (stp <= stmin || stp >= stmax)) || (brackt && stmax - stmin <= xtol * stmax)) stp = stx;
+    return 0;
+  }
+};
+
+struct Dev {
+  a2c_fe_ctx* c;
+  size_t n;      // 4N
+  size_t nsite;  // N
+  double* part = nullptr;  // MAXDOT*NBD
+  double* res = nullptr;   // MAXDOT (device)
+  double* hres = nullptr;  // MAXDOT + 1 (pinned host)
+  int grid() const { return (int)std::min<size_t>((n + TB - 1) / TB, 4096); }
+  int sgrid() const { return (int)((nsite + TB - 1) / TB); }
+
+  int axpby(double a, const double* x, double b, const double* y, double* z) {
+    if (n) hipLaunchKernelGGL(k_axpby, dim3(grid()), dim3(TB), 0, c->stream, n, a, x, b, y, z);
+    A2C_HIP(c, hipGetLastError());
+    return A2C_OK;
+  }
+  // queue up to MAXDOT dot products plus, optionally, a copy of the device
+  // objective; one synchronisation.
+  int dots(int k, const double* const* a, const double* const* b, double* out, const double* dF = nullptr,
+           double* fout = nullptr) {
+    DotArgs d{};
+    d.k = k;
+    for (int i = 0; i < k; ++i) { d.a[i] = a[i]; d.b[i] = b[i]; }
+    hipLaunchKernelGGL(k_dots, dim3(NBD), dim3(TB), 0, c->stream, n, d, part);
+    A2C_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_dots_final, dim3(k), dim3(NBD), 0, c->stream, part, res);
+    A2C_HIP(c, hipGetLastError());
+    A2C_HIP(c, hipMemcpyAsync(hres, res, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+    if (dF) A2C_HIP(c, hipMemcpyAsync(hres + MAXDOT, dF, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    A2C_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < k; ++i) out[i] = hres[i];
+    if (fout) *fout = hres[MAXDOT];
+    return A2C_OK;
+  }
+  int stepmax(double lb, const double* x, const double* d, double* out) {
+    hipLaunchKernelGGL(k_stepmax, dim3(NBD), dim3(TB), 0, c->stream, nsite, lb, x, d, part);
+    A2C_HIP(c, hipGetLastError());
+    std::vector<double> h(NBD);
+    A2C_HIP(c, hipMemcpyAsync(h.data(), part, sizeof(double) * NBD, hipMemcpyDeviceToHost, c->stream));
+    A2C_HIP(c, hipStreamSynchronize(c->stream));
+    double m = 1e300;
+    for (double v : h) m = std::min(m, v);
+    *out = m;
+    return A2C_OK;
+  }
+};
+
+}  // namespace
+
+int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, double* Xout, a2c_lmvm_report* rep) {
+  std::memset(rep, 0, sizeof(*rep));
+  const int m = std::max(1, std::min(o->m, 64));
+  const double lb = o->sigma_lb;
+  Dev D{c, 4 * (size_t)c->N, (size_t)c->N};
+  const size_t n = D.n;
+  const int prec0 = c->desc.precision;
+  // device vectors
+  std::vector<double*> bufs;
+  auto dalloc = [&](double** p) -> int {
+    A2C_HIP(c, hipMalloc(p, sizeof(double) * std::max<size_t>(n, 1)));
+    bufs.push_back(*p);
+    return A2C_OK;
+  };
+  struct Cleanup {
+    std::vector<double*>& b;
+    a2c_fe_ctx* c;
+    int prec;
+    double* part;
+    double* res;
+    double* hres;
+    ~Cleanup() {
+      (void)hipStreamSynchronize(c->stream);
+      for (double* p : b) (void)hipFree(p);
+      (void)hipFree(part);
+      (void)hipFree(res);
+      if (hres) (void)hipHostFree(hres);
+      c->desc.precision = prec;
+    }
+  } cleanup{bufs, c, prec0, nullptr, nullptr, nullptr};
+  A2C_HIP(c, hipMalloc(&D.part, sizeof(double) * MAXDOT * NBD));
+  cleanup.part = D.part;
+  A2C_HIP(c, hipMalloc(&D.res, sizeof(double) * MAXDOT));
+  cleanup.res = D.res;
+  A2C_HIP(c, hipHostMalloc(&D.hres, sizeof(double) * (MAXDOT + 1)));
+  cleanup.hres = D.hres;
+  double *x, *g, *gp, *d, *xo, *go, *q;
+  for (double** p : {&x, &g, &gp, &d, &xo, &go, &q})
+    if (int rc = dalloc(p)) return rc;
+  std::vector<double*> S(m), Y(m);
+  for (int i = 0; i < m; ++i) {
+    if (int rc = dalloc(&S[i])) return rc;
+    if (int rc = dalloc(&Y[i])) return rc;
+  }
+  std::vector<double> rho(m), alpha(m);
+  double* dF = c->dF;
+
+  A2C_HIP(c, hipMemcpyAsync(x, X0, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (c->N) hipLaunchKernelGGL(k_project, dim3(D.sgrid()), dim3(TB), 0, c->stream, D.nsite, lb, x);
+
+  int evals = 0, evals64 = 0;
+  bool mixed = (prec0 == A2C_PREC_MIXED);
+  double f = 0.0;
+  auto evaluate = [&](const double* xx, double* gg, const double* dvec, double* gd) -> int {
+    if (evals >= o->max_funcs) return A2C_ENOTCONV;
+    c->desc.precision = mixed ? A2C_PREC_MIXED : A2C_PREC_FP64;
+    if (int rc = eval_dispatch(c, xx, gg, dF, false, nullptr)) return rc;
+    ++evals;
+    if (!mixed) ++evals64;
+    double out[1] = {0.0};
+    if (dvec) {
+      const double* a[1] = {gg};
+      const double* b[1] = {dvec};
+      if (int rc = D.dots(1, a, b, out, dF, &f)) return rc;
+      *gd = out[0];
+    } else {
+      A2C_HIP(c, hipMemcpyAsync(D.hres + MAXDOT, dF, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      A2C_HIP(c, hipStreamSynchronize(c->stream));
+      f = D.hres[MAXDOT];
+    }
+    if (!std::isfinite(f)) return set_err(c, A2C_ENONFINITE, "non-finite objective in the line search");
+    return A2C_OK;
+  };
+  auto pgnorm = [&](double* out) -> int {
+    if (c->N) hipLaunchKernelGGL(k_projgrad, dim3(D.sgrid()), dim3(TB), 0, c->stream, D.nsite, lb, x, g, gp);
+    const double* a[1] = {gp};
+    double v[1];
+    if (int rc = D.dots(1, a, a, v)) return rc;
+    *out = std::sqrt(v[0]);
+    return A2C_OK;
+  };
+
+  if (int rc = evaluate(x, g, nullptr, nullptr)) return rc;
+  double gnorm = 0.0;
+  if (int rc = pgnorm(&gnorm)) return rc;
+  if (mixed && o->mixed_switch && gnorm <= o->mixed_gtol) {
+    mixed = false;
+    if (int rc = evaluate(x, g, nullptr, nullptr)) return rc;
+    if (int rc = pgnorm(&gnorm)) return rc;
+  }
+  const double gnorm0 = gnorm;
+  int npairs = 0, head = 0, iter = 0, reason = 0;
+  auto converged = [&]() -> int {
+    if (!std::isfinite(gnorm)) return -4;
+    if (gnorm <= o->gatol) return 1;
+    if (o->grtol > 0 && gnorm <= o->grtol * std::fabs(f)) return 2;
+    if (o->gttol > 0 && gnorm <= o->gttol * gnorm0) return 3;
+    return 0;
+  };
+  reason = converged();
+  while (reason == 0) {
+    if (iter >= o->max_it) { reason = -2; break; }
+    bool steepest = (npairs == 0);
+    double gd = 0.0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      // ---- direction d = -H g
+      if (!steepest) {
+        if (int rc = D.axpby(1.0, g, 0.0, g, q)) return rc;
+        for (int t = 0; t < npairs; ++t) {  // newest to oldest
+          const int i = (head - 1 - t + m) % m;
+          const double* a[1] = {S[i]};
+          const double* b[1] = {q};
+          double v[1];
+          if (int rc = D.dots(1, a, b, v)) return rc;
+          alpha[i] = rho[i] * v[0];
+          if (int rc = D.axpby(1.0, q, -alpha[i], Y[i], q)) return rc;
+        }
+        const int nw = (head - 1 + m) % m;
+        double sy_yy[2];
+        {
+          const double* a[2] = {S[nw], Y[nw]};
+          const double* b[2] = {Y[nw], Y[nw]};
+          if (int rc = D.dots(2, a, b, sy_yy)) return rc;
+        }
+        const double gam = sy_yy[0] / sy_yy[1];
+        if (int rc = D.axpby(gam, q, 0.0, q, q)) return rc;
+        for (int t = npairs - 1; t >= 0; --t) {  // oldest to newest
+          const int i = (head - 1 - t + m) % m;
+          const double* a[1] = {Y[i]};
+          const double* b[1] = {q};
+          double v[1];
+          if (int rc = D.dots(1, a, b, v)) return rc;
+          const double beta = rho[i] * v[0];
+          if (int rc = D.axpby(1.0, q, alpha[i] - beta, S[i], q)) return rc;
+        }
+        if (int rc = D.axpby(-1.0, q, 0.0, q, d)) return rc;
+      } else {
+        // scaled steepest descent: TAO's initial scale 2|f|/||g||^2, capped so
+        // that no coordinate moves by more than 0.05 A on the first trial
+        double gg[1];
+        const double* a[1] = {g};
+        if (int rc = D.dots(1, a, a, gg)) return rc;
+        double delta = (f != 0.0 ? 2.0 * std::fabs(f) : 2.0) / std::max(gg[0], 1e-300);
+        delta = std::min(delta, 0.05 / std::max(std::sqrt(gg[0]), 1e-300));
+        if (int rc = D.axpby(-delta, g, 0.0, g, d)) return rc;
+      }
+      if (c->N) hipLaunchKernelGGL(k_zero_active, dim3(D.sgrid()), dim3(TB), 0, c->stream, D.nsite, lb, x, d);
+      {
+        const double* a[1] = {g};
+        const double* b[1] = {d};
+        double v[1];
+        if (int rc = D.dots(1, a, b, v)) return rc;
+        gd = v[0];
+      }
+      if (gd < 0.0) break;
+      steepest = true;  // not a descent direction: reset
+      npairs = 0;
+    }
+    if (!(gd < 0.0)) { reason = -5; break; }  // no descent direction left
+
+    // ---- line search along d from x
+    double smax = 0.0;
+    if (int rc = D.stepmax(lb, x, d, &smax)) return rc;
+    const double fold = f;
+    if (int rc = D.axpby(1.0, x, 0.0, x, xo)) return rc;
+    if (int rc = D.axpby(1.0, g, 0.0, g, go)) return rc;
+    MoreThuente ls;
+    ls.stpmax = std::min(1e20, smax);
+    double stp = std::min(1.0, ls.stpmax);
+    ls.start(f, gd, stp);
+    int status = 0, nfev = 0;
+    double gdt = gd;
+    while (true) {
+      if (int rc = D.axpby(1.0, xo, stp, d, x)) return rc;
+      int rc = evaluate(x, g, d, &gdt);
+      if (rc == A2C_ENOTCONV) { status = -1; break; }
+      if (rc) return rc;
+      ++nfev;
+      status = ls.iterate(f, gdt, stp);
+      if (status != 0) break;
+      if (nfev >= 30) { status = 2; break; }
+    }
+    const bool decreased = (f < fold) || (status == 1);
+    if (status < 0 || !decreased) {
+      // restore and either leave mixed precision or drop the pairs
+      if (int rc2 = D.axpby(1.0, xo, 0.0, xo, x)) return rc2;
+      if (int rc2 = D.axpby(1.0, go, 0.0, go, g)) return rc2;
+      f = fold;
+      if (status == -1) { reason = -3; break; }  // max_funcs
+      if (mixed && o->mixed_switch) {
+        mixed = false;
+        if (int rc2 = evaluate(x, g, nullptr, nullptr)) return rc2;
+        if (int rc2 = pgnorm(&gnorm)) return rc2;
+        npairs = 0;
+        reason = converged();
+        continue;
+      }
+      if (npairs == 0) { reason = -6; break; }  // steepest descent failed too
+      npairs = 0;
+      continue;
+    }
+    // ---- update the pairs: s = x - xo, y = g - go
+    {
+      const int i = head;
+      if (int rc = D.axpby(1.0, x, -1.0, xo, S[i])) return rc;
+      if (int rc = D.axpby(1.0, g, -1.0, go, Y[i])) return rc;
+      const double* a[1] = {S[i]};
+      const double* b[1] = {Y[i]};
+      double sy[1];
+      if (int rc = D.dots(1, a, b, sy)) return rc;
+      if (sy[0] > 1e-300) {  // keep only pairs with positive curvature
+        rho[i] = 1.0 / sy[0];
+        head = (head + 1) % m;
+        npairs = std::min(npairs + 1, m);
+      }
+    }
+    if (int rc = pgnorm(&gnorm)) return rc;
+    ++iter;
+    if (mixed && o->mixed_switch && gnorm <= o->mixed_gtol) {
+      mixed = false;
+      if (int rc = evaluate(x, g, nullptr, nullptr)) return rc;
+      if (int rc = pgnorm(&gnorm)) return rc;
+    }
+    reason = converged();
+  }
+  A2C_HIP(c, hipMemcpyAsync(Xout, x, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  A2C_HIP(c, hipStreamSynchronize(c->stream));
+  rep->reason = reason;
+  rep->iterations = iter;
+  rep->evals = evals;
+  rep->evals_fp64 = evals64;
+  rep->f = f;
+  rep->gnorm = gnorm;
+  return A2C_OK;
+}
+
+}  // namespace a2c
+
+extern "C" void a2c_lmvm_opts_default(a2c_lmvm_opts* o) {
+  o->m = 5;            // -tao_lmm_vectors
+  o->max_it = 2000;    // -tao_max_it
+  o->max_funcs = 4000; // -tao_max_funcs
+  o->gatol = 1e-8;
+  o->grtol = 1e-8;
+  o->gttol = 0.0;
+  o->sigma_lb = 1e-6;  // formVariableBounds, minimizer.cpp:815,822
+  o->mixed_switch = 1;
+  o->mixed_gtol = 1e-5;
+}
+
+extern "C" int a2c_fe_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* opts, double* X_out,
+                               a2c_lmvm_report* report) {
+  if (!c) return A2C_EINVAL;
+  c->err.clear();
+  if (!X0 || !X_out || !opts || !report) return a2c::set_err(c, A2C_EINVAL, "null argument");
+  if (!c->have_nbr || !c->have_x) return a2c::set_err(c, A2C_ESTATE, "neighbors/occupancy not set");
+  if (opts->m < 1 || opts->max_it < 0 || opts->max_funcs < 1 || !(opts->sigma_lb >= 0))
+    return a2c::set_err(c, A2C_EINVAL, "bad options");
+  return a2c::lmvm_minimize(c, X0, opts, X_out, report);
+}

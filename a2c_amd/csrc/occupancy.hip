@@ -1,0 +1,199 @@
+// Occupancy kernels (FP64):
+//   chemical potential  Minimizer::calculateChemicalPotential
+//                       (minimizer.cpp:588-802)
+//   diffusion step      integrateDiffusion (integrate.cpp:9-87)
+//
+// Chemical potential of H site a (A.5 of the reference's algebra):
+//   S_a = F_H(rho_a) + 1/2 T cH0 (cH1 n^2 + 2 cH2 n X_a + cH3 X_a^2)
+//       + sum_{j in HPd(a)} [ sum_k w (F'_Pd(rho_j) f_H + phi_PdH) + P_j ]
+//       + sum_{b in HH(a)} x_b [ sum_k w (F'_H(rho_b) f_H + phi_H) + Q_b ]
+//   n = |HPd(a)|, X_a = sum_{b in HH(a)} x_b,
+//   P_j = T cP0 (cP2 |PdPd(j)| + cP3 sum_{l in PdH(j)} x_l),
+//   Q_b = T cH0 (cH2 |HPd(b)| + cH3 sum_{c in HH(b)} x_c).
+// The reference's double loop over HH(a) for the cH3 term (:671-675) is the
+// square X_a^2; P_j and Q_b are computed once per site (k_chem_pre) instead of
+// once per (site, neighbor).  gamma = ln(x/(1-x)) + S_a/(kB T) + 1.5 for
+// 0 < x < 1 (else 1.5), capped at gamma_bd with full_H = 1 when x > 0.9999
+// (:721-722, :793-800); full_H sites keep gamma.
+//
+// Diffusion: f_a = gamma_a - ln(x_a/(1-x_a)) once per site (k_diff_f), then
+// x_a += (rate x_deriv) dt with x_deriv = sum over HH1(a) of
+// x_b(1-x_a)e^{-(f_a-f_b)/2} - x_a(1-x_b)e^{(f_a-f_b)/2} (k_diff_step).  The
+// edge term is evaluated from either end with the same operands, so it is
+// exactly antisymmetric; the total changes only by the rounding of the site
+// sums (plus clamps and full_H reservoirs, as in the reference).
+#include <cmath>
+
+#include "eam64.hpp"
+#include "fe_internal.hpp"
+
+namespace a2c {
+namespace {
+
+constexpr int TB = 256;
+constexpr double QW = 1.0 / 12.0;
+constexpr double SQ6 = 2.449489742783178098;
+
+struct Site {
+  double q[3];
+  double s;
+};
+__device__ __forceinline__ Site load_site(const double* __restrict__ X, int s) {
+  const double4 v = *reinterpret_cast<const double4*>(X + 4 * (size_t)s);
+  return {{v.x, v.y, v.z}, v.w};
+}
+__device__ __forceinline__ void pair_vec(const DevSys& S, const Site& a, const Site& b, double d[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double qj = b.q[c];
+    if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) / S.box[c]);
+    d[c] = a.q[c] - qj;
+  }
+}
+
+// 12-point sums of f_H and of phi (phi_PdH or phi_H) for one pair.
+template <int HH>
+__device__ __forceinline__ void pair_fh_phi(const double d[3], double ai, double aj, double& sf, double& sp) {
+  sf = 0.0;
+  sp = 0.0;
+#pragma unroll
+  for (int p = 0; p < 12; ++p) {
+    const int c = (p >> 1) % 3;
+    const double sg = (p & 1) ? -1.0 : 1.0;
+    double r[3] = {d[0], d[1], d[2]};
+    r[c] += (p < 6) ? sg * ai : -sg * aj;
+    const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    sf += eam::f_H(rn);
+    sp += eam::phi_morse<HH>(rn);
+  }
+}
+
+// Per-site correction sums: P_j for Pd sites, Q_b for H sites.
+__global__ void k_chem_pre(DevSys S, const double* __restrict__ xo, double T, double cp0, double cp2, double cp3,
+                           double ch0, double ch2, double ch3, double* __restrict__ corr) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  const Ell LP = pd ? S.pdpd : S.hpd;
+  const Ell LH = pd ? S.pdh : S.hh;
+  double X = 0.0;
+  const int nh = LH.cnt[row];
+  for (int t = 0; t < nh; ++t) X += xo[LH.idx[(size_t)t * LH.rows + row]];
+  const double n = (double)LP.cnt[row];
+  corr[s] = pd ? T * cp0 * (cp2 * n + cp3 * X) : T * ch0 * (ch2 * n + ch3 * X);
+}
+
+__global__ __launch_bounds__(TB) void k_chem(DevSys S, const double* __restrict__ X, const double* __restrict__ xo,
+                                             const double* __restrict__ rho, const double* __restrict__ Fp,
+                                             const double* __restrict__ corr, double ch0, double ch1, double ch2,
+                                             double ch3, double gamma_bd, double* __restrict__ gamma,
+                                             int32_t* __restrict__ full) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= S.nh) return;
+  if (full[a] == 1) return;  // fully occupied reservoir: gamma untouched
+  const int s = S.npd + a;
+  const Site si = load_site(X, s);
+  const double ai = SQ6 * si.s;
+  const double T = S.T;
+  double sum = eam::F_H(rho[s]);
+  // correction of the site itself
+  const double n = (double)S.hpd.cnt[a];
+  double Xa = 0.0;
+  const int nhh = S.hh.cnt[a];
+  for (int t = 0; t < nhh; ++t) Xa += xo[S.hh.idx[(size_t)t * S.hh.rows + a]];
+  sum += 0.5 * T * ch0 * (ch1 * n * n + 2.0 * ch2 * n * Xa + ch3 * Xa * Xa);
+  const int npn = S.hpd.cnt[a];
+  for (int t = 0; t < npn; ++t) {  // H-Pd pairs (minimizer.cpp:679-698)
+    const int j = S.hpd.idx[(size_t)t * S.hpd.rows + a];
+    const Site b = load_site(X, j);
+    double d[3];
+    pair_vec(S, si, b, d);
+    double sf, sp;
+    pair_fh_phi<0>(d, ai, SQ6 * b.s, sf, sp);
+    sum += QW * (Fp[j] * sf + sp) + corr[j];
+  }
+  for (int t = 0; t < nhh; ++t) {  // H-H pairs (:701-719)
+    const int h = S.hh.idx[(size_t)t * S.hh.rows + a];
+    const Site b = load_site(X, S.npd + h);
+    double d[3];
+    pair_vec(S, si, b, d);
+    double sf, sp;
+    pair_fh_phi<1>(d, ai, SQ6 * b.s, sf, sp);
+    sum += xo[h] * (QW * (Fp[S.npd + h] * sf + sp) + corr[S.npd + h]);
+  }
+  const double x = xo[a];
+  const double raw = (x > 0.0 && x < 1.0) ? log(x / (1.0 - x)) + sum / (KB * T) : 0.0;
+  double g = raw + 1.5;
+  if (g > gamma_bd) {
+    g = gamma_bd;
+    if (x > 0.9999) full[a] = 1;
+  }
+  gamma[a] = g;
+}
+
+__global__ void k_diff_f(int nh, const double* __restrict__ gamma, const double* __restrict__ xo,
+                         double* __restrict__ f) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nh) return;
+  f[a] = gamma[a] - log(xo[a] / (1.0 - xo[a]));
+}
+
+__global__ void k_diff_step(Ell hh1, const double* __restrict__ xo, const double* __restrict__ f,
+                            const int32_t* __restrict__ full, double pref, double dt, double xLb, double xUb,
+                            double* __restrict__ xn) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= hh1.rows) return;
+  const double xa = xo[a];
+  if (full[a] == 1) {
+    xn[a] = xa;
+    return;
+  }
+  const double fa = f[a];
+  double xd = 0.0;
+  const int n = hh1.cnt[a];
+  for (int t = 0; t < n; ++t) {
+    const int b = hh1.idx[(size_t)t * hh1.rows + a];
+    const double xb = xo[b], df = fa - f[b];
+    xd += xb * (1.0 - xa) * exp(-df / 2.0) - xa * (1.0 - xb) * exp(df / 2.0);
+  }
+  double v = xa + pref * xd * dt;
+  if (v < xLb) v = xLb;
+  else if (v > xUb) v = xUb;
+  xn[a] = v;
+}
+
+}  // namespace
+
+int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h, double gamma_bd,
+                double* dgam, int32_t* dfull, double* dcorr) {
+  const DevSys S = c->sys();
+  const int N = c->N;
+  if (int rc = launch_density64(c, dX)) return rc;  // rho, F'(rho) at X
+  if (N > 0) {
+    hipLaunchKernelGGL(k_chem_pre, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, S, c->dx, S.T, coe_pd[0],
+                       coe_pd[2], coe_pd[3], coe_h[0], coe_h[2], coe_h[3], dcorr);
+    A2C_HIP(c, hipGetLastError());
+  }
+  if (S.nh > 0) {
+    hipLaunchKernelGGL(k_chem, dim3((S.nh + TB - 1) / TB), dim3(TB), 0, c->stream, S, dX, c->dx, c->drho, c->dFp,
+                       dcorr, coe_h[0], coe_h[1], coe_h[2], coe_h[3], gamma_bd, dgam, dfull);
+    A2C_HIP(c, hipGetLastError());
+  }
+  return A2C_OK;
+}
+
+int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb, const double* dgam,
+                   const int32_t* dfull, double* df, double* dxnew) {
+  const int nh = c->desc.n_h;
+  if (nh == 0) return A2C_OK;
+  hipLaunchKernelGGL(k_diff_f, dim3((nh + TB - 1) / TB), dim3(TB), 0, c->stream, nh, dgam, c->dx, df);
+  A2C_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_diff_step, dim3((nh + TB - 1) / TB), dim3(TB), 0, c->stream, c->ell[4], c->dx, df, dfull,
+                     prefactor, dt, xLb, xUb, dxnew);
+  A2C_HIP(c, hipGetLastError());
+  return A2C_OK;
+}
+
+}  // namespace a2c

@@ -173,6 +173,12 @@ typedef struct a2c_timing {
 int a2c_fe_time_eval(a2c_fe_ctx* ctx, const double* dX, double* dG, double* df,
                      int32_t iters, a2c_timing* out);
 
+/* Host-only (no GPU): the FP32 EAM tables of the mixed path, 5 tables
+ * (f_Pd, f_H, phi_Pd, phi_PdH, phi_H) x 500 intervals x 4 cubic coefficients;
+ * interval k covers r in [1 + 0.01k, 1 + 0.01(k+1)) A.  Returns the float count
+ * (10000); fills out when cap is large enough. */
+int a2c_eam_tables(float* out, int32_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,81 @@
+"""Relaxations: the device BLMVM (a2c_fe_minimize) against an independent CPU
+relaxation of the oracle objective (scipy L-BFGS-B, same bounds).  The TAO
+boundary itself is not vendored in the reference, so the optimizer's path is
+not compared -- the minimiser is: relaxed positions and widths must agree to
+1e-6 A after removing the uniform translation (an exact zero mode with
+periodic images)."""
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+import a2c_amd as A
+import oracle as O
+from helpers import specimen
+
+pytestmark = pytest.mark.gpu
+
+_REF = {}
+
+
+def oracle_relaxed(n, seed, T):
+    key = (n, seed, T)
+    if key not in _REF:
+        X, x, box, per, npd, nh = specimen(n, seed=seed, disp=0.02, sigma=(0.10, 0.10), x=(0.10, 0.10))
+        orc = O.Oracle(npd, nh, box, per, T=T, threads=8)
+        orc.build_neighbors(X)
+        bounds = [(None, None)] * 3 + [(1e-6, None)]
+        r = minimize(lambda v: orc.eval(v, x), X, jac=True, method="L-BFGS-B", bounds=bounds * (npd + nh),
+                     options={"maxiter": 20000, "maxfun": 40000, "gtol": 1e-12, "ftol": 1e-16, "maxcor": 20})
+        f, G = orc.eval(r.x, x)
+        _REF[key] = (X, x, box, per, npd, nh, r.x, f, np.linalg.norm(G))
+    return _REF[key]
+
+
+def compare(Xa, Xb):
+    a, b = Xa.reshape(-1, 4), Xb.reshape(-1, 4)
+    d = a[:, :3] - b[:, :3]
+    d -= d.mean(axis=0)
+    return np.abs(d).max(), np.abs(a[:, 3] - b[:, 3]).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_relaxed_state_matches_oracle(precision):
+    X, x, box, per, npd, nh, Xref, fref, gref = oracle_relaxed(3, 1, 300.0)
+    assert gref < 1e-8
+    fe = A.FreeEntropy(npd, nh, box, per, T=300.0, precision=precision)
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    Xr, rep = fe.minimizeFreeEntropy(X, gatol=1e-9, grtol=0.0)
+    assert rep["reason"] == 1, rep
+    dq, ds = compare(Xr, Xref)
+    assert dq <= 1e-6 and ds <= 1e-6, (dq, ds, rep)
+    assert abs(rep["f"] - fref) <= 1e-10 * abs(fref)
+    if precision == "mixed":
+        assert rep["evals_fp64"] < rep["evals"]  # the bulk ran in mixed precision
+
+
+def test_lmvm_report_and_limits():
+    X, x, box, per, npd, nh = specimen(3, seed=4, disp=0.03)[:6]
+    fe = A.FreeEntropy(npd, nh, box, per, T=300.0)
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    f0, G0 = fe.eval(X)
+    Xr, rep = fe.minimizeFreeEntropy(X, max_it=3)
+    assert rep["reason"] == -2 and rep["iterations"] == 3
+    assert rep["f"] < f0
+    f1, G1 = fe.eval(Xr)
+    assert f1 == pytest.approx(rep["f"], rel=1e-14)
+    Xr2, rep2 = fe.minimizeFreeEntropy(X, max_it=3)  # deterministic
+    assert np.array_equal(Xr, Xr2)
+
+
+def test_sigma_bound_respected():
+    X, x, box, per, npd, nh = specimen(3, seed=6, sigma=(0.10, 0.10))[:6]
+    X = X.copy()
+    X[3::4] = 1e-6  # every width starts at the bound
+    fe = A.FreeEntropy(npd, nh, box, per, T=300.0)
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    Xr, rep = fe.minimizeFreeEntropy(X, max_it=20)
+    assert np.all(Xr[3::4] >= 1e-6)
+    assert rep["f"] < fe.eval(X)[0]
