@@ -122,6 +122,7 @@ def _load():
         "a2c_dev_upload": (I, [VP, VP, VP, C.c_size_t]),
         "a2c_dev_download": (I, [VP, VP, VP, C.c_size_t]),
         "a2c_fe_sync": (I, [VP]),
+        "a2c_dev_copy": (I, [VP, VP, VP, C.c_size_t]),
         "a2c_fe_time_eval": (I, [VP, VP, VP, VP, C.c_int32, P(Timing)]),
         "a2c_lattice_desc_default": (None, [P(LatticeDesc)]),
         "a2c_lattice_fcc_pdh": (I, [P(LatticeDesc), dp, dp, dp]),
@@ -338,6 +339,9 @@ class FreeEntropy:
     def download(self, arr, dptr):
         self._check(lib.a2c_dev_download(self._ctx, arr.ctypes.data_as(C.c_void_p), C.c_void_p(dptr), arr.nbytes))
         return arr
+
+    def dev_copy(self, dst, src, nbytes):
+        self._check(lib.a2c_dev_copy(self._ctx, C.c_void_p(dst), C.c_void_p(src), nbytes))
 
     def eval_device(self, dX, dG, df):
         self._check(lib.a2c_fe_eval_device(self._ctx, C.c_void_p(dX), C.c_void_p(dG), C.c_void_p(df)))

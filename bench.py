@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--precision", choices=["mixed", "fp64"], default=os.environ.get("A2C_BENCH_PREC", "mixed"))
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--calib", action="store_true",
+                   help="also run a 256 MiB copy kernel (counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
 
 
@@ -110,6 +112,12 @@ def main():
     flop_grad = K * sum(c * u for c, u in zip(FLOP_GRAD, U))
     flop_dens = K * sum(c * u for c, u in zip(FLOP_DENS, U))
     log("neighbors: %.1f ms, max %d, totals %s" % (nbr_ms, st["max_neib"], st["total"]))
+    if args.calib:
+        nb = 256 << 20
+        a, b = fe.dev_alloc(nb), fe.dev_alloc(nb)
+        fe.dev_copy(b, a, nb)
+        fe.dev_free(a)
+        fe.dev_free(b)
 
     # FP64 comparator (always measured; the reference answer for the error)
     tm64, wall64, f64, G64 = run_gpu(fe, X, max(3, args.steps // 10), 2)

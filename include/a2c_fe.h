@@ -161,6 +161,10 @@ int a2c_dev_download(a2c_fe_ctx* ctx, void* dst, const void* src, size_t bytes);
 int a2c_fe_sync(a2c_fe_ctx* ctx);
 
 /* ---- measurement ------------------------------------------------------- */
+/* Plain device-to-device copy kernel (16-byte aligned), synchronous; used to
+ * calibrate HBM byte counters against a transfer of known size. */
+int a2c_dev_copy(a2c_fe_ctx* ctx, void* dst, const void* src, size_t bytes);
+
 typedef struct a2c_timing {
   int32_t n_kernels;
   char names[8][32];    /* kernel names in launch order */

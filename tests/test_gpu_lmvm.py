@@ -1,9 +1,14 @@
 """Relaxations: the device BLMVM (a2c_fe_minimize) against an independent CPU
-relaxation of the oracle objective (scipy L-BFGS-B, same bounds).  The TAO
-boundary itself is not vendored in the reference, so the optimizer's path is
-not compared -- the minimiser is: relaxed positions and widths must agree to
-1e-6 A after removing the uniform translation (an exact zero mode with
-periodic images)."""
+relaxation of the oracle objective.  The TAO boundary itself is not vendored
+in the reference, so the optimizer's path is not compared -- the minimiser
+is: relaxed positions and widths must agree to 1e-6 A after removing the
+uniform translation (an exact zero mode with periodic images).
+
+The CPU reference is scipy's L-BFGS-B followed by Barzilai-Borwein steps
+(gradient only): the FP64 objective of the reference's polynomial forms has
+~2e-13 relative rounding noise, which stalls any sufficient-decrease line
+search near ||g|| ~ 5e-8, still ~1e-5 A from the minimiser; the gradient's
+noise is ~1e-12, so the gradient-only polish reaches ||g|| < 1e-10."""
 import numpy as np
 import pytest
 from scipy.optimize import minimize
@@ -26,8 +31,18 @@ def oracle_relaxed(n, seed, T):
         bounds = [(None, None)] * 3 + [(1e-6, None)]
         r = minimize(lambda v: orc.eval(v, x), X, jac=True, method="L-BFGS-B", bounds=bounds * (npd + nh),
                      options={"maxiter": 20000, "maxfun": 40000, "gtol": 1e-12, "ftol": 1e-16, "maxcor": 20})
-        f, G = orc.eval(r.x, x)
-        _REF[key] = (X, x, box, per, npd, nh, r.x, f, np.linalg.norm(G))
+        xk, (f, gk) = r.x.copy(), orc.eval(r.x, x)
+        step = 1e-3
+        for _ in range(5000):  # Barzilai-Borwein polish
+            if np.linalg.norm(gk) < 1e-10:
+                break
+            xn = xk - step * gk
+            xn[3::4] = np.maximum(xn[3::4], 1e-6)
+            fn, gn = orc.eval(xn, x)
+            s_, y_ = xn - xk, gn - gk
+            step = (s_ @ s_) / (s_ @ y_) if s_ @ y_ > 0 else 1e-3
+            xk, f, gk = xn, fn, gn
+        _REF[key] = (X, x, box, per, npd, nh, xk, f, np.linalg.norm(gk))
     return _REF[key]
 
 
@@ -41,7 +56,7 @@ def compare(Xa, Xb):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_relaxed_state_matches_oracle(precision):
     X, x, box, per, npd, nh, Xref, fref, gref = oracle_relaxed(3, 1, 300.0)
-    assert gref < 1e-8
+    assert gref < 1e-10
     fe = A.FreeEntropy(npd, nh, box, per, T=300.0, precision=precision)
     fe.set_occupancy(x)
     fe.findNeighbors(X)
