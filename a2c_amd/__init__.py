@@ -81,6 +81,13 @@ class Timing(C.Structure):
     ]
 
 
+class BatchDesc(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("precision", C.c_int32), ("deriv_mode", C.c_int32), ("fd_eps", C.c_double),
+        ("max_lattices", C.c_int32), ("hbm_budget_bytes", C.c_size_t), ("max_neighbors", C.c_int32),
+    ]
+
+
 class LatticeDesc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("a", C.c_double),
@@ -128,6 +135,20 @@ def _load():
         "a2c_lattice_fcc_pdh": (I, [P(LatticeDesc), dp, dp, dp]),
         "a2c_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "a2c_eam_tables": (I, [P(C.c_float), C.c_int32]),
+        "a2c_batch_desc_default": (None, [P(BatchDesc)]),
+        "a2c_batch_create": (I, [P(BatchDesc), P(VP)]),
+        "a2c_batch_destroy": (None, [VP]),
+        "a2c_batch_last_error": (C.c_char_p, [VP]),
+        "a2c_batch_footprint": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
+        "a2c_batch_mem_used": (C.c_size_t, [VP]),
+        "a2c_batch_add": (I, [VP, P(FeDesc), dp, dp, P(C.c_int32)]),
+        "a2c_batch_remove": (I, [VP, C.c_int32]),
+        "a2c_batch_set_state": (I, [VP, C.c_int32, dp]),
+        "a2c_batch_get_state": (I, [VP, C.c_int32, dp, dp]),
+        "a2c_batch_eval": (I, [VP, C.c_int32, ip, C.c_int32, dp]),
+        "a2c_batch_lmvm_start": (I, [VP, C.c_int32, P(LmvmOpts)]),
+        "a2c_batch_lmvm_step": (I, [VP, C.c_int32, P(C.c_int32)]),
+        "a2c_batch_lmvm_report": (I, [VP, C.c_int32, P(LmvmReport)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -357,12 +378,124 @@ class FreeEntropy:
         return {"kernels_ms": kern, "total_ms": t.total_ms}
 
 
+def lmvm_opts(**kw):
+    o = LmvmOpts()
+    lib.a2c_lmvm_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def fe_desc(n_pd, n_h, box=(0.0, 0.0, 0.0), periodic=(0, 0, 0), rc=5.35, a_pd=3.89, T=300.0, max_neighbors=0):
+    d = FeDesc()
+    lib.a2c_fe_desc_default(C.byref(d))
+    d.n_pd, d.n_h = n_pd, n_h
+    for c in range(3):
+        d.box[c] = float(box[c])
+        d.periodic[c] = int(bool(periodic[c]))
+    d.rc, d.a_pd, d.temperature, d.max_neighbors = rc, a_pd, T, max_neighbors
+    return d
+
+
+class Batch:
+    """Many lattices on one GPU (include/a2c_batch.h): fused evaluation and a
+    lockstep LMVM with admission/retirement between iterations."""
+
+    def __init__(self, precision="mixed", max_lattices=64, hbm_budget_bytes=200 << 30, device=0,
+                 deriv="analytic", eps=1e-5):
+        d = BatchDesc()
+        lib.a2c_batch_desc_default(C.byref(d))
+        d.device, d.precision, d.deriv_mode, d.fd_eps = device, PREC[precision], DERIV[deriv], eps
+        d.max_lattices, d.hbm_budget_bytes = max_lattices, hbm_budget_bytes
+        self._b = C.c_void_p()
+        rc = lib.a2c_batch_create(C.byref(d), C.byref(self._b))
+        if rc != 0:
+            raise A2CError(rc, "a2c_batch_create failed")
+        self.sizes = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib.a2c_batch_last_error(self._b)
+            raise A2CError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self._b:
+            lib.a2c_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def footprint(n_pd, n_h, m=5):
+        return lib.a2c_batch_footprint(n_pd, n_h, m)
+
+    def mem_used(self):
+        return lib.a2c_batch_mem_used(self._b)
+
+    def add(self, X, x, n_pd, n_h, box=(0, 0, 0), periodic=(0, 0, 0), T=300.0, rc=5.35, a_pd=3.89):
+        d = fe_desc(n_pd, n_h, box, periodic, rc, a_pd, T)
+        X = _f64(X, 4 * (n_pd + n_h))
+        x = _f64(x, n_h)
+        slot = C.c_int32()
+        self._check(lib.a2c_batch_add(self._b, C.byref(d), _dptr(X), _dptr(x), C.byref(slot)))
+        self.sizes[slot.value] = n_pd + n_h
+        return slot.value
+
+    def remove(self, slot):
+        self._check(lib.a2c_batch_remove(self._b, slot))
+        self.sizes.pop(slot, None)
+
+    def set_state(self, slot, X):
+        X = _f64(X, 4 * self.sizes[slot])
+        self._check(lib.a2c_batch_set_state(self._b, slot, _dptr(X)))
+
+    def get_state(self, slot):
+        n = 4 * self.sizes[slot]
+        X, G = np.zeros(n), np.zeros(n)
+        self._check(lib.a2c_batch_get_state(self._b, slot, _dptr(X), _dptr(G)))
+        return X, G
+
+    def eval(self, slots, precision="mixed"):
+        ids = np.ascontiguousarray(slots, dtype=np.int32)
+        f = np.zeros(max(len(ids), 1))
+        self._check(lib.a2c_batch_eval(self._b, len(ids), _iptr(ids), PREC[precision], _dptr(f)))
+        return f[: len(ids)]
+
+    def start(self, slot, **opts):
+        o = lmvm_opts(**opts)
+        self._check(lib.a2c_batch_lmvm_start(self._b, slot, C.byref(o)))
+
+    def step(self, max_iters=1):
+        n = C.c_int32()
+        self._check(lib.a2c_batch_lmvm_step(self._b, max_iters, C.byref(n)))
+        return n.value
+
+    def report(self, slot):
+        r = LmvmReport()
+        self._check(lib.a2c_batch_lmvm_report(self._b, slot, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in LmvmReport._fields_}
+
+    def run(self, max_iters=1 << 20):
+        while self.step(max_iters) > 0:
+            pass
+
+
 def exported_symbols():
     """Symbols include/*.h declares (checked by the CPU test suite)."""
     import re
     root = os.path.dirname(_HERE)
     names = []
-    for h in ("a2c_fe.h", "a2c_lattice.h"):
+    for h in ("a2c_fe.h", "a2c_lattice.h", "a2c_batch.h"):
         txt = open(os.path.join(root, "include", h)).read()
         names += re.findall(r"\b(a2c_\w+)\s*\(", txt)
     return sorted(set(names))

@@ -1,0 +1,980 @@
+// Batched engine: many lattices of uneven size evaluated by one fused launch,
+// and a lockstep bounded LMVM over all of them.
+//
+// Layout.  Every lattice (slot) keeps its own device buffers (state,
+// gradient, neighbor lists, per-site work arrays) in an a2c_fe_ctx.  A launch
+// over a set of slots uploads one LatDev per slot and a block -> slot map; the
+// site blocks of a slot always start at its site 0 and its reductions
+// (objective, dot products) have a fixed shape that depends only on its own
+// size, so every slot's result is bitwise the same alone or in any mix.
+//
+// Optimizer.  The single-lattice minimizer (a2c_fe_minimize) runs on this
+// code with a batch of one, so it is the same algorithm: L-BFGS two-loop
+// recursion (m pairs, H0 = s'y/y'y), active-set handling of the sigma >= lb
+// bound, Moré-Thuente line search with approximate Wolfe acceptance in FP64,
+// steepest-descent restart on failure, TAO's gatol/grtol/gttol tests, and the
+// mixed -> FP64 switch.  In a batch every step is one launch over all slots
+// that need it (segmented kernels), and each scalar decision is taken per slot
+// on the host after one synchronisation per round.  Slots can be added and
+// started between iterations (continuous batching).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "a2c_batch.h"
+#include "fe_internal.hpp"
+#include "linesearch.hpp"
+
+namespace a2c {
+namespace {
+
+constexpr int TB = SITE_BLOCK;
+
+// ---------------------------------------------------------------- kernels
+struct SegAx {  // z = a x + b y on 4N doubles
+  double* z;
+  const double* x;
+  const double* y;
+  double a, b;
+  int32_t nsite, blk0;
+};
+struct SegDot {  // out[0] = a1 . b1, out[1] = a2 . b2 (a2 may be null)
+  const double* a1;
+  const double* b1;
+  const double* a2;
+  const double* b2;
+  double* out;
+  int32_t nsite, blk0;
+};
+struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero active d, 3 step max
+  double* z;        // x (0), gp (1), d (2)
+  const double* x;  // x (1, 2, 3)
+  const double* y;  // g (1), d (3)
+  double lb;
+  double* out;      // step max (3)
+  int32_t nsite, blk0;
+};
+
+__global__ __launch_bounds__(TB) void k_seg_axpby(const SegAx* __restrict__ ops, const int32_t* __restrict__ b2l) {
+  const SegAx& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  if (s >= o.nsite) return;
+  const size_t i = 4 * (size_t)s;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) o.z[i + c] = o.a * o.x[i + c] + o.b * o.y[i + c];
+}
+
+__global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                                double* __restrict__ part, int nblk) {
+  __shared__ double r1[TB], r2[TB];
+  const SegDot& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  double v1 = 0.0, v2 = 0.0;
+  if (s < o.nsite) {
+    const size_t i = 4 * (size_t)s;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v1 += o.a1[i + c] * o.b1[i + c];
+    if (o.a2) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v2 += o.a2[i + c] * o.b2[i + c];
+    }
+  }
+  r1[threadIdx.x] = v1;
+  r2[threadIdx.x] = v2;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = r1[0];
+    part[nblk + blockIdx.x] = r2[0];
+  }
+}
+
+// one block per op: fixed-order sum of the op's block partials
+__global__ __launch_bounds__(1024) void k_seg_dot_final(const SegDot* __restrict__ ops, const double* __restrict__ part,
+                                                        int nblk) {
+  __shared__ double r1[1024], r2[1024];
+  const SegDot& o = ops[blockIdx.x];
+  const int n = site_blocks(o.nsite);
+  double v1 = 0.0, v2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    v1 += part[o.blk0 + i];
+    v2 += part[nblk + o.blk0 + i];
+  }
+  r1[threadIdx.x] = v1;
+  r2[threadIdx.x] = v2;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    o.out[0] = r1[0];
+    o.out[1] = r2[0];
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                                  double* __restrict__ part) {
+  __shared__ double red[TB];
+  const SegBound& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  double m = 1e300;
+  if (s < o.nsite) {
+    const size_t i = 4 * (size_t)s;
+    if (MODE == 0) {
+      o.z[i + 3] = fmax(o.z[i + 3], o.lb);
+    } else if (MODE == 1) {
+      o.z[i] = o.y[i];
+      o.z[i + 1] = o.y[i + 1];
+      o.z[i + 2] = o.y[i + 2];
+      const double gs = o.y[i + 3];
+      o.z[i + 3] = (o.x[i + 3] <= o.lb && gs > 0.0) ? 0.0 : gs;
+    } else if (MODE == 2) {
+      if (o.x[i + 3] <= o.lb && o.z[i + 3] < 0.0) o.z[i + 3] = 0.0;
+    } else {
+      const double ds = o.y[i + 3];
+      if (ds < 0.0) m = (o.x[i + 3] - o.lb) / (-ds);
+    }
+  }
+  if (MODE != 3) return;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_seg_min_final(const SegBound* __restrict__ ops,
+                                                        const double* __restrict__ part) {
+  __shared__ double red[1024];
+  const SegBound& o = ops[blockIdx.x];
+  const int n = site_blocks(o.nsite);
+  double m = 1e300;
+  for (int i = threadIdx.x; i < n; i += 1024) m = fmin(m, part[o.blk0 + i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *o.out = red[0];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+// Pinned staging for per-launch descriptor arrays.  Space is handed out
+// bump-style and reclaimed only after the stream has been synchronised.
+struct Arena {
+  char* h = nullptr;
+  char* d = nullptr;
+  size_t cap = 0, used = 0;
+  template <class T>
+  T* push(const T* src, size_t n, hipStream_t st) {
+    const size_t bytes = ((n * sizeof(T)) + 255) & ~(size_t)255;
+    if (used + bytes > cap) return nullptr;
+    std::memcpy(h + used, src, n * sizeof(T));
+    if (hipMemcpyAsync(d + used, h + used, n * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess) return nullptr;
+    T* out = reinterpret_cast<T*>(d + used);
+    used += bytes;
+    return out;
+  }
+};
+
+struct Slot {
+  a2c_fe_ctx* fe = nullptr;
+  bool owned = false;
+  bool used = false;
+  // optimizer
+  bool running = false, started = false, mixed = false;
+  a2c_lmvm_opts opts{};
+  a2c_lmvm_report rep{};
+  int m = 0, npairs = 0, head = 0;
+  double *x = nullptr, *g = nullptr, *gp = nullptr, *d = nullptr, *xo = nullptr, *go = nullptr, *q = nullptr;
+  std::vector<double*> S, Y;
+  std::vector<double> rho, alpha;
+  double f = 0, fold = 0, gnorm = 0, gnorm0 = 0, gd = 0, stp = 0, smax = 0, sy = 0, yy = 0;
+  bool steepest = false;
+  int lsstat = 0, nfev = 0;
+  MoreThuente ls;
+  int N() const { return fe->N; }
+};
+
+}  // namespace a2c
+
+struct a2c_batch {
+  a2c_batch_desc desc{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<a2c::Slot> slots;
+  a2c::Arena arena;
+  double* dF = nullptr;    // objective per slot (device)
+  double* hF = nullptr;    // pinned mirror
+  double* dres = nullptr;  // 2 doubles per slot (device dot/min results)
+  double* hres = nullptr;  // pinned mirror
+  double* dpart = nullptr;  // 2 * max blocks
+  size_t part_cap = 0;      // blocks
+  float4* dtab = nullptr;
+  size_t mem_used = 0;
+};
+
+namespace a2c {
+namespace {
+
+int berr(a2c_batch* b, int code, const std::string& m) {
+  b->err = m;
+  return code;
+}
+#define B_HIP(b, call)                                                                            \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return berr((b), A2C_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int bsync(a2c_batch* b) {
+  B_HIP(b, hipStreamSynchronize(b->stream));
+  b->arena.used = 0;
+  return A2C_OK;
+}
+
+// make sure the partial-sum buffer holds nblk blocks (x2 for two dots)
+int ensure_part(a2c_batch* b, size_t nblk) {
+  if (nblk <= b->part_cap) return A2C_OK;
+  if (int rc = bsync(b)) return rc;
+  (void)hipFree(b->dpart);
+  b->dpart = nullptr;
+  B_HIP(b, hipMalloc(&b->dpart, sizeof(double) * 2 * nblk));
+  b->part_cap = nblk;
+  return A2C_OK;
+}
+
+// block -> op map for ops over the listed slots; fills blk0 via setter
+template <class Op, class Fill>
+int seg_launch_prep(a2c_batch* b, const std::vector<int>& ids, Fill fill, Op** dops, int32_t** db2l, int* nblk) {
+  std::vector<Op> ops(ids.size());
+  std::vector<int32_t> map;
+  int cum = 0;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    Slot& s = b->slots[ids[k]];
+    fill(ops[k], s);
+    ops[k].nsite = s.N();
+    ops[k].blk0 = cum;
+    const int nb = site_blocks(s.N());
+    map.insert(map.end(), nb, (int32_t)k);
+    cum += nb;
+  }
+  *nblk = cum;
+  if (int rc = ensure_part(b, cum)) return rc;
+  size_t need = ((ops.size() * sizeof(Op) + 255) & ~(size_t)255) + ((map.size() * 4 + 255) & ~(size_t)255);
+  if (b->arena.used + need > b->arena.cap)
+    if (int rc = bsync(b)) return rc;
+  *dops = b->arena.push(ops.data(), ops.size(), b->stream);
+  *db2l = map.empty() ? nullptr : b->arena.push(map.data(), map.size(), b->stream);
+  if (!*dops || (!map.empty() && !*db2l)) return berr(b, A2C_ENOMEM, "staging arena too small");
+  return A2C_OK;
+}
+
+template <class Fill>
+int seg_axpby(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
+  if (ids.empty()) return A2C_OK;
+  SegAx* ops;
+  int32_t* map;
+  int nblk;
+  if (int rc = seg_launch_prep<SegAx>(b, ids, fill, &ops, &map, &nblk)) return rc;
+  if (nblk) hipLaunchKernelGGL(k_seg_axpby, dim3(nblk), dim3(TB), 0, b->stream, ops, map);
+  B_HIP(b, hipGetLastError());
+  return A2C_OK;
+}
+
+// dot products per slot; results (2 per slot) returned in out[k*2 + 0/1]
+template <class Fill>
+int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<double>& out) {
+  out.assign(2 * ids.size(), 0.0);
+  if (ids.empty()) return A2C_OK;
+  auto fill2 = [&](SegDot& o, Slot& s) {
+    o.a2 = o.b2 = nullptr;
+    fill(o, s);
+    o.out = b->dres + 2 * (&s - b->slots.data());
+  };
+  SegDot* ops;
+  int32_t* map;
+  int nblk;
+  if (int rc = seg_launch_prep<SegDot>(b, ids, fill2, &ops, &map, &nblk)) return rc;
+  if (nblk) hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, b->stream, ops, map, b->dpart, nblk);
+  B_HIP(b, hipGetLastError());
+  hipLaunchKernelGGL(k_seg_dot_final, dim3(ids.size()), dim3(1024), 0, b->stream, ops, b->dpart, nblk);
+  B_HIP(b, hipGetLastError());
+  B_HIP(b, hipMemcpyAsync(b->hres, b->dres, sizeof(double) * 2 * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
+  if (int rc = bsync(b)) return rc;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    out[2 * k] = b->hres[2 * ids[k]];
+    out[2 * k + 1] = b->hres[2 * ids[k] + 1];
+  }
+  return A2C_OK;
+}
+
+template <int MODE, class Fill>
+int seg_bound(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<double>* out = nullptr) {
+  if (ids.empty()) return A2C_OK;
+  auto fill2 = [&](SegBound& o, Slot& s) {
+    fill(o, s);
+    o.out = b->dres + 2 * (&s - b->slots.data());
+  };
+  SegBound* ops;
+  int32_t* map;
+  int nblk;
+  if (int rc = seg_launch_prep<SegBound>(b, ids, fill2, &ops, &map, &nblk)) return rc;
+  if (nblk) hipLaunchKernelGGL(k_seg_bound<MODE>, dim3(nblk), dim3(TB), 0, b->stream, ops, map, b->dpart);
+  B_HIP(b, hipGetLastError());
+  if (MODE == 3) {
+    hipLaunchKernelGGL(k_seg_min_final, dim3(ids.size()), dim3(1024), 0, b->stream, ops, b->dpart);
+    B_HIP(b, hipGetLastError());
+    B_HIP(b, hipMemcpyAsync(b->hres, b->dres, sizeof(double) * 2 * b->slots.size(), hipMemcpyDeviceToHost,
+                            b->stream));
+    if (int rc = bsync(b)) return rc;
+    out->resize(ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) (*out)[k] = b->hres[2 * ids[k]];
+  }
+  return A2C_OK;
+}
+
+// Fused objective + gradient of the listed slots at X[slot] -> G[slot].
+// state_of(s) selects the state/gradient buffers.  Slots are evaluated in
+// FP64 or mixed precision as `fp64` says.  f values land in hF after sync.
+template <class StateOf>
+int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf state_of, bool sync = true) {
+  if (ids.empty()) return A2C_OK;
+  std::vector<LatDev> lats(ids.size());
+  std::vector<int32_t> map;
+  int cum = 0;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    Slot& s = b->slots[ids[k]];
+    LatDev L = s.fe->sys();
+    const double* X;
+    double* G;
+    state_of(s, &X, &G);
+    L.X = X;
+    L.G = G;
+    L.f = b->dF + ids[k];
+    L.blk0 = cum;
+    lats[k] = L;
+    const int nb = site_blocks(s.N());
+    map.insert(map.end(), nb, (int32_t)k);
+    cum += nb;
+  }
+  size_t need = ((lats.size() * sizeof(LatDev) + 255) & ~(size_t)255) + ((map.size() * 4 + 255) & ~(size_t)255);
+  if (b->arena.used + need > b->arena.cap)
+    if (int rc = bsync(b)) return rc;
+  LatDev* dl = b->arena.push(lats.data(), lats.size(), b->stream);
+  int32_t* dm = map.empty() ? nullptr : b->arena.push(map.data(), map.size(), b->stream);
+  if (!dl || (!map.empty() && !dm)) return berr(b, A2C_ENOMEM, "staging arena too small");
+  Launch L;
+  L.lats = dl;
+  L.b2l = dm;
+  L.nblk = cum;
+  L.nlat = (int)ids.size();
+  L.stream = b->stream;
+  L.deriv_mode = b->desc.deriv_mode;
+  L.fd_eps = b->desc.fd_eps;
+  L.tab = b->dtab;
+  int rc = fp64 ? run_eval64(L, nullptr) : run_eval32(L, nullptr);
+  if (rc) return berr(b, rc, "fused eval launch failed");
+  if (!sync) return A2C_OK;
+  B_HIP(b, hipMemcpyAsync(b->hF, b->dF, sizeof(double) * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
+  return bsync(b);
+}
+
+// ---------------------------------------------------------------- optimizer
+void lmvm_free(Slot& s) {
+  for (double* p : {s.x, s.g, s.gp, s.d, s.xo, s.go, s.q}) (void)hipFree(p);
+  for (double* p : s.S) (void)hipFree(p);
+  for (double* p : s.Y) (void)hipFree(p);
+  s.x = s.g = s.gp = s.d = s.xo = s.go = s.q = nullptr;
+  s.S.clear();
+  s.Y.clear();
+}
+
+int lmvm_alloc(a2c_batch* b, Slot& s) {
+  lmvm_free(s);
+  const size_t n = std::max<size_t>(4 * (size_t)s.N(), 4);
+  for (double** p : {&s.x, &s.g, &s.gp, &s.d, &s.xo, &s.go, &s.q}) B_HIP(b, hipMalloc(p, n * sizeof(double)));
+  s.S.assign(s.m, nullptr);
+  s.Y.assign(s.m, nullptr);
+  for (int i = 0; i < s.m; ++i) {
+    B_HIP(b, hipMalloc(&s.S[i], n * sizeof(double)));
+    B_HIP(b, hipMalloc(&s.Y[i], n * sizeof(double)));
+  }
+  s.rho.assign(s.m, 0.0);
+  s.alpha.assign(s.m, 0.0);
+  return A2C_OK;
+}
+
+auto lmvm_state = [](Slot& s, const double** X, double** G) {
+  *X = s.x;
+  *G = s.g;
+};
+
+int converged_reason(const Slot& s) {
+  if (!std::isfinite(s.gnorm) || !std::isfinite(s.f)) return -4;
+  if (s.gnorm <= s.opts.gatol) return 1;
+  if (s.opts.grtol > 0 && s.gnorm <= s.opts.grtol * std::fabs(s.f)) return 2;
+  if (s.opts.gttol > 0 && s.gnorm <= s.opts.gttol * s.gnorm0) return 3;
+  return 0;
+}
+
+// evaluate f, g at x for the listed slots, each in its own precision
+int eval_slots(a2c_batch* b, const std::vector<int>& ids) {
+  std::vector<int> m32, m64;
+  for (int i : ids) {
+    Slot& s = b->slots[i];
+    (s.mixed ? m32 : m64).push_back(i);
+    s.rep.evals++;
+    if (!s.mixed) s.rep.evals_fp64++;
+  }
+  if (int rc = fused_eval(b, m32, false, lmvm_state, false)) return rc;
+  if (int rc = fused_eval(b, m64, true, lmvm_state, false)) return rc;
+  B_HIP(b, hipMemcpyAsync(b->hF, b->dF, sizeof(double) * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
+  if (int rc = bsync(b)) return rc;
+  for (int i : ids) b->slots[i].f = b->hF[i];
+  return A2C_OK;
+}
+
+int proj_gnorm(a2c_batch* b, const std::vector<int>& ids) {
+  if (int rc = seg_bound<1>(b, ids, [](SegBound& o, Slot& s) {
+        o.z = s.gp; o.x = s.x; o.y = s.g; o.lb = s.opts.sigma_lb;
+      }))
+    return rc;
+  std::vector<double> v;
+  if (int rc = seg_dot(b, ids, [](SegDot& o, Slot& s) { o.a1 = s.gp; o.b1 = s.gp; }, v)) return rc;
+  for (size_t k = 0; k < ids.size(); ++k) b->slots[ids[k]].gnorm = std::sqrt(v[2 * k]);
+  return A2C_OK;
+}
+
+// slots whose mixed phase ended: re-evaluate in FP64 at the same point
+int switch_to_fp64(a2c_batch* b, const std::vector<int>& ids) {
+  if (ids.empty()) return A2C_OK;
+  for (int i : ids) b->slots[i].mixed = false;
+  if (int rc = eval_slots(b, ids)) return rc;
+  return proj_gnorm(b, ids);
+}
+
+// first evaluation of newly started slots
+int lmvm_init(a2c_batch* b, const std::vector<int>& ids) {
+  if (ids.empty()) return A2C_OK;
+  if (int rc = seg_bound<0>(b, ids, [](SegBound& o, Slot& s) { o.z = s.x; o.lb = s.opts.sigma_lb; })) return rc;
+  if (int rc = eval_slots(b, ids)) return rc;
+  if (int rc = proj_gnorm(b, ids)) return rc;
+  std::vector<int> sw;
+  for (int i : ids) {
+    Slot& s = b->slots[i];
+    if (s.mixed && s.opts.mixed_switch && s.gnorm <= s.opts.mixed_gtol) sw.push_back(i);
+  }
+  if (int rc = switch_to_fp64(b, sw)) return rc;
+  for (int i : ids) {
+    Slot& s = b->slots[i];
+    s.gnorm0 = s.gnorm;
+    s.started = true;
+    s.rep.reason = converged_reason(s);
+    s.rep.f = s.f;
+    s.rep.gnorm = s.gnorm;
+    if (s.rep.reason != 0) s.running = false;
+  }
+  return A2C_OK;
+}
+
+// d = -H g for the listed slots (two-loop recursion, or scaled steepest
+// descent for slots with no pairs or marked steepest)
+int directions(a2c_batch* b, const std::vector<int>& ids) {
+  std::vector<int> tl, sd;
+  int maxp = 0;
+  for (int i : ids) {
+    Slot& s = b->slots[i];
+    if (s.steepest || s.npairs == 0) {
+      s.npairs = 0;
+      sd.push_back(i);
+    } else {
+      tl.push_back(i);
+      maxp = std::max(maxp, s.npairs);
+    }
+  }
+  std::vector<double> v;
+  if (!tl.empty()) {
+    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; }))
+      return rc;
+    for (int t = 0; t < maxp; ++t) {  // newest to oldest
+      std::vector<int> sub;
+      for (int i : tl)
+        if (t < b->slots[i].npairs) sub.push_back(i);
+      auto pidx = [t](const Slot& s) { return (s.head - 1 - t + s.m) % s.m; };
+      if (int rc = seg_dot(b, sub, [&](SegDot& o, Slot& s) { o.a1 = s.S[pidx(s)]; o.b1 = s.q; }, v)) return rc;
+      for (size_t k = 0; k < sub.size(); ++k) {
+        Slot& s = b->slots[sub[k]];
+        s.alpha[pidx(s)] = s.rho[pidx(s)] * v[2 * k];
+      }
+      if (int rc = seg_axpby(b, sub, [&](SegAx& o, Slot& s) {
+            o.z = s.q; o.x = s.q; o.y = s.Y[pidx(s)]; o.a = 1.0; o.b = -s.alpha[pidx(s)];
+          }))
+        return rc;
+    }
+    if (int rc = seg_dot(b, tl, [](SegDot& o, Slot& s) {
+          const int nw = (s.head - 1 + s.m) % s.m;
+          o.a1 = s.S[nw]; o.b1 = s.Y[nw]; o.a2 = s.Y[nw]; o.b2 = s.Y[nw];
+        }, v))
+      return rc;
+    for (size_t k = 0; k < tl.size(); ++k) b->slots[tl[k]].gd = v[2 * k] / v[2 * k + 1];  // H0 scale (temporary)
+    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.q; o.y = s.q; o.a = s.gd; o.b = 0.0; }))
+      return rc;
+    for (int t = maxp - 1; t >= 0; --t) {  // oldest to newest
+      std::vector<int> sub;
+      for (int i : tl)
+        if (t < b->slots[i].npairs) sub.push_back(i);
+      auto pidx = [t](const Slot& s) { return (s.head - 1 - t + s.m) % s.m; };
+      if (int rc = seg_dot(b, sub, [&](SegDot& o, Slot& s) { o.a1 = s.Y[pidx(s)]; o.b1 = s.q; }, v)) return rc;
+      std::vector<double> coef(sub.size());
+      for (size_t k = 0; k < sub.size(); ++k) {
+        Slot& s = b->slots[sub[k]];
+        coef[k] = s.alpha[pidx(s)] - s.rho[pidx(s)] * v[2 * k];
+      }
+      size_t kk = 0;
+      if (int rc = seg_axpby(b, sub, [&](SegAx& o, Slot& s) {
+            o.z = s.q; o.x = s.q; o.y = s.S[pidx(s)]; o.a = 1.0; o.b = coef[kk++];
+          }))
+        return rc;
+    }
+    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.d; o.x = s.q; o.y = s.q; o.a = -1.0; o.b = 0.0; }))
+      return rc;
+  }
+  if (!sd.empty()) {
+    // scaled steepest descent: TAO's initial scale 2|f|/||g||^2, capped so
+    // that the first trial moves no coordinate by more than 0.05 A
+    if (int rc = seg_dot(b, sd, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.g; }, v)) return rc;
+    std::vector<double> del(sd.size());
+    for (size_t k = 0; k < sd.size(); ++k) {
+      Slot& s = b->slots[sd[k]];
+      const double gg = std::max(v[2 * k], 1e-300);
+      double delta = (s.f != 0.0 ? 2.0 * std::fabs(s.f) : 2.0) / gg;
+      del[k] = std::min(delta, 0.05 / std::sqrt(gg));
+    }
+    size_t kk = 0;
+    if (int rc = seg_axpby(b, sd, [&](SegAx& o, Slot& s) { o.z = s.d; o.x = s.g; o.y = s.g; o.a = -del[kk++]; o.b = 0.0; }))
+      return rc;
+  }
+  if (int rc = seg_bound<2>(b, ids, [](SegBound& o, Slot& s) { o.z = s.d; o.x = s.x; o.lb = s.opts.sigma_lb; }))
+    return rc;
+  if (int rc = seg_dot(b, ids, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
+  for (size_t k = 0; k < ids.size(); ++k) b->slots[ids[k]].gd = v[2 * k];
+  return A2C_OK;
+}
+
+// One LMVM iteration for every running, started slot.
+int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
+  for (int i : act) b->slots[i].steepest = false;
+  if (int rc = directions(b, act)) return rc;
+  // retry non-descent directions with steepest descent; give up if still bad
+  std::vector<int> redo, go;
+  for (int i : act) (b->slots[i].gd < 0.0 ? go : redo).push_back(i);
+  if (!redo.empty()) {
+    for (int i : redo) b->slots[i].steepest = true;
+    if (int rc = directions(b, redo)) return rc;
+    for (int i : redo) {
+      Slot& s = b->slots[i];
+      if (s.gd < 0.0) go.push_back(i);
+      else { s.rep.reason = -5; s.running = false; }
+    }
+  }
+  std::sort(go.begin(), go.end());
+  if (go.empty()) return A2C_OK;
+  // line search setup
+  std::vector<double> smax;
+  if (int rc = seg_bound<3>(b, go, [](SegBound& o, Slot& s) { o.x = s.x; o.y = s.d; o.lb = s.opts.sigma_lb; }, &smax))
+    return rc;
+  if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.xo; o.x = s.x; o.y = s.x; o.a = 1.0; o.b = 0.0; })) return rc;
+  if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.go; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; })) return rc;
+  for (size_t k = 0; k < go.size(); ++k) {
+    Slot& s = b->slots[go[k]];
+    s.fold = s.f;
+    s.ls = MoreThuente();
+    s.ls.stpmax = std::min(1e20, smax[k]);
+    s.ls.epsf = s.mixed ? 0.0 : 1e-11 * std::fabs(s.f);
+    s.stp = std::min(1.0, s.ls.stpmax);
+    s.ls.start(s.f, s.gd, s.stp);
+    s.nfev = 0;
+    s.lsstat = 0;
+  }
+  std::vector<int> srch = go;
+  std::vector<double> v;
+  while (!srch.empty()) {
+    std::vector<int> run, cap;
+    for (int i : srch) (b->slots[i].rep.evals >= b->slots[i].opts.max_funcs ? cap : run).push_back(i);
+    for (int i : cap) b->slots[i].lsstat = -1;
+    if (run.empty()) break;
+    if (int rc = seg_axpby(b, run, [](SegAx& o, Slot& s) { o.z = s.x; o.x = s.xo; o.y = s.d; o.a = 1.0; o.b = s.stp; }))
+      return rc;
+    if (int rc = eval_slots(b, run)) return rc;
+    if (int rc = seg_dot(b, run, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
+    std::vector<int> next;
+    for (size_t k = 0; k < run.size(); ++k) {
+      Slot& s = b->slots[run[k]];
+      s.nfev++;
+      if (!std::isfinite(s.f)) { s.lsstat = -2; continue; }
+      s.lsstat = s.ls.iterate(s.f, v[2 * k], s.stp);
+      if (s.lsstat == 0 && s.nfev >= 30) s.lsstat = 2;
+      if (s.lsstat == 0) next.push_back(run[k]);
+    }
+    srch = next;
+  }
+  // outcomes
+  std::vector<int> ok, bad, sw;
+  for (int i : go) {
+    Slot& s = b->slots[i];
+    const bool decreased = (s.f < s.fold) || s.lsstat == 1;
+    (s.lsstat >= 0 && decreased ? ok : bad).push_back(i);
+  }
+  if (!bad.empty()) {
+    if (int rc = seg_axpby(b, bad, [](SegAx& o, Slot& s) { o.z = s.x; o.x = s.xo; o.y = s.xo; o.a = 1.0; o.b = 0.0; })) return rc;
+    if (int rc = seg_axpby(b, bad, [](SegAx& o, Slot& s) { o.z = s.g; o.x = s.go; o.y = s.go; o.a = 1.0; o.b = 0.0; })) return rc;
+    for (int i : bad) {
+      Slot& s = b->slots[i];
+      s.f = s.fold;
+      if (s.lsstat == -1) { s.rep.reason = -3; s.running = false; }
+      else if (s.lsstat == -2) { s.rep.reason = -4; s.running = false; }
+      else if (s.mixed && s.opts.mixed_switch) { sw.push_back(i); s.npairs = 0; }
+      else if (s.npairs == 0) { s.rep.reason = -6; s.running = false; }
+      else s.npairs = 0;
+    }
+  }
+  if (int rc = switch_to_fp64(b, sw)) return rc;
+  for (int i : sw) {
+    Slot& s = b->slots[i];
+    s.rep.reason = converged_reason(s);
+    if (s.rep.reason) s.running = false;
+  }
+  if (!ok.empty()) {
+    if (int rc = seg_axpby(b, ok, [](SegAx& o, Slot& s) { o.z = s.S[s.head]; o.x = s.x; o.y = s.xo; o.a = 1.0; o.b = -1.0; }))
+      return rc;
+    if (int rc = seg_axpby(b, ok, [](SegAx& o, Slot& s) { o.z = s.Y[s.head]; o.x = s.g; o.y = s.go; o.a = 1.0; o.b = -1.0; }))
+      return rc;
+    if (int rc = seg_dot(b, ok, [](SegDot& o, Slot& s) { o.a1 = s.S[s.head]; o.b1 = s.Y[s.head]; }, v)) return rc;
+    for (size_t k = 0; k < ok.size(); ++k) {
+      Slot& s = b->slots[ok[k]];
+      if (v[2 * k] > 1e-300) {  // keep only pairs with positive curvature
+        s.rho[s.head] = 1.0 / v[2 * k];
+        s.head = (s.head + 1) % s.m;
+        s.npairs = std::min(s.npairs + 1, s.m);
+      }
+    }
+    if (int rc = proj_gnorm(b, ok)) return rc;
+    std::vector<int> sw2;
+    for (int i : ok) {
+      Slot& s = b->slots[i];
+      s.rep.iterations++;
+      if (s.mixed && s.opts.mixed_switch && s.gnorm <= s.opts.mixed_gtol) sw2.push_back(i);
+    }
+    if (int rc = switch_to_fp64(b, sw2)) return rc;
+    for (int i : ok) {
+      Slot& s = b->slots[i];
+      s.rep.reason = converged_reason(s);
+      if (s.rep.reason == 0 && s.rep.iterations >= s.opts.max_it) s.rep.reason = -2;
+      if (s.rep.reason) s.running = false;
+    }
+  }
+  for (int i : act) {
+    Slot& s = b->slots[i];
+    s.rep.f = s.f;
+    s.rep.gnorm = s.gnorm;
+  }
+  return A2C_OK;
+}
+
+}  // namespace
+
+// Advance every running slot by up to max_iters iterations.
+int batch_step(a2c_batch* b, int max_iters, int* n_running) {
+  std::vector<int> init;
+  for (size_t i = 0; i < b->slots.size(); ++i)
+    if (b->slots[i].used && b->slots[i].running && !b->slots[i].started) init.push_back((int)i);
+  if (int rc = lmvm_init(b, init)) return rc;
+  for (int it = 0; it < max_iters; ++it) {
+    std::vector<int> act;
+    for (size_t i = 0; i < b->slots.size(); ++i) {
+      Slot& s = b->slots[i];
+      if (s.used && s.running && s.started) {
+        if (s.rep.iterations >= s.opts.max_it) { s.rep.reason = -2; s.running = false; continue; }
+        act.push_back((int)i);
+      }
+    }
+    if (act.empty()) break;
+    if (int rc = lmvm_iteration(b, act)) return rc;
+  }
+  int nr = 0;
+  for (auto& s : b->slots) nr += (s.used && s.running) ? 1 : 0;
+  if (n_running) *n_running = nr;
+  return A2C_OK;
+}
+
+int batch_eval_slots(a2c_batch* b, const std::vector<int>& ids, bool fp64) {
+  return fused_eval(b, ids, fp64, [](Slot& s, const double** X, double** G) {
+    *X = s.fe->dX;
+    *G = s.fe->dG;
+  });
+}
+
+}  // namespace a2c
+
+// =========================================================================
+// C-ABI
+// =========================================================================
+using namespace a2c;
+
+extern "C" {
+
+void a2c_batch_desc_default(a2c_batch_desc* d) {
+  std::memset(d, 0, sizeof(*d));
+  d->device = 0;
+  d->precision = A2C_PREC_MIXED;
+  d->deriv_mode = A2C_DERIV_ANALYTIC;
+  d->fd_eps = 1e-5;
+  d->max_lattices = 64;
+  d->hbm_budget_bytes = (size_t)200 << 30;
+  d->max_neighbors = NBR_CAP;
+}
+
+int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
+  if (!desc || !out || desc->max_lattices <= 0 || desc->max_lattices > (1 << 20)) return A2C_EINVAL;
+  if (desc->precision != A2C_PREC_FP64 && desc->precision != A2C_PREC_MIXED) return A2C_EINVAL;
+  *out = nullptr;
+  a2c_batch* b = new (std::nothrow) a2c_batch;
+  if (!b) return A2C_ENOMEM;
+  b->desc = *desc;
+  auto fail = [&](int rc) { a2c_batch_destroy(b); return rc; };
+  if (hipSetDevice(desc->device) != hipSuccess) return fail(A2C_EHIP);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(A2C_EHIP);
+  const size_t cap = desc->max_lattices;
+  b->slots.resize(cap);
+  b->arena.cap = (size_t)32 << 20;
+  bool ok = hipHostMalloc(&b->arena.h, b->arena.cap) == hipSuccess && hipMalloc(&b->arena.d, b->arena.cap) == hipSuccess &&
+            hipMalloc(&b->dF, 8 * cap) == hipSuccess && hipHostMalloc(&b->hF, 8 * cap) == hipSuccess &&
+            hipMalloc(&b->dres, 16 * cap) == hipSuccess && hipHostMalloc(&b->hres, 16 * cap) == hipSuccess;
+  if (!ok) return fail(A2C_ENOMEM);
+  std::vector<float4> t;
+  host_tables(t);
+  if (hipMalloc(&b->dtab, sizeof(float4) * t.size()) != hipSuccess) return fail(A2C_ENOMEM);
+  if (hipMemcpy(b->dtab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice) != hipSuccess) return fail(A2C_EHIP);
+  *out = b;
+  return A2C_OK;
+}
+
+void a2c_batch_destroy(a2c_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& s : b->slots) {
+    lmvm_free(s);
+    if (s.owned) a2c_fe_destroy(s.fe);
+  }
+  (void)hipFree(b->arena.d);
+  if (b->arena.h) (void)hipHostFree(b->arena.h);
+  (void)hipFree(b->dF);
+  if (b->hF) (void)hipHostFree(b->hF);
+  (void)hipFree(b->dres);
+  if (b->hres) (void)hipHostFree(b->hres);
+  (void)hipFree(b->dpart);
+  (void)hipFree(b->dtab);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* a2c_batch_last_error(const a2c_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+size_t a2c_batch_footprint(int32_t n_pd, int32_t n_h, int32_t lmvm_m) {
+  const size_t N = (size_t)n_pd + n_h;
+  // state/gradient/staging 4N doubles x 4, per-site arrays ~5 doubles, lists
+  // ~90 int32 per site, optimizer (7 + 2m) x 4N doubles
+  return N * (4 * 32 + 40 + 90 * 4 + (7 + 2 * (size_t)std::max(lmvm_m, 1)) * 32) + ((size_t)1 << 20);
+}
+
+size_t a2c_batch_mem_used(const a2c_batch* b) { return b ? b->mem_used : 0; }
+
+int a2c_batch_add(a2c_batch* b, const a2c_fe_desc* lat, const double* X, const double* x, int32_t* slot) {
+  if (!b) return A2C_EINVAL;
+  b->err.clear();
+  if (!lat || !X || !slot || (lat->n_h > 0 && !x)) return berr(b, A2C_EINVAL, "null argument");
+  int free_slot = -1;
+  for (size_t i = 0; i < b->slots.size(); ++i)
+    if (!b->slots[i].used) { free_slot = (int)i; break; }
+  if (free_slot < 0) return berr(b, A2C_ENOMEM, "no free slot");
+  const size_t need = a2c_batch_footprint(lat->n_pd, lat->n_h, 5);
+  if (b->mem_used + need > b->desc.hbm_budget_bytes) return berr(b, A2C_ENOMEM, "HBM budget exceeded");
+  a2c_fe_desc d = *lat;
+  d.device = b->desc.device;
+  d.precision = b->desc.precision;
+  d.deriv_mode = b->desc.deriv_mode;
+  d.fd_eps = b->desc.fd_eps;
+  if (d.max_neighbors == 0) d.max_neighbors = b->desc.max_neighbors;
+  a2c_fe_ctx* fe = nullptr;
+  int rc = a2c_fe_create(&d, &fe);
+  if (rc) return berr(b, rc, "lattice context creation failed");
+  if ((rc = a2c_fe_set_occupancy(fe, x)) || (rc = a2c_fe_build_neighbors(fe, X, nullptr))) {
+    b->err = a2c_fe_last_error(fe);
+    a2c_fe_destroy(fe);
+    return rc;
+  }
+  Slot& s = b->slots[free_slot];
+  s = Slot();
+  s.fe = fe;
+  s.owned = true;
+  s.used = true;
+  b->mem_used += need;
+  *slot = free_slot;
+  return A2C_OK;
+}
+
+int a2c_batch_remove(a2c_batch* b, int32_t slot) {
+  if (!b) return A2C_EINVAL;
+  if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return berr(b, A2C_EINVAL, "bad slot");
+  if (int rc = bsync(b)) return rc;
+  Slot& s = b->slots[slot];
+  b->mem_used -= std::min(b->mem_used, a2c_batch_footprint(s.fe->desc.n_pd, s.fe->desc.n_h, 5));
+  lmvm_free(s);
+  if (s.owned) a2c_fe_destroy(s.fe);
+  s = Slot();
+  return A2C_OK;
+}
+
+int a2c_batch_set_state(a2c_batch* b, int32_t slot, const double* X) {
+  if (!b || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used || !X) return A2C_EINVAL;
+  if (int rc = bsync(b)) return rc;  // no launch may still read the state
+  Slot& s = b->slots[slot];
+  B_HIP(b, hipMemcpy(s.fe->dX, X, 32 * (size_t)s.N(), hipMemcpyHostToDevice));
+  return A2C_OK;
+}
+
+int a2c_batch_get_state(a2c_batch* b, int32_t slot, double* X, double* G) {
+  if (!b || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return A2C_EINVAL;
+  if (int rc = bsync(b)) return rc;
+  Slot& s = b->slots[slot];
+  const double* src = s.x ? s.x : s.fe->dX;
+  const double* gsrc = s.x ? s.g : s.fe->dG;
+  if (X) B_HIP(b, hipMemcpy(X, src, 32 * (size_t)s.N(), hipMemcpyDeviceToHost));
+  if (G) B_HIP(b, hipMemcpy(G, gsrc, 32 * (size_t)s.N(), hipMemcpyDeviceToHost));
+  return A2C_OK;
+}
+
+int a2c_batch_eval(a2c_batch* b, int32_t n, const int32_t* slots, int32_t precision, double* f) {
+  if (!b || n < 0 || (n > 0 && (!slots || !f))) return A2C_EINVAL;
+  b->err.clear();
+  std::vector<int> ids(slots, slots + n);
+  for (int i : ids)
+    if (i < 0 || i >= (int)b->slots.size() || !b->slots[i].used) return berr(b, A2C_EINVAL, "bad slot");
+  if (int rc = batch_eval_slots(b, ids, precision != A2C_PREC_MIXED)) return rc;
+  for (int k = 0; k < n; ++k) f[k] = b->hF[ids[k]];
+  return A2C_OK;
+}
+
+int a2c_batch_lmvm_start(a2c_batch* b, int32_t slot, const a2c_lmvm_opts* o) {
+  if (!b || !o) return A2C_EINVAL;
+  b->err.clear();
+  if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return berr(b, A2C_EINVAL, "bad slot");
+  if (o->m < 1 || o->m > 64 || o->max_it < 0 || o->max_funcs < 1 || !(o->sigma_lb >= 0))
+    return berr(b, A2C_EINVAL, "bad options");
+  Slot& s = b->slots[slot];
+  s.opts = *o;
+  s.m = o->m;
+  if (int rc = lmvm_alloc(b, s)) return rc;
+  B_HIP(b, hipMemcpyAsync(s.x, s.fe->dX, 32 * (size_t)s.N(), hipMemcpyDeviceToDevice, b->stream));
+  s.rep = a2c_lmvm_report{};
+  s.npairs = s.head = 0;
+  s.mixed = s.fe->desc.precision == A2C_PREC_MIXED;
+  s.running = true;
+  s.started = false;
+  return A2C_OK;
+}
+
+int a2c_batch_lmvm_step(a2c_batch* b, int32_t max_iters, int32_t* n_running) {
+  if (!b || max_iters < 0) return A2C_EINVAL;
+  b->err.clear();
+  return batch_step(b, max_iters, n_running);
+}
+
+int a2c_batch_lmvm_report(a2c_batch* b, int32_t slot, a2c_lmvm_report* rep) {
+  if (!b || !rep || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return A2C_EINVAL;
+  *rep = b->slots[slot].rep;
+  return A2C_OK;
+}
+
+}  // extern "C"
+
+// Single-lattice minimizer on the same code: a batch of one wrapping ctx.
+namespace a2c {
+int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, double* Xout, a2c_lmvm_report* rep) {
+  a2c_batch_desc bd;
+  a2c_batch_desc_default(&bd);
+  bd.device = c->desc.device;
+  bd.precision = c->desc.precision;
+  bd.deriv_mode = c->desc.deriv_mode;
+  bd.fd_eps = c->desc.fd_eps;
+  bd.max_lattices = 1;
+  a2c_batch* b = nullptr;
+  if (int rc = a2c_batch_create(&bd, &b)) return set_err(c, rc, "batch creation failed");
+  struct Guard {
+    a2c_batch* b;
+    ~Guard() { a2c_batch_destroy(b); }
+  } guard{b};
+  if (c->desc.precision == A2C_PREC_MIXED && !c->daux)
+    if (int rc = build_tables(c)) return rc;
+  Slot& s = b->slots[0];
+  s.fe = c;
+  s.owned = false;
+  s.used = true;
+  A2C_HIP(c, hipMemcpy(c->dX, X0, 32 * (size_t)c->N, hipMemcpyHostToDevice));
+  c->lat_dirty = true;
+  int rc = a2c_batch_lmvm_start(b, 0, o);
+  if (!rc) {
+    int nr = 1;
+    while (nr > 0 && !rc) rc = batch_step(b, 1 << 20, &nr);
+  }
+  if (rc) return set_err(c, rc, b->err);
+  *rep = s.rep;
+  A2C_HIP(c, hipMemcpy(Xout, s.x, 32 * (size_t)c->N, hipMemcpyDeviceToHost));
+  return A2C_OK;
+}
+}  // namespace a2c
+
+extern "C" void a2c_lmvm_opts_default(a2c_lmvm_opts* o) {
+  o->m = 5;            // -tao_lmm_vectors
+  o->max_it = 2000;    // -tao_max_it
+  o->max_funcs = 4000; // -tao_max_funcs
+  o->gatol = 1e-8;
+  o->grtol = 1e-8;
+  o->gttol = 0.0;
+  o->sigma_lb = 1e-6;  // formVariableBounds, minimizer.cpp:815,822
+  o->mixed_switch = 1;
+  o->mixed_gtol = 1e-5;
+}
+
+extern "C" int a2c_fe_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* opts, double* X_out,
+                               a2c_lmvm_report* report) {
+  if (!c) return A2C_EINVAL;
+  c->err.clear();
+  if (!X0 || !X_out || !opts || !report) return a2c::set_err(c, A2C_EINVAL, "null argument");
+  if (!c->have_nbr || !c->have_x) return a2c::set_err(c, A2C_ESTATE, "neighbors/occupancy not set");
+  if (opts->m < 1 || opts->max_it < 0 || opts->max_funcs < 1 || !(opts->sigma_lb >= 0))
+    return a2c::set_err(c, A2C_EINVAL, "bad options");
+  return a2c::lmvm_minimize(c, X0, opts, X_out, report);
+}

@@ -90,8 +90,13 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
             hipMalloc(&c->dx, 8 * nh) == hipSuccess && hipMalloc(&c->drho, 8 * N) == hipSuccess &&
             hipMalloc(&c->dFp, 8 * N) == hipSuccess && hipMalloc(&c->demb, 8 * N) == hipSuccess &&
             hipMalloc(&c->dpart, 8 * (size_t)c->nblk_red) == hipSuccess && hipMalloc(&c->dF, 64) == hipSuccess &&
-            hipHostMalloc(&c->hX, 32 * N) == hipSuccess && hipHostMalloc(&c->hG, 32 * N) == hipSuccess;
+            hipHostMalloc(&c->hX, 32 * N) == hipSuccess && hipHostMalloc(&c->hG, 32 * N) == hipSuccess &&
+            hipHostMalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && hipMalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
+            hipMalloc(&c->db2l, 4 * (size_t)c->nblk_red) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
+  new (c->hlat) LatDev();
+  if (hipMemset(c->db2l, 0, 4 * (size_t)c->nblk_red) != hipSuccess) return fail(A2C_EHIP);
+  c->lat_dirty = true;
   if (desc->precision == A2C_PREC_MIXED && build_tables(c) != A2C_OK) return fail(A2C_EHIP);
   *out = c;
   return A2C_OK;
@@ -105,6 +110,9 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   for (int w = 0; w < 5; ++w) { hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt); }
   if (c->hX) hipHostFree(c->hX);
   if (c->hG) hipHostFree(c->hG);
+  if (c->hlat) hipHostFree(c->hlat);
+  hipFree(c->dlat);
+  hipFree(c->db2l);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -116,6 +124,7 @@ int a2c_fe_set_precision(a2c_fe_ctx* c, int32_t precision) {
   if (precision != A2C_PREC_FP64 && precision != A2C_PREC_MIXED) return set_err(c, A2C_EINVAL, "bad precision");
   if (precision == A2C_PREC_MIXED && !c->dtab) {
     if (int rc = build_tables(c)) return rc;
+    c->lat_dirty = true;
   }
   c->desc.precision = precision;
   return A2C_OK;
@@ -134,6 +143,7 @@ int a2c_fe_build_neighbors(a2c_fe_ctx* c, const double* X, a2c_nbr_stats* stats)
   if (int rc = check_ctx(c)) return rc;
   if (!X && c->N > 0) return set_err(c, A2C_EINVAL, "X is null");
   if (int rc = upload_state(c, X)) return rc;
+  c->lat_dirty = true;
   if (int rc = nbr_build(c)) return rc;
   // totals for the export
   for (int w = 0; w < 5; ++w) {

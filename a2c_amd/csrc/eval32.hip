@@ -28,7 +28,7 @@
 namespace a2c {
 namespace {
 
-constexpr int TB = 256;
+constexpr int TB = SITE_BLOCK;
 constexpr int NT = 500;               // intervals per table
 constexpr float R0 = 1.0f, R1 = 6.0f, INVH = 100.0f;
 constexpr double H64 = 0.01;
@@ -81,7 +81,7 @@ __device__ __forceinline__ Site load_site(const double* __restrict__ X, int s) {
   return {{v.x, v.y, v.z}, v.w};
 }
 // d = q_i - image(q_j) in FP64, rounded to FP32
-__device__ __forceinline__ void pair_vec32(const DevSys& S, const Site& a, const Site& b, float d[3]) {
+__device__ __forceinline__ void pair_vec32(const LatDev& S, const Site& a, const Site& b, float d[3]) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double qj = b.q[c];
@@ -114,15 +114,16 @@ __device__ __forceinline__ float pair_density(const float4* tab, int w, const fl
   return acc;
 }
 
-__global__ __launch_bounds__(TB) void k_density32(DevSys S, const double* __restrict__ X,
-                                                  const double* __restrict__ xo, const float4* __restrict__ gtab,
-                                                  double* __restrict__ rho_out, float2* __restrict__ aux,
-                                                  double* __restrict__ emb) {
+__global__ __launch_bounds__(TB) void k_density32(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+                                                  const float4* __restrict__ gtab) {
   __shared__ float4 tab[2 * NT];  // f_Pd, f_H
   stage_tables(tab, gtab, 2 * NT);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
+  const double* __restrict__ X = S.X;
+  const double* __restrict__ xo = S.x;
   const bool pd = s < S.npd;
   const int row = pd ? s : s - S.npd;
   const Ell LP = pd ? S.pdpd : S.hpd;
@@ -147,11 +148,11 @@ __global__ __launch_bounds__(TB) void k_density32(DevSys S, const double* __rest
     rho += xo[h] * (double)pair_density(tab, T_FH, d, ai, SQ6F * (float)b.s);
   }
   rho *= QW;
-  rho_out[s] = rho;
+  S.rho[s] = rho;
   const double xs = pd ? 1.0 : xo[row];
   const eam::VD F = pd ? eam::F_Pd_vd(rho) : eam::F_H_vd(rho);
-  aux[s] = make_float2((float)F.d, (float)xs);
-  emb[s] = xs * F.v;
+  S.aux[s] = make_float2((float)F.d, (float)xs);
+  S.emb[s] = xs * F.v;
 }
 
 // ---------------------------------------------------------------- gradient
@@ -218,14 +219,17 @@ __device__ __forceinline__ PairSums pair_grad(const float4* tab, const float d[3
   return p;
 }
 
-__global__ __launch_bounds__(TB) void k_grad32(DevSys S, const double* __restrict__ X, const double* __restrict__ xo,
-                                               const float4* __restrict__ gtab, const float2* __restrict__ aux,
-                                               const double* __restrict__ emb,
-                                               double* __restrict__ G, double* __restrict__ part) {
+__global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+                                               const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
   stage_tables(tab, gtab, NTAB * NT);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int lb = blockIdx.x - S.blk0;
+  const int s = lb * TB + threadIdx.x;
   const int N = S.npd + S.nh;
+  const double* __restrict__ X = S.X;
+  const double* __restrict__ xo = S.x;
+  const float2* __restrict__ aux = S.aux;
   double fs = 0.0;
   if (s < N) {
     const bool pd = s < S.npd;
@@ -261,8 +265,9 @@ __global__ __launch_bounds__(TB) void k_grad32(DevSys S, const double* __restric
       e += w * p.pe; g0 += w * p.g[0]; g1 += w * p.g[1]; g2 += w * p.g[2]; gs += w * p.gs;
     }
     const double T = S.T;
-    const double V = emb[s] + 0.5 * QW * xi * e;
+    const double V = S.emb[s] + 0.5 * QW * xi * e;
     const double sc = QW * xi / T;
+    double* __restrict__ G = S.G;
     G[4 * (size_t)s + 0] = sc * g0;
     G[4 * (size_t)s + 1] = sc * g1;
     G[4 * (size_t)s + 2] = sc * g2;
@@ -279,20 +284,7 @@ __global__ __launch_bounds__(TB) void k_grad32(DevSys S, const double* __restric
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(1024) void k_final_sum32(const double* __restrict__ part, int n, double* __restrict__ out) {
-  __shared__ double red[1024];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) acc += part[i];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) S.part[lb] = red[0];
 }
 
 // ---------------------------------------------------------------- tables
@@ -330,27 +322,20 @@ int build_tables(a2c_fe_ctx* c) {
   return A2C_OK;
 }
 
-int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
-  if (!c->dtab) return set_err(c, A2C_ESTATE, "tables not built");
-  const int N = c->N;
-  const DevSys S = c->sys();
-  const int nb = (N + TB - 1) / TB;
-  const float4* tab = c->dtab;
-  if (timed) tm->mark(c->stream, "density32");
-  if (N > 0) {
-    hipLaunchKernelGGL(k_density32, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, tab, c->drho, c->daux, c->demb);
-    A2C_HIP(c, hipGetLastError());
+int run_eval32(const Launch& L, Timing* tm) {
+  if (tm) tm->mark(L.stream, "density32");
+  if (L.nblk > 0) {
+    hipLaunchKernelGGL(k_density32, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+    if (hipGetLastError() != hipSuccess) return A2C_EHIP;
   }
-  if (timed) tm->mark(c->stream, "grad32");
-  if (N > 0) {
-    hipLaunchKernelGGL(k_grad32, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, tab, c->daux, c->demb, dG,
-                       c->dpart);
-    A2C_HIP(c, hipGetLastError());
+  if (tm) tm->mark(L.stream, "grad32");
+  if (L.nblk > 0) {
+    hipLaunchKernelGGL(k_grad32, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+    if (hipGetLastError() != hipSuccess) return A2C_EHIP;
   }
-  if (timed) tm->mark(c->stream, "final_sum");
-  hipLaunchKernelGGL(k_final_sum32, dim3(1), dim3(1024), 0, c->stream, c->dpart, nb, dF);
-  A2C_HIP(c, hipGetLastError());
-  if (timed) tm->mark(c->stream, nullptr);
+  if (tm) tm->mark(L.stream, "final_sum");
+  if (int rc = run_final_sum(L)) return rc;
+  if (tm) tm->mark(L.stream, nullptr);
   return A2C_OK;
 }
 

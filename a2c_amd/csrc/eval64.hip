@@ -1,17 +1,18 @@
-// FP64 objective + gradient: the plain single-instance kernel of
+// FP64 objective + gradient: the plain kernel of
 // Minimizer::formFunctionAndGradient (minimizer.cpp:83-369).
 //
 //   k_density64  phase I  (:140-157; electrondensity.cpp:12-56): rho per
 //                site, then F(rho), F'(rho) once per site.
 //   k_grad64     phase II (:162-351): pair terms and gradient per site, the
 //                per-site objective share, and a fixed-shape block reduction.
-//   k_final_sum  deterministic sum of the block partials -> f.
+//   k_final_sum  per lattice, a fixed-order sum of its block partials -> f.
 //
 // One lane per site over ELL rows: a wave holds 64 consecutive sites of one
 // species, reads slot t of their rows as one coalesced load, and keeps the
 // per-site sums in registers (no atomics, so results are bitwise
-// reproducible).  The 12-point rule of quadrature.cpp:43-75 is used in its
-// structured form: point (c, +-) of site i displaces r along axis c by
+// reproducible).  Blocks map to lattices through b2l, so one launch serves a
+// batch of uneven lattices.  The 12-point rule of quadrature.cpp:43-75 is used
+// in its structured form: point (c, +-) of site i displaces r along axis c by
 // +-sqrt(6) sigma_i, point (c, +-) of site j by -+sqrt(6) sigma_j.
 #include <cmath>
 
@@ -21,8 +22,8 @@
 namespace a2c {
 namespace {
 
-constexpr int TB = 256;       // threads per block (reduction shape is fixed)
-constexpr double QW = 1.0 / 12.0;  // 0.5/dim, quadrature.cpp:39
+constexpr int TB = SITE_BLOCK;
+constexpr double QW = 1.0 / 12.0;             // 0.5/dim, quadrature.cpp:39
 constexpr double SQ6 = 2.449489742783178098;  // sqrt(2)*sqrt(dim/2)
 
 struct Site {
@@ -36,7 +37,7 @@ __device__ __forceinline__ Site load_site(const double* __restrict__ X, int s) {
 }
 
 // d = q_i - image(q_j): periodic axes shift q_j by L*rint((q_j - q_i)/L).
-__device__ __forceinline__ void pair_vec(const DevSys& S, const Site& a, const Site& b, double d[3]) {
+__device__ __forceinline__ void pair_vec(const LatDev& S, const Site& a, const Site& b, double d[3]) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double qj = b.q[c];
@@ -65,12 +66,14 @@ __device__ __forceinline__ void point_r(int p, const double d[3], double ai, dou
 
 // ---------------------------------------------------------------- density
 template <int DM>
-__global__ __launch_bounds__(TB) void k_density64(DevSys S, const double* __restrict__ X,
-                                                  const double* __restrict__ xo, double* __restrict__ rho_out,
-                                                  double* __restrict__ Fp, double* __restrict__ emb, double eps) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(TB) void k_density64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+                                                  double eps) {
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
+  const double* __restrict__ X = S.X;
+  const double* __restrict__ xo = S.x;
   const bool pd = s < S.npd;
   const int row = pd ? s : s - S.npd;
   const Ell LP = pd ? S.pdpd : S.hpd;
@@ -110,10 +113,10 @@ __global__ __launch_bounds__(TB) void k_density64(DevSys S, const double* __rest
     }
     rho += xo[h] * acc * QW;
   }
-  rho_out[s] = rho;
-  double F = pd ? eam::F_Pd(rho) : eam::F_H(rho);
-  Fp[s] = dF<DM>(pd, rho, eps);
-  emb[s] = pd ? F : xo[row] * F;
+  S.rho[s] = rho;
+  const double F = pd ? eam::F_Pd(rho) : eam::F_H(rho);
+  S.Fp[s] = dF<DM>(pd, rho, eps);
+  S.emb[s] = pd ? F : xo[row] * F;
 }
 
 // ---------------------------------------------------------------- gradient
@@ -187,15 +190,18 @@ __device__ __forceinline__ void pair_accum(const double d[3], double ai, double 
 }
 
 template <int DM>
-__global__ __launch_bounds__(TB) void k_grad64(DevSys S, const double* __restrict__ X,
-                                               const double* __restrict__ xo, const double* __restrict__ Fp,
-                                               const double* __restrict__ emb, double* __restrict__ G,
-                                               double* __restrict__ part, double eps) {
+__global__ __launch_bounds__(TB) void k_grad64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+                                               double eps) {
   __shared__ double red[TB];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int lb = blockIdx.x - S.blk0;
+  const int s = lb * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   double fs = 0.0;
   if (s < N) {
+    const double* __restrict__ X = S.X;
+    const double* __restrict__ xo = S.x;
+    const double* __restrict__ Fp = S.Fp;
     const bool pd = s < S.npd;
     const int row = pd ? s : s - S.npd;
     const Ell LP = pd ? S.pdpd : S.hpd;
@@ -229,9 +235,10 @@ __global__ __launch_bounds__(TB) void k_grad64(DevSys S, const double* __restric
       gs += xh * gs1;
     }
     // V share of site s: embedding + half of every pair (minimizer.cpp:169-286)
-    const double V = emb[s] + 0.5 * QW * xi * (ePd + eH);
+    const double V = S.emb[s] + 0.5 * QW * xi * (ePd + eH);
     const double T = S.T;
     const double sc = QW * xi / T;
+    double* __restrict__ G = S.G;
     G[4 * (size_t)s + 0] = sc * g[0];
     G[4 * (size_t)s + 1] = sc * g[1];
     G[4 * (size_t)s + 2] = sc * g[2];
@@ -245,61 +252,106 @@ __global__ __launch_bounds__(TB) void k_grad64(DevSys S, const double* __restric
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) S.part[lb] = red[0];
 }
 
-// Single block: fixed-order sum of the block partials.
-__global__ __launch_bounds__(1024) void k_final_sum(const double* __restrict__ part, int n, double* __restrict__ out) {
+}  // namespace
+
+// One block per lattice: fixed-order sum of its block partials.
+__global__ __launch_bounds__(1024) void k_final_sum(const LatDev* __restrict__ lats) {
   __shared__ double red[1024];
+  const LatDev& S = lats[blockIdx.x];
+  const int n = site_blocks(S.npd + S.nh);
   double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) acc += part[i];
+  for (int i = threadIdx.x; i < n; i += 1024) acc += S.part[i];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) *S.f = red[0];
 }
 
-}  // namespace
+int run_density64(const Launch& L) {
+  if (L.nblk == 0) return A2C_OK;
+  if (L.deriv_mode == A2C_DERIV_FD)
+    hipLaunchKernelGGL(k_density64<A2C_DERIV_FD>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.fd_eps);
+  else
+    hipLaunchKernelGGL(k_density64<A2C_DERIV_ANALYTIC>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l,
+                       L.fd_eps);
+  return hipGetLastError() == hipSuccess ? A2C_OK : A2C_EHIP;
+}
+
+int run_eval64(const Launch& L, Timing* tm) {
+  if (tm) tm->mark(L.stream, "density64");
+  if (int rc = run_density64(L)) return rc;
+  if (tm) tm->mark(L.stream, "grad64");
+  if (L.nblk > 0) {
+    if (L.deriv_mode == A2C_DERIV_FD)
+      hipLaunchKernelGGL(k_grad64<A2C_DERIV_FD>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.fd_eps);
+    else
+      hipLaunchKernelGGL(k_grad64<A2C_DERIV_ANALYTIC>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l,
+                         L.fd_eps);
+    if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+  }
+  if (tm) tm->mark(L.stream, "final_sum");
+  if (int rc = run_final_sum(L)) return rc;
+  if (tm) tm->mark(L.stream, nullptr);
+  return A2C_OK;
+}
+
+int run_final_sum(const Launch& L) {
+  if (L.nlat == 0) return A2C_OK;
+  hipLaunchKernelGGL(k_final_sum, dim3(L.nlat), dim3(1024), 0, L.stream, L.lats);
+  return hipGetLastError() == hipSuccess ? A2C_OK : A2C_EHIP;
+}
+
+static Launch single_launch(a2c_fe_ctx* c) {
+  Launch L;
+  L.lats = c->dlat;
+  L.b2l = c->db2l;
+  L.nblk = site_blocks(c->N);
+  L.nlat = 1;
+  L.stream = c->stream;
+  L.deriv_mode = c->desc.deriv_mode;
+  L.fd_eps = c->desc.fd_eps;
+  L.tab = c->dtab;
+  return L;
+}
 
 int launch_density64(a2c_fe_ctx* c, const double* dX) {
-  const int N = c->N;
-  if (N == 0) return A2C_OK;
-  const DevSys S = c->sys();
-  dim3 grid((N + TB - 1) / TB);
-  if (c->desc.deriv_mode == A2C_DERIV_FD)
-    hipLaunchKernelGGL(k_density64<A2C_DERIV_FD>, grid, dim3(TB), 0, c->stream, S, dX, c->dx, c->drho, c->dFp,
-                       c->demb, c->desc.fd_eps);
-  else
-    hipLaunchKernelGGL(k_density64<A2C_DERIV_ANALYTIC>, grid, dim3(TB), 0, c->stream, S, dX, c->dx, c->drho,
-                       c->dFp, c->demb, c->desc.fd_eps);
-  A2C_HIP(c, hipGetLastError());
+  if (int rc = sync_lattice(c, dX, c->dG, c->dF)) return rc;
+  if (run_density64(single_launch(c))) return set_err(c, A2C_EHIP, "density64 launch failed");
   return A2C_OK;
 }
 
 int launch_eval64(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
-  const int N = c->N;
-  const DevSys S = c->sys();
-  const int nb = (N + TB - 1) / TB;
-  if (timed) tm->mark(c->stream, "density64");
-  int rc = launch_density64(c, dX);
-  if (rc) return rc;
-  if (timed) tm->mark(c->stream, "grad64");
-  if (N > 0) {
-    if (c->desc.deriv_mode == A2C_DERIV_FD)
-      hipLaunchKernelGGL(k_grad64<A2C_DERIV_FD>, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, c->dFp, c->demb,
-                         dG, c->dpart, c->desc.fd_eps);
-    else
-      hipLaunchKernelGGL(k_grad64<A2C_DERIV_ANALYTIC>, dim3(nb), dim3(TB), 0, c->stream, S, dX, c->dx, c->dFp,
-                         c->demb, dG, c->dpart, c->desc.fd_eps);
-    A2C_HIP(c, hipGetLastError());
-  }
-  if (timed) tm->mark(c->stream, "final_sum");
-  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(1024), 0, c->stream, c->dpart, nb, dF);
-  A2C_HIP(c, hipGetLastError());
-  if (timed) tm->mark(c->stream, nullptr);
+  if (int rc = sync_lattice(c, dX, dG, dF)) return rc;
+  if (run_eval64(single_launch(c), timed ? tm : nullptr)) return set_err(c, A2C_EHIP, "eval64 launch failed");
+  return A2C_OK;
+}
+
+int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
+  if (!c->dtab) return set_err(c, A2C_ESTATE, "tables not built");
+  if (int rc = sync_lattice(c, dX, dG, dF)) return rc;
+  if (run_eval32(single_launch(c), timed ? tm : nullptr)) return set_err(c, A2C_EHIP, "eval32 launch failed");
+  return A2C_OK;
+}
+
+// Refresh the device descriptor of the context's lattice when the state,
+// gradient or objective buffers (or the neighbor lists) changed.
+int sync_lattice(a2c_fe_ctx* c, const double* dX, double* dG, double* dF) {
+  LatDev& h = *c->hlat;
+  if (!c->lat_dirty && h.X == dX && h.G == dG && h.f == dF) return A2C_OK;
+  A2C_HIP(c, hipStreamSynchronize(c->stream));  // the pinned copy may still be in flight
+  h = c->sys();
+  h.blk0 = 0;
+  h.X = dX;
+  h.G = dG;
+  h.f = dF;
+  A2C_HIP(c, hipMemcpyAsync(c->dlat, c->hlat, sizeof(LatDev), hipMemcpyHostToDevice, c->stream));
+  c->lat_dirty = false;
   return A2C_OK;
 }
 

@@ -23,14 +23,31 @@ struct Ell {
   int32_t* cnt = nullptr;  // rows
 };
 
-// Device view of one lattice: what every eval kernel needs.
-struct DevSys {
-  int32_t npd, nh;
-  double box[3];
-  int32_t periodic[3];
-  double T;
+// Device view of one lattice: what every kernel needs.  The eval kernels
+// read it from device memory through a block -> lattice map, so one launch
+// covers any set of lattices (a batch) and a single lattice is a batch of one.
+// Each lattice's sites are split into blocks of 256 starting at its site 0 in
+// every launch, so its results do not depend on the other lattices present.
+struct LatDev {
+  int32_t npd = 0, nh = 0;
+  int32_t periodic[3] = {0, 0, 0};
+  int32_t blk0 = 0;          // first block of this lattice in the current launch
+  double box[3] = {0, 0, 0};
+  double T = 0;
   Ell pdpd, pdh, hpd, hh, hh1;
+  const double* X = nullptr;  // 4N state (32-byte aligned)
+  double* G = nullptr;        // 4N gradient
+  const double* x = nullptr;  // nh occupancy
+  double* rho = nullptr;      // N densities
+  double* Fp = nullptr;       // N F'(rho) (FP64 path)
+  float2* aux = nullptr;      // N {F'(rho), x} (mixed path)
+  double* emb = nullptr;      // N embedding energy (x-weighted for H)
+  double* part = nullptr;     // ceil(N/256) block partial sums of f
+  double* f = nullptr;        // objective
 };
+using DevSys = LatDev;
+constexpr int SITE_BLOCK = 256;  // sites per block of every site-parallel kernel
+__host__ __device__ inline int site_blocks(int n) { return (n + SITE_BLOCK - 1) / SITE_BLOCK; }
 
 struct Timing;
 
@@ -64,6 +81,12 @@ struct a2c_fe_ctx {
   float4* dtab = nullptr;    // cubic coefficient tables (5 x 500 intervals)
   float2* daux = nullptr;    // per site {F'(rho), x} in FP32
 
+  // launch descriptors: this context is a batch of one lattice
+  a2c::LatDev* hlat = nullptr;   // pinned host copy
+  a2c::LatDev* dlat = nullptr;   // device copy
+  int32_t* db2l = nullptr;       // block -> lattice (all zero)
+  bool lat_dirty = true;
+
   // host staging (pinned)
   double* hX = nullptr;
   double* hG = nullptr;
@@ -75,6 +98,7 @@ struct a2c_fe_ctx {
     for (int c = 0; c < 3; ++c) { s.box[c] = desc.box[c]; s.periodic[c] = desc.periodic[c]; }
     s.T = desc.temperature;
     s.pdpd = ell[0]; s.pdh = ell[1]; s.hpd = ell[2]; s.hh = ell[3]; s.hh1 = ell[4];
+    s.x = dx; s.rho = drho; s.Fp = dFp; s.aux = daux; s.emb = demb; s.part = dpart;
     return s;
   }
 };
@@ -92,11 +116,30 @@ int set_err(a2c_fe_ctx* c, int code, const std::string& msg);
 
 // kernels (launchers) ---------------------------------------------------------
 int nbr_build(a2c_fe_ctx* c);                           // nbr.hip
+// Multi-lattice launches: lats/b2l on the device, nblk = total site blocks,
+// nlat = lattices.  The FP64 path fills rho, Fp, emb, G, part, f; the mixed
+// path rho, aux, emb, G, part, f.
+struct Launch {
+  const LatDev* lats;
+  const int32_t* b2l;
+  int nblk, nlat;
+  hipStream_t stream;
+  int deriv_mode;
+  double fd_eps;
+  const float4* tab;  // mixed path
+};
+int run_density64(const Launch& L);
+int run_eval64(const Launch& L, Timing* tm);
+int run_eval32(const Launch& L, Timing* tm);
+int run_final_sum(const Launch& L);
+int sync_lattice(a2c_fe_ctx* c, const double* dX, double* dG, double* dF);  // refresh the one-lattice descriptor
 int launch_eval64(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm);
 int launch_density64(a2c_fe_ctx* c, const double* dX);  // fills drho, dFp, demb
 int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm);
 int build_tables(a2c_fe_ctx* c);                        // eval32.hip
 int tables_download(a2c_fe_ctx* c, float* out);         // 5*500 float4
+void host_tables(std::vector<float4>& t);              // FP32 cubic tables (host)
+int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, double* Xout, a2c_lmvm_report* rep);
 int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h,
                 double gamma_bd, double* dgam, int32_t* dfull, double* dcorr);  // occupancy.hip
 int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb,
