@@ -88,6 +88,22 @@ class BatchDesc(C.Structure):
     ]
 
 
+class DmdParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("nu", C.c_double), ("Qm", C.c_double), ("xLb", C.c_double), ("xUb", C.c_double),
+        ("gamma_bd", C.c_double), ("coe_pd", C.c_double * 4), ("coe_h", C.c_double * 4),
+        ("rebuild_neighbors", C.c_int32), ("lmvm", LmvmOpts),
+    ]
+
+
+class DmdReport(C.Structure):
+    _fields_ = [
+        ("f", C.c_double), ("lmvm_reason", C.c_int32), ("lmvm_iterations", C.c_int32), ("lmvm_evals", C.c_int32),
+        ("total_h_before", C.c_double), ("total_h_after", C.c_double), ("xH", C.c_double),
+        ("xH_interior", C.c_double),
+    ]
+
+
 class LatticeDesc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("a", C.c_double),
@@ -135,6 +151,12 @@ def _load():
         "a2c_lattice_fcc_pdh": (I, [P(LatticeDesc), dp, dp, dp]),
         "a2c_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "a2c_eam_tables": (I, [P(C.c_float), C.c_int32]),
+        "a2c_enforce_boundary_conditions": (I, [C.c_int32, ip, D, D, dp, dp, ip]),
+        "a2c_macro_results": (I, [C.c_int32, dp, C.c_int32, ip, dp, dp]),
+        "a2c_read_results": (I, [C.c_char_p, C.c_int32, C.c_int32, D, D, dp, dp, dp, ip]),
+        "a2c_write_results": (I, [C.c_char_p, C.c_int32, C.c_int32, dp, dp, dp, ip, D, C.c_int32]),
+        "a2c_dmd_params_default": (None, [P(DmdParams)]),
+        "a2c_dmd_step": (I, [VP, P(DmdParams), C.c_int32, ip, C.c_int32, ip, dp, dp, dp, ip, P(DmdReport)]),
         "a2c_batch_desc_default": (None, [P(BatchDesc)]),
         "a2c_batch_create": (I, [P(BatchDesc), P(VP)]),
         "a2c_batch_destroy": (None, [VP]),
@@ -378,6 +400,78 @@ class FreeEntropy:
         return {"kernels_ms": kern, "total_ms": t.total_ms}
 
 
+# --------------------------------------------------------------------------
+# DMD helpers (include/a2c_dmd.h)
+# --------------------------------------------------------------------------
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def enforceBoundaryConditions(h_subsurf, gamma_bd, xUb, x0, gamma, full_h):
+    """boundaryconditions.cpp:30-55; updates x0, gamma, full_h in place."""
+    sub = _i32(h_subsurf)
+    assert x0.dtype == np.float64 and gamma.dtype == np.float64 and full_h.dtype == np.int32
+    rc = lib.a2c_enforce_boundary_conditions(len(sub), _iptr(sub), gamma_bd, xUb, _dptr(x0), _dptr(gamma),
+                                             _iptr(full_h))
+    if rc:
+        raise A2CError(rc, "enforceBoundaryConditions")
+
+
+def calculateMacroResults(x, h_surf=()):
+    """calculateresults.cpp:8-78 -> (mean H fraction, interior mean)."""
+    x = _f64(x)
+    surf = _i32(h_surf)
+    a, b = C.c_double(), C.c_double()
+    rc = lib.a2c_macro_results(len(x), _dptr(x), len(surf), _iptr(surf), C.byref(a), C.byref(b))
+    if rc:
+        raise A2CError(rc, "calculateMacroResults")
+    return a.value, b.value
+
+
+def readResults(path, n_pd, n_h, xLb=1e-3, xUb=1 - 1e-16):
+    """readresults.cpp:10-55 -> (X, x, gamma, full_h)."""
+    X, x, g = np.zeros(4 * (n_pd + n_h)), np.zeros(max(n_h, 1)), np.zeros(max(n_h, 1))
+    fh = np.zeros(max(n_h, 1), dtype=np.int32)
+    rc = lib.a2c_read_results(path.encode(), n_pd, n_h, xLb, xUb, _dptr(X), _dptr(x), _dptr(g), _iptr(fh))
+    if rc:
+        raise A2CError(rc, "cannot read restart file %s" % path)
+    return X, x[:n_h], g[:n_h], fh[:n_h]
+
+
+def writeResults(path, n_pd, n_h, X, x, gamma, full_h, time=0.0, step=0):
+    X, x, g = _f64(X), _f64(x), _f64(gamma)
+    fh = _i32(full_h)
+    rc = lib.a2c_write_results(path.encode(), n_pd, n_h, _dptr(X), _dptr(x), _dptr(g), _iptr(fh), time, step)
+    if rc:
+        raise A2CError(rc, "cannot write restart file %s" % path)
+
+
+def dmd_params(**kw):
+    p = DmdParams()
+    lib.a2c_dmd_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "lmvm":
+            for kk, vv in v.items():
+                setattr(p.lmvm, kk, vv)
+        elif k in ("coe_pd", "coe_h"):
+            for i in range(4):
+                getattr(p, k)[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def dmd_step(fe, params, X, x, gamma, full_h, h_subsurf=(), h_surf=()):
+    """One DMD step (include/a2c_dmd.h); X, x, gamma, full_h updated in place."""
+    sub, surf = _i32(h_subsurf), _i32(h_surf)
+    rep = DmdReport()
+    rc = lib.a2c_dmd_step(fe._ctx, C.byref(params), len(sub), _iptr(sub), len(surf), _iptr(surf), _dptr(X),
+                          _dptr(x), _dptr(gamma), _iptr(full_h), C.byref(rep))
+    fe._check(rc)
+    fe._x = x.copy()
+    return {k: getattr(rep, k) for k, _ in DmdReport._fields_}
+
+
 def lmvm_opts(**kw):
     o = LmvmOpts()
     lib.a2c_lmvm_opts_default(C.byref(o))
@@ -495,7 +589,7 @@ def exported_symbols():
     import re
     root = os.path.dirname(_HERE)
     names = []
-    for h in ("a2c_fe.h", "a2c_lattice.h", "a2c_batch.h"):
+    for h in ("a2c_fe.h", "a2c_lattice.h", "a2c_batch.h", "a2c_dmd.h"):
         txt = open(os.path.join(root, "include", h)).read()
         names += re.findall(r"\b(a2c_\w+)\s*\(", txt)
     return sorted(set(names))

@@ -1,0 +1,138 @@
+"""Continuous-batching relaxation server on one GPU.
+
+Requests (a lattice to relax) arrive as a Poisson stream.  The loop keeps
+every admitted lattice resident in HBM in one a2c_batch and advances all
+running relaxations together, one LMVM iteration per tick (one fused launch
+per evaluation round for all of them).  Between ticks it admits the requests
+that have arrived, as long as their device footprint fits the HBM budget, and
+retires the converged ones.  Latency = finish - arrival.
+
+Workload (defaults): edge n log-uniform in [6, 48] cells (n^3 cells, 8 sites
+per cell), T ~ U(300, 700) K, mean H fraction ~ U(0, 0.6) with iid noise
++-0.05, displacements N(0, 0.02 A), sigma 0.10 A, periodic, seed 4 + request
+id.  The comparator relaxes the same requests one at a time in FP64.
+
+    python -m a2c_amd.serve --rates 1,5,20 --requests 24
+"""
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import Batch, FreeEntropy, lattice
+
+
+def make_requests(n, rate, seed=4, nmin=6, nmax=48):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    reqs = []
+    for i in range(n):
+        r = np.random.default_rng(seed + 1000 + i)
+        edge = int(round(math.exp(r.uniform(math.log(nmin), math.log(nmax)))))
+        T = float(r.uniform(300.0, 700.0))
+        xbar = float(r.uniform(0.0, 0.6))
+        reqs.append({"id": i, "arrival": float(t[i]), "n": edge, "T": T, "xbar": xbar, "seed": seed + i})
+    return reqs
+
+
+def materialize(req):
+    lo, hi = max(req["xbar"] - 0.05, 1e-3), min(req["xbar"] + 0.05, 1 - 1e-16)
+    X, x, box, npd, nh = lattice(req["n"], seed=req["seed"], disp_std=0.02, sigma=(0.10, 0.10), x=(lo, hi),
+                                 x_clip=(1e-3, 1 - 1e-16))
+    return X, x, box, npd, nh
+
+
+def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None):
+    opts = opts or {"gatol": 1e-8, "grtol": 0.0}
+    pending = list(reqs)
+    running = {}  # slot -> request
+    done = []
+    t0 = time.perf_counter()
+    with Batch(precision=precision, max_lattices=max_lattices, hbm_budget_bytes=budget) as B:
+        while pending or running:
+            now = time.perf_counter() - t0
+            # admit arrived requests while the footprint fits
+            while pending and pending[0]["arrival"] <= now:
+                req = pending[0]
+                n_sites = 8 * req["n"] ** 3
+                if B.mem_used() + Batch.footprint(n_sites // 2, n_sites // 2) > budget and running:
+                    break
+                X, x, box, npd, nh = materialize(req)
+                slot = B.add(X, x, npd, nh, box, (1, 1, 1), T=req["T"])
+                B.start(slot, **opts)
+                req["admitted"] = time.perf_counter() - t0
+                running[slot] = req
+                pending.pop(0)
+            if not running:
+                time.sleep(max(0.0, pending[0]["arrival"] - (time.perf_counter() - t0)))
+                continue
+            B.step(1)
+            for slot in list(running):
+                rep = B.report(slot)
+                if rep["reason"] != 0:
+                    req = running.pop(slot)
+                    req["finish"] = time.perf_counter() - t0
+                    req["latency"] = req["finish"] - req["arrival"]
+                    req["report"] = rep
+                    done.append(req)
+                    B.remove(slot)
+                    if log:
+                        log(req)
+    wall = time.perf_counter() - t0
+    return done, wall
+
+
+def serve_one_at_a_time(reqs, precision="fp64", opts=None):
+    """Comparator: requests in arrival order, one lattice at a time."""
+    opts = opts or {"gatol": 1e-8, "grtol": 0.0}
+    t0 = time.perf_counter()
+    done = []
+    for req in reqs:
+        now = time.perf_counter() - t0
+        if req["arrival"] > now:
+            time.sleep(req["arrival"] - now)
+        X, x, box, npd, nh = materialize(req)
+        with FreeEntropy(npd, nh, box, (1, 1, 1), T=req["T"], precision=precision) as fe:
+            fe.set_occupancy(x)
+            fe.findNeighbors(X)
+            _, rep = fe.minimizeFreeEntropy(X, **opts)
+        r = dict(req)
+        r["finish"] = time.perf_counter() - t0
+        r["latency"] = r["finish"] - r["arrival"]
+        r["report"] = rep
+        done.append(r)
+    return done, time.perf_counter() - t0
+
+
+def summarize(done, wall):
+    lat = np.array([d["latency"] for d in done])
+    conv = sum(1 for d in done if d["report"]["reason"] > 0)
+    return {"requests": len(done), "converged": conv, "wall_s": wall, "relaxations_per_s": len(done) / wall,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "sites": int(sum(8 * d["n"] ** 3 for d in done))}
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--rates", default="1,5,20")
+    p.add_argument("--requests", type=int, default=24)
+    p.add_argument("--nmax", type=int, default=48)
+    p.add_argument("--no-baseline", action="store_true")
+    a = p.parse_args()
+    out = []
+    for rate in [float(r) for r in a.rates.split(",")]:
+        reqs = make_requests(a.requests, rate, nmax=a.nmax)
+        done, wall = serve([dict(r) for r in reqs])
+        row = {"rate": rate, "batched_mixed": summarize(done, wall)}
+        if not a.no_baseline:
+            d1, w1 = serve_one_at_a_time([dict(r) for r in reqs])
+            row["one_at_a_time_fp64"] = summarize(d1, w1)
+        print(json.dumps(row), flush=True)
+        out.append(row)
+    return out
+
+
+if __name__ == "__main__":
+    main()

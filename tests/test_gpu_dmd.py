@@ -1,0 +1,41 @@
+"""DMD steps on the GPU: each step's chemical potential and diffusion update
+against the oracle from the same relaxed state; total H conserved to round-off
+when no reservoir or clamp acts; slab initial condition of the coupled
+workload (occupancy 0.6 in the first cell layers, xLb elsewhere)."""
+import numpy as np
+import pytest
+
+import a2c_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dmd_steps_conserve_and_match_oracle():
+    X, x, box, npd, nh = A.lattice((8, 4, 4), seed=5, disp_std=0.02, x=(1e-3, 1e-3), slab_cells=2, slab_x=0.6)
+    T = 400.0
+    fe = A.FreeEntropy(npd, nh, box, (1, 1, 1), T=T, precision="mixed")
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    prm = A.dmd_params(dt=1e-13, nu=1e13, Qm=0.23, xLb=1e-3, xUb=1 - 1e-16, gamma_bd=1e300,
+                       lmvm={"gatol": 1e-8, "grtol": 0.0})
+    g = np.zeros(nh)
+    fh = np.zeros(nh, np.int32)
+    orc = O.Oracle(npd, nh, box, (1, 1, 1), T=T, threads=8)
+    for step in range(3):
+        x_before = x.copy()
+        rep = A.dmd_step(fe, prm, X, x, g, fh)
+        assert rep["lmvm_reason"] > 0
+        # oracle from the relaxed state of this step
+        orc.build_neighbors(X)
+        go, fo = orc.chem_potential(X, x_before, np.zeros(4), np.zeros(4), 1e300, np.zeros(nh), np.zeros(nh, np.int32))
+        assert np.max(np.abs(g - go) / np.maximum(np.abs(go), 1.0)) <= 1e-10
+        xo = orc.diffuse(1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16, go, fo, x_before)
+        assert np.max(np.abs(x - xo)) <= 1e-13
+        assert rep["total_h_before"] == pytest.approx(x_before.sum(), rel=1e-15)
+        clamped = np.any(x <= 1e-3) and np.any(x_before > 1e-3)
+        if not clamped:
+            assert abs(rep["total_h_after"] - rep["total_h_before"]) <= 1e-12 * rep["total_h_before"]
+    # H moved out of the slab
+    ix = (np.arange(nh) // 4) % 8
+    assert x[ix >= 2].sum() > (ix >= 2).sum() * 1e-3
