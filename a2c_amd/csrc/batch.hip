@@ -394,6 +394,7 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   L.deriv_mode = b->desc.deriv_mode;
   L.fd_eps = b->desc.fd_eps;
   L.tab = b->dtab;
+  L.mixed_full = mixed_full_pairs();
   int rc = fp64 ? run_eval64(L, nullptr) : run_eval32(L, nullptr);
   if (rc) return berr(b, rc, "fused eval launch failed");
   if (!sync) return A2C_OK;

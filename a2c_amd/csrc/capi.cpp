@@ -107,7 +107,11 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   hipFree(c->dX); hipFree(c->dG); hipFree(c->dx); hipFree(c->drho); hipFree(c->dFp);
   hipFree(c->demb); hipFree(c->dpart); hipFree(c->dF); hipFree(c->dpdpdnum); hipFree(c->dtab); hipFree(c->daux);
-  for (int w = 0; w < 5; ++w) { hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt); }
+  for (int w = 0; w < 5; ++w) {
+    hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt);
+    hipFree(c->ell[w].rev); hipFree(c->ell[w].rb1); hipFree(c->ell[w].rb4);
+  }
+  hipFree(c->depair);
   if (c->hX) hipHostFree(c->hX);
   if (c->hG) hipHostFree(c->hG);
   if (c->hlat) hipHostFree(c->hlat);
@@ -127,6 +131,8 @@ int a2c_fe_set_precision(a2c_fe_ctx* c, int32_t precision) {
     c->lat_dirty = true;
   }
   c->desc.precision = precision;
+  if (precision == A2C_PREC_MIXED)
+    if (int rc = ensure_pair_buffers(c)) return rc;
   return A2C_OK;
 }
 

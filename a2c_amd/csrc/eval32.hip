@@ -287,6 +287,261 @@ __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, 
   if (threadIdx.x == 0) S.part[lb] = red[0];
 }
 
+// ================================================================ pair-once
+// Each unordered pair {s, j} is evaluated once, by its owner:
+//   own(s, j) = (s < j) XOR ((s ^ j) & 1),
+// which gives every site about half of each of its lists.  The owner adds its
+// own share and writes the partner's share into the partner's list entry
+// (reverse slot); a gather kernel adds the received shares in FP64.  Each
+// buffer entry is written by exactly one lane, so results are deterministic.
+__device__ __forceinline__ bool owns(int s, int j) { return (s < j) != (((s ^ j) & 1) != 0); }
+
+// 12-point sums of two density tables at the same points
+__device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb, const float d[3], float ai,
+                                              float aj, float& sa, float& sb) {
+  const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float bi = D2 + ai * ai, bj = D2 + aj * aj;
+  sa = 0.0f;
+  sb = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float ti = 2.0f * ai * d[c], tj = 2.0f * aj * d[c];
+    const float r2[4] = {bi + ti, bi - ti, bj - tj, bj + tj};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = fmaxf(r2[q], 1e-12f);
+      const float r = x * rsq(x);
+      sa += tab_v(tab, wa, r);
+      if (wb != wa) sb += tab_v(tab, wb, r);
+    }
+  }
+  if (wb == wa) sb = sa;
+}
+
+// Partner list of each list: PdPd<->PdPd, PdH<->HPd, HH<->HH.
+__device__ __forceinline__ const Ell& partner_of(const LatDev& S, int list) {
+  return list == 0 ? S.pdpd : (list == 1 ? S.hpd : (list == 2 ? S.pdh : S.hh));
+}
+
+__global__ __launch_bounds__(TB) void k_dens_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+                                                 const float4* __restrict__ gtab) {
+  __shared__ float4 tab[2 * NT];  // f_Pd, f_H
+  stage_tables(tab, gtab, 2 * NT);
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const double* __restrict__ X = S.X;
+  const double* __restrict__ xo = S.x;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  const Site a = load_site(X, s);
+  const float ai = SQ6F * (float)a.s;
+  const int mytab = pd ? T_FPD : T_FH;
+  double rho = 0.0;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {  // Pd neighbors, then H neighbors
+    const int list = pd ? side : 2 + side;
+    const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
+    const Ell& P = partner_of(S, list);
+    const int n = L.cnt[row];
+    for (int t = 0; t < n; ++t) {
+      const size_t e = (size_t)t * L.rows + row;
+      const int jl = L.idx[e];
+      const int j = side == 0 ? jl : S.npd + jl;
+      if (!owns(s, j)) continue;
+      const Site b = load_site(X, j);
+      float d[3];
+      pair_vec32(S, a, b, d);
+      float mine, theirs;  // neighbor-type density (for me), my-type density (for the neighbor)
+      pair_density2(tab, side == 0 ? T_FPD : T_FH, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
+      rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
+      const int rv = L.rev[e];
+      if (rv >= 0) P.rb1[(size_t)rv * P.rows + jl] = theirs;
+    }
+  }
+  S.rho[s] = rho;
+}
+
+__global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const double* __restrict__ xo = S.x;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  double rho = S.rho[s];
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int list = pd ? side : 2 + side;
+    const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
+    const int n = L.cnt[row];
+    for (int t = 0; t < n; ++t) {
+      const size_t e = (size_t)t * L.rows + row;
+      const int jl = L.idx[e];
+      const int j = side == 0 ? jl : S.npd + jl;
+      if (owns(s, j)) continue;
+      rho += (side == 0 ? 1.0 : xo[jl]) * (double)L.rb1[e];
+    }
+  }
+  rho *= QW;
+  S.rho[s] = rho;
+  const double xs = pd ? 1.0 : xo[row];
+  const eam::VD F = pd ? eam::F_Pd_vd(rho) : eam::F_H_vd(rho);
+  S.aux[s] = make_float2((float)F.d, (float)xs);
+  S.emb[s] = xs * F.v;
+}
+
+struct PairSums2 {
+  float pe;    // sum of phi
+  float g[3];  // sum of (c/|r|) r           (q-gradient of site s; the partner gets -g)
+  float gsi;   // site-s sigma term (x sqrt(6) QW x_s / T later)
+  float gsj;   // partner sigma term
+};
+
+template <int KIND>
+__device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d[3], float ai, float aj, float Fi,
+                                                float Fj) {
+  const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float bi = D2 + ai * ai, bj = D2 + aj * aj;
+  float su = 0.0f, ui = 0.0f, uj = 0.0f, pe = 0.0f, ti[3], tj[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float t1 = 2.0f * ai * d[c], t2 = 2.0f * aj * d[c];
+    const float r2[4] = {bi + t1, bi - t1, bj - t2, bj + t2};
+    float u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = fmaxf(r2[q], 1e-12f);
+      const float rinv = rsq(x);
+      float phi, cc;
+      point_coef<KIND>(tab, x * rinv, Fi, Fj, phi, cc);
+      pe += phi;
+      u[q] = cc * rinv;
+    }
+    ti[c] = u[0] - u[1];
+    tj[c] = u[2] - u[3];
+    ui += u[0] + u[1];
+    uj += u[2] + u[3];
+  }
+  su = ui + uj;
+  PairSums2 p;
+  p.pe = pe;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) p.g[c] = fmaf(d[c], su, ai * ti[c] - aj * tj[c]);
+  p.gsi = fmaf(d[0], ti[0], fmaf(d[1], ti[1], fmaf(d[2], ti[2], ai * ui)));
+  p.gsj = fmaf(aj, uj, -fmaf(d[0], tj[0], fmaf(d[1], tj[1], d[2] * tj[2])));
+  return p;
+}
+
+__global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+                                                 const float4* __restrict__ gtab) {
+  __shared__ float4 tab[NTAB * NT];
+  stage_tables(tab, gtab, NTAB * NT);
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const double* __restrict__ X = S.X;
+  const double* __restrict__ xo = S.x;
+  const float2* __restrict__ aux = S.aux;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  const Site a = load_site(X, s);
+  const float ai = SQ6F * (float)a.s;
+  const float Fi = aux[s].x;
+  double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, gs = 0.0;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int list = pd ? side : 2 + side;
+    const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
+    const Ell& P = partner_of(S, list);
+    const int n = L.cnt[row];
+    for (int t = 0; t < n; ++t) {
+      const size_t ei = (size_t)t * L.rows + row;
+      const int jl = L.idx[ei];
+      const int j = side == 0 ? jl : S.npd + jl;
+      if (!owns(s, j)) continue;
+      const Site b = load_site(X, j);
+      const float Fj = aux[j].x;
+      float d[3];
+      pair_vec32(S, a, b, d);
+      const float aj = SQ6F * (float)b.s;
+      PairSums2 p;
+      if (list == 0) p = pair_grad2<0>(tab, d, ai, aj, Fi, Fj);
+      else if (list == 1) p = pair_grad2<1>(tab, d, ai, aj, Fi, Fj);
+      else if (list == 2) p = pair_grad2<2>(tab, d, ai, aj, Fi, Fj);
+      else p = pair_grad2<3>(tab, d, ai, aj, Fi, Fj);
+      const double w = side == 0 ? 1.0 : xo[jl];
+      e += w * p.pe;
+      g0 += w * p.g[0]; g1 += w * p.g[1]; g2 += w * p.g[2];
+      gs += w * p.gsi;
+      const int rv = L.rev[ei];
+      if (rv >= 0) P.rb4[(size_t)rv * P.rows + jl] = make_float4(-p.g[0], -p.g[1], -p.g[2], p.gsj);
+    }
+  }
+  double* __restrict__ G = S.G;
+  G[4 * (size_t)s + 0] = g0;
+  G[4 * (size_t)s + 1] = g1;
+  G[4 * (size_t)s + 2] = g2;
+  G[4 * (size_t)s + 3] = gs;
+  S.epair[s] = e;
+}
+
+__global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
+  __shared__ double red[TB];
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int lb = blockIdx.x - S.blk0;
+  const int s = lb * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  double fs = 0.0;
+  if (s < N) {
+    const double* __restrict__ xo = S.x;
+    const bool pd = s < S.npd;
+    const int row = pd ? s : s - S.npd;
+    double* __restrict__ G = S.G;
+    double g0 = G[4 * (size_t)s], g1 = G[4 * (size_t)s + 1], g2 = G[4 * (size_t)s + 2], gs = G[4 * (size_t)s + 3];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int list = pd ? side : 2 + side;
+      const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
+      const int n = L.cnt[row];
+      for (int t = 0; t < n; ++t) {
+        const size_t ei = (size_t)t * L.rows + row;
+        const int jl = L.idx[ei];
+        const int j = side == 0 ? jl : S.npd + jl;
+        if (owns(s, j)) continue;
+        const float4 v = L.rb4[ei];
+        const double w = side == 0 ? 1.0 : xo[jl];
+        g0 += w * v.x; g1 += w * v.y; g2 += w * v.z;
+        gs += w * v.w;
+      }
+    }
+    const double xi = pd ? 1.0 : xo[row];
+    const double sg = S.X[4 * (size_t)s + 3];
+    const double T = S.T;
+    const double V = S.emb[s] + QW * xi * S.epair[s];  // owned pairs carry their full energy
+    const double sc = QW * xi / T;
+    G[4 * (size_t)s + 0] = sc * g0;
+    G[4 * (size_t)s + 1] = sc * g1;
+    G[4 * (size_t)s + 2] = sc * g2;
+    G[4 * (size_t)s + 3] = SQ6 * sc * gs - 3.0 * KB * xi / sg;
+    fs = V / T - 3.0 * KB * xi * log(sg);
+  }
+  red[threadIdx.x] = fs;
+  __syncthreads();
+#pragma unroll
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S.part[lb] = red[0];
+}
+
+}  // namespace
+
+namespace {
 // ---------------------------------------------------------------- tables
 template <class Fn>
 void fill_table(float4* out, Fn g) {
@@ -323,14 +578,29 @@ int build_tables(a2c_fe_ctx* c) {
 }
 
 int run_eval32(const Launch& L, Timing* tm) {
-  if (tm) tm->mark(L.stream, "density32");
-  if (L.nblk > 0) {
-    hipLaunchKernelGGL(k_density32, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+  if (L.mixed_full) {  // every pair from both ends (no reverse buffers)
+    if (tm) tm->mark(L.stream, "density32");
+    if (L.nblk > 0) {
+      hipLaunchKernelGGL(k_density32, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+    }
+    if (tm) tm->mark(L.stream, "grad32");
+    if (L.nblk > 0) {
+      hipLaunchKernelGGL(k_grad32, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+    }
+  } else if (L.nblk > 0) {
+    if (tm) tm->mark(L.stream, "dens_own");
+    hipLaunchKernelGGL(k_dens_own, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
-  }
-  if (tm) tm->mark(L.stream, "grad32");
-  if (L.nblk > 0) {
-    hipLaunchKernelGGL(k_grad32, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+    if (tm) tm->mark(L.stream, "dens_recv");
+    hipLaunchKernelGGL(k_dens_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
+    if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+    if (tm) tm->mark(L.stream, "grad_own");
+    hipLaunchKernelGGL(k_grad_own, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+    if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+    if (tm) tm->mark(L.stream, "grad_recv");
+    hipLaunchKernelGGL(k_grad_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
   }
   if (tm) tm->mark(L.stream, "final_sum");

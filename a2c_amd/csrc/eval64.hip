@@ -15,6 +15,8 @@
 // in its structured form: point (c, +-) of site i displaces r along axis c by
 // +-sqrt(6) sigma_i, point (c, +-) of site j by -+sqrt(6) sigma_j.
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 #include "eam64.hpp"
 #include "fe_internal.hpp"
@@ -307,8 +309,17 @@ int run_final_sum(const Launch& L) {
   return hipGetLastError() == hipSuccess ? A2C_OK : A2C_EHIP;
 }
 
+bool mixed_full_pairs() {
+  static const bool full = [] {
+    const char* v = getenv("A2C_MIXED_PAIRS");
+    return v && std::string(v) == "full";
+  }();
+  return full;
+}
+
 static Launch single_launch(a2c_fe_ctx* c) {
   Launch L;
+  L.mixed_full = mixed_full_pairs();
   L.lats = c->dlat;
   L.b2l = c->db2l;
   L.nblk = site_blocks(c->N);

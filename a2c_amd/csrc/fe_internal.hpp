@@ -21,6 +21,12 @@ struct Ell {
   int32_t width = 0;
   int32_t* idx = nullptr;  // width * rows
   int32_t* cnt = nullptr;  // rows
+  // mixed path (each unordered pair evaluated once): slot of this entry's
+  // pair in the partner's list, and the partner-written contributions that
+  // arrive at this list's entries
+  int32_t* rev = nullptr;  // width * rows
+  float* rb1 = nullptr;    // width * rows  (density pass)
+  float4* rb4 = nullptr;   // width * rows  (gradient pass)
 };
 
 // Device view of one lattice: what every kernel needs.  The eval kernels
@@ -42,6 +48,7 @@ struct LatDev {
   double* Fp = nullptr;       // N F'(rho) (FP64 path)
   float2* aux = nullptr;      // N {F'(rho), x} (mixed path)
   double* emb = nullptr;      // N embedding energy (x-weighted for H)
+  double* epair = nullptr;    // N pair-energy partial (mixed path)
   double* part = nullptr;     // ceil(N/256) block partial sums of f
   double* f = nullptr;        // objective
 };
@@ -66,6 +73,7 @@ struct a2c_fe_ctx {
   double* drho = nullptr;    // N
   double* dFp = nullptr;     // N  F'(rho)
   double* demb = nullptr;    // N  embedding energy per site (x-weighted for H)
+  double* depair = nullptr;  // N  pair-energy partial of the mixed path
   double* dpart = nullptr;   // block partial sums
   double* dF = nullptr;      // 1 objective (device)
   int32_t nblk_red = 0;
@@ -98,7 +106,7 @@ struct a2c_fe_ctx {
     for (int c = 0; c < 3; ++c) { s.box[c] = desc.box[c]; s.periodic[c] = desc.periodic[c]; }
     s.T = desc.temperature;
     s.pdpd = ell[0]; s.pdh = ell[1]; s.hpd = ell[2]; s.hh = ell[3]; s.hh1 = ell[4];
-    s.x = dx; s.rho = drho; s.Fp = dFp; s.aux = daux; s.emb = demb; s.part = dpart;
+    s.x = dx; s.rho = drho; s.Fp = dFp; s.aux = daux; s.emb = demb; s.part = dpart; s.epair = depair;
     return s;
   }
 };
@@ -116,6 +124,7 @@ int set_err(a2c_fe_ctx* c, int code, const std::string& msg);
 
 // kernels (launchers) ---------------------------------------------------------
 int nbr_build(a2c_fe_ctx* c);                           // nbr.hip
+int ensure_pair_buffers(a2c_fe_ctx* c);                 // reverse slots + contribution buffers
 // Multi-lattice launches: lats/b2l on the device, nblk = total site blocks,
 // nlat = lattices.  The FP64 path fills rho, Fp, emb, G, part, f; the mixed
 // path rho, aux, emb, G, part, f.
@@ -127,7 +136,9 @@ struct Launch {
   int deriv_mode;
   double fd_eps;
   const float4* tab;  // mixed path
+  bool mixed_full = false;  // mixed path: every pair from both ends (A/B reference)
 };
+bool mixed_full_pairs();  // A2C_MIXED_PAIRS=full selects the both-ends kernels
 int run_density64(const Launch& L);
 int run_eval64(const Launch& L, Timing* tm);
 int run_eval32(const Launch& L, Timing* tm);

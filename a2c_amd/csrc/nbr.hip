@@ -185,6 +185,27 @@ __global__ __launch_bounds__(64) void k_nbr(NbrArgs a) {
   }
 }
 
+// Reverse slots: for entry (row r, slot t) of list L with neighbor j, the slot
+// of r in row j of the partner list P (PdPd<->PdPd, PdH<->HPd, HH<->HH).
+// Rows are sorted, so a binary search finds it; lists are symmetric.
+__global__ void k_rev(Ell L, Ell P) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L.rows) return;
+  const int n = L.cnt[r];
+  for (int t = 0; t < n; ++t) {
+    const int j = L.idx[(size_t)t * L.rows + r];
+    int lo = 0, hi = P.cnt[j] - 1, k = -1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int v = P.idx[(size_t)mid * P.rows + j];
+      if (v == r) { k = mid; break; }
+      if (v < r) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    L.rev[(size_t)t * L.rows + r] = k;
+  }
+}
+
 }  // namespace
 
 // Host driver: grid from the positions, bin, scan, count, allocate, fill.
@@ -244,6 +265,7 @@ int nbr_build(a2c_fe_ctx* c) {
   // row-length arrays (kept in the ELL structs)
   for (int w = 0; w < 5; ++w) {
     hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt);
+    hipFree(c->ell[w].rev); hipFree(c->ell[w].rb1); hipFree(c->ell[w].rb4);
     c->ell[w] = Ell{};
     c->ell[w].rows = (w < 2) ? npd : nh;
     A2C_HIP(c, hipMalloc(&c->ell[w].cnt, sizeof(int32_t) * std::max(c->ell[w].rows, 1)));
@@ -290,6 +312,36 @@ int nbr_build(a2c_fe_ctx* c) {
   c->stats.max_neib = hm[5];
   for (int w = 0; w < 5; ++w) c->stats.width[w] = hm[w];
   c->have_nbr = true;
+  if (c->desc.precision == A2C_PREC_MIXED)
+    if (int rc = ensure_pair_buffers(c)) return rc;
+  return A2C_OK;
+}
+
+// Reverse slots and contribution buffers of the four pair lists (mixed path).
+int ensure_pair_buffers(a2c_fe_ctx* c) {
+  if (!c->have_nbr) return A2C_OK;
+  static const int partner[4] = {0, 2, 1, 3};  // PdPd, PdH<->HPd, HH
+  bool fresh = false;
+  for (int w = 0; w < 4; ++w) {
+    Ell& L = c->ell[w];
+    if (L.rev) continue;
+    const size_t n = (size_t)std::max(L.width, 1) * std::max(L.rows, 1);
+    A2C_HIP(c, hipMalloc(&L.rev, sizeof(int32_t) * n));
+    A2C_HIP(c, hipMalloc(&L.rb1, sizeof(float) * n));
+    A2C_HIP(c, hipMalloc(&L.rb4, sizeof(float4) * n));
+    fresh = true;
+  }
+  if (!fresh) return A2C_OK;
+  for (int w = 0; w < 4; ++w) {
+    const Ell& L = c->ell[w];
+    if (L.rows > 0) {
+      hipLaunchKernelGGL(k_rev, dim3((L.rows + 255) / 256), dim3(256), 0, c->stream, L, c->ell[partner[w]]);
+      A2C_HIP(c, hipGetLastError());
+    }
+  }
+  if (!c->depair) A2C_HIP(c, hipMalloc(&c->depair, sizeof(double) * std::max(c->N, 1)));
+  A2C_HIP(c, hipStreamSynchronize(c->stream));
+  c->lat_dirty = true;
   return A2C_OK;
 }
 
