@@ -112,6 +112,7 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
     hipFree(c->ell[w].rev); hipFree(c->ell[w].rb1); hipFree(c->ell[w].rb4);
   }
   hipFree(c->depair);
+  hipFree(c->nb_cid); hipFree(c->nb_order); hipFree(c->nb_cc); hipFree(c->nb_cursor); hipFree(c->nb_misc);
   if (c->hX) hipHostFree(c->hX);
   if (c->hG) hipHostFree(c->hG);
   if (c->hlat) hipHostFree(c->hlat);

@@ -27,6 +27,8 @@ struct Ell {
   int32_t* rev = nullptr;  // width * rows
   float* rb1 = nullptr;    // width * rows  (density pass)
   float4* rb4 = nullptr;   // width * rows  (gradient pass)
+  int64_t cap = 0;         // allocated entries of idx (grow-only across rebuilds)
+  int64_t pcap = 0;        // allocated entries of rev/rb1/rb4
 };
 
 // Device view of one lattice: what every kernel needs.  The eval kernels
@@ -82,6 +84,13 @@ struct a2c_fe_ctx {
 
   // neighbor lists
   a2c::Ell ell[5];           // PdPd, PdH, HPd, HH, HH1
+  // neighbor-build scratch (grow-only)
+  int* nb_cid = nullptr;     // N
+  int* nb_order = nullptr;   // N
+  int* nb_cc = nullptr;      // cells + 1
+  int* nb_cursor = nullptr;  // cells + 1
+  int* nb_misc = nullptr;    // 8
+  int64_t nb_cap_cells = 0;
   int32_t* dpdpdnum = nullptr;  // nPd*3
   a2c_nbr_stats stats{};
 

@@ -233,15 +233,21 @@ int nbr_build(a2c_fe_ctx* c) {
   }
   const long long ncell = (long long)g.nc[0] * g.nc[1] * g.nc[2];
   if (ncell > (1LL << 28)) return set_err(c, A2C_EINVAL, "neighbor grid too large");
-  int *dcid = nullptr, *dcc = nullptr, *dorder = nullptr, *dmisc = nullptr;
-  A2C_HIP(c, hipMalloc(&dcid, sizeof(int) * std::max(N, 1)));
-  A2C_HIP(c, hipMalloc(&dcc, sizeof(int) * (ncell + 1)));
-  A2C_HIP(c, hipMalloc(&dorder, sizeof(int) * std::max(N, 1)));
-  A2C_HIP(c, hipMalloc(&dmisc, sizeof(int) * 8));
-  struct Free {
-    int *a, *b, *d, *e;
-    ~Free() { hipFree(a); hipFree(b); hipFree(d); hipFree(e); }
-  } fr{dcid, dcc, dorder, dmisc};
+  // scratch buffers persist across rebuilds (grow-only)
+  if (!c->nb_cid) {
+    A2C_HIP(c, hipMalloc(&c->nb_cid, sizeof(int) * std::max(N, 1)));
+    A2C_HIP(c, hipMalloc(&c->nb_order, sizeof(int) * std::max(N, 1)));
+    A2C_HIP(c, hipMalloc(&c->nb_misc, sizeof(int) * 8));
+  }
+  if (ncell + 1 > c->nb_cap_cells) {
+    (void)hipFree(c->nb_cc);
+    (void)hipFree(c->nb_cursor);
+    c->nb_cc = c->nb_cursor = nullptr;
+    A2C_HIP(c, hipMalloc(&c->nb_cc, sizeof(int) * (ncell + 1)));
+    A2C_HIP(c, hipMalloc(&c->nb_cursor, sizeof(int) * (ncell + 1)));
+    c->nb_cap_cells = ncell + 1;
+  }
+  int *dcid = c->nb_cid, *dcc = c->nb_cc, *dorder = c->nb_order, *dmisc = c->nb_misc, *dcursor = c->nb_cursor;
   A2C_HIP(c, hipMemsetAsync(dcc, 0, sizeof(int) * (ncell + 1), c->stream));
   A2C_HIP(c, hipMemsetAsync(dmisc, 0, sizeof(int) * 8, c->stream));
   const int TB = 256;
@@ -255,21 +261,17 @@ int nbr_build(a2c_fe_ctx* c) {
   std::vector<int> start(ncell + 1, 0);
   for (long long k = 0; k < ncell; ++k) start[k + 1] = start[k] + hc[k];
   A2C_HIP(c, hipMemcpyAsync(dcc, start.data(), sizeof(int) * (ncell + 1), hipMemcpyHostToDevice, c->stream));
-  int* dcursor = nullptr;
-  A2C_HIP(c, hipMalloc(&dcursor, sizeof(int) * (ncell + 1)));
   A2C_HIP(c, hipMemcpyAsync(dcursor, start.data(), sizeof(int) * (ncell + 1), hipMemcpyHostToDevice, c->stream));
   if (N > 0) {
     hipLaunchKernelGGL(k_fill, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, N, dcid, dcursor, dorder);
     A2C_HIP(c, hipGetLastError());
   }
-  // row-length arrays (kept in the ELL structs)
+  // row-length arrays (rows are fixed for a context)
   for (int w = 0; w < 5; ++w) {
-    hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt);
-    hipFree(c->ell[w].rev); hipFree(c->ell[w].rb1); hipFree(c->ell[w].rb4);
-    c->ell[w] = Ell{};
-    c->ell[w].rows = (w < 2) ? npd : nh;
-    A2C_HIP(c, hipMalloc(&c->ell[w].cnt, sizeof(int32_t) * std::max(c->ell[w].rows, 1)));
-    A2C_HIP(c, hipMemsetAsync(c->ell[w].cnt, 0, sizeof(int32_t) * std::max(c->ell[w].rows, 1), c->stream));
+    Ell& L = c->ell[w];
+    L.rows = (w < 2) ? npd : nh;
+    if (!L.cnt) A2C_HIP(c, hipMalloc(&L.cnt, sizeof(int32_t) * std::max(L.rows, 1)));
+    A2C_HIP(c, hipMemsetAsync(L.cnt, 0, sizeof(int32_t) * std::max(L.rows, 1), c->stream));
   }
   NbrArgs a{};
   a.X = c->dX; a.npd = npd; a.N = N; a.g = g; a.cid = dcid; a.cstart = dcc; a.order = dorder;
@@ -287,22 +289,25 @@ int nbr_build(a2c_fe_ctx* c) {
   int hm[8];
   A2C_HIP(c, hipMemcpyAsync(hm, dmisc, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
   A2C_HIP(c, hipStreamSynchronize(c->stream));
-  hipFree(dcursor);
   if (hm[6]) {
     c->have_nbr = false;
     return set_err(c, A2C_ENBR_OVERFLOW, "a site has more neighbors than max_neighbors");
   }
   for (int w = 0; w < 5; ++w) {
-    c->ell[w].width = hm[w];
-    size_t n = (size_t)std::max(hm[w], 1) * std::max(c->ell[w].rows, 1);
-    A2C_HIP(c, hipMalloc(&c->ell[w].idx, sizeof(int32_t) * n));
-    A2C_HIP(c, hipMemsetAsync(c->ell[w].idx, 0xff, sizeof(int32_t) * n, c->stream));
-    a.idx[w] = c->ell[w].idx;
+    Ell& L = c->ell[w];
+    L.width = hm[w];
+    const int64_t n = (int64_t)std::max(hm[w], 1) * std::max(L.rows, 1);
+    if (n > L.cap) {  // grow with slack so small changes of the widest row reuse the arrays
+      (void)hipFree(L.idx);
+      L.idx = nullptr;
+      L.cap = (int64_t)(std::max(hm[w], 1) + 4) * std::max(L.rows, 1);
+      A2C_HIP(c, hipMalloc(&L.idx, sizeof(int32_t) * L.cap));
+    }
+    A2C_HIP(c, hipMemsetAsync(L.idx, 0xff, sizeof(int32_t) * n, c->stream));
+    a.idx[w] = L.idx;
     a.width[w] = hm[w];
   }
-  hipFree(c->dpdpdnum);
-  c->dpdpdnum = nullptr;
-  A2C_HIP(c, hipMalloc(&c->dpdpdnum, sizeof(int32_t) * 3 * std::max(npd, 1)));
+  if (!c->dpdpdnum) A2C_HIP(c, hipMalloc(&c->dpdpdnum, sizeof(int32_t) * 3 * std::max(npd, 1)));
   a.pdpdnum = c->dpdpdnum;
   if (N > 0) {
     hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), 0, c->stream, a);
@@ -317,21 +322,22 @@ int nbr_build(a2c_fe_ctx* c) {
   return A2C_OK;
 }
 
-// Reverse slots and contribution buffers of the four pair lists (mixed path).
+// Reverse slots and contribution buffers of the four pair lists (mixed path);
+// the slots are recomputed after every build.
 int ensure_pair_buffers(a2c_fe_ctx* c) {
   if (!c->have_nbr) return A2C_OK;
   static const int partner[4] = {0, 2, 1, 3};  // PdPd, PdH<->HPd, HH
-  bool fresh = false;
   for (int w = 0; w < 4; ++w) {
     Ell& L = c->ell[w];
-    if (L.rev) continue;
-    const size_t n = (size_t)std::max(L.width, 1) * std::max(L.rows, 1);
-    A2C_HIP(c, hipMalloc(&L.rev, sizeof(int32_t) * n));
-    A2C_HIP(c, hipMalloc(&L.rb1, sizeof(float) * n));
-    A2C_HIP(c, hipMalloc(&L.rb4, sizeof(float4) * n));
-    fresh = true;
+    const int64_t n = (int64_t)std::max(L.width, 1) * std::max(L.rows, 1);
+    if (n <= L.pcap) continue;
+    (void)hipFree(L.rev); (void)hipFree(L.rb1); (void)hipFree(L.rb4);
+    L.rev = nullptr; L.rb1 = nullptr; L.rb4 = nullptr;
+    L.pcap = L.cap;  // same capacity as idx
+    A2C_HIP(c, hipMalloc(&L.rev, sizeof(int32_t) * L.pcap));
+    A2C_HIP(c, hipMalloc(&L.rb1, sizeof(float) * L.pcap));
+    A2C_HIP(c, hipMalloc(&L.rb4, sizeof(float4) * L.pcap));
   }
-  if (!fresh) return A2C_OK;
   for (int w = 0; w < 4; ++w) {
     const Ell& L = c->ell[w];
     if (L.rows > 0) {

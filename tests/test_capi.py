@@ -93,3 +93,13 @@ def test_mixed_tables_reproduce_the_closed_forms():
         assert np.max(np.abs(v - ev)) <= 2e-7 * np.abs(ev).max(), t
         assert abs(np.mean(v - ev)) <= 1e-9 * np.abs(ev).max(), t
         assert np.max(np.abs(dv - ed)) <= 5e-6 * np.abs(ed).max(), t
+
+
+def test_serving_request_stream_is_seeded():
+    from a2c_amd import serve
+    a = serve.make_requests(20, rate=5.0)
+    b = serve.make_requests(20, rate=5.0)
+    assert a == b
+    assert all(6 <= r["n"] <= 48 for r in a) and all(300 <= r["T"] <= 700 for r in a)
+    t = [r["arrival"] for r in a]
+    assert t == sorted(t) and 1.0 < 20 / t[-1] < 20.0
