@@ -40,6 +40,27 @@ def _compile(src):
     return obj
 
 
+HOST = os.path.join(HERE, "host")
+HOST_BINS = {"a2c_dmd": "a2c_dmd.cpp"}
+
+
+def build_host(verbose=False):
+    """C++ host programs over the library (g++, rpath to this directory)."""
+    out = []
+    for name, src in HOST_BINS.items():
+        exe = os.path.join(HERE, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + HOST,
+               os.path.join(HOST, src), "-L" + HERE, "-la2c_fe", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("host build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    if verbose:
+        print("built", " ".join(out))
+    return out
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -53,6 +74,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -39,3 +39,29 @@ def test_dmd_steps_conserve_and_match_oracle():
     # H moved out of the slab
     ix = (np.arange(nh) // 4) % 8
     assert x[ix >= 2].sum() > (ix >= 2).sum() * 1e-3
+
+
+def test_host_program_runs_a_deck(tmp_path):
+    """The C++ host program: deck -> specimen -> DMD steps -> restart files
+    that the reader (readresults.cpp format) accepts."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    deck = tmp_path / "slab.st"
+    deck.write_text("""
+    under Specimen { Cells_x = 8; Cells_y = 3; Cells_z = 3; Seed = 5; Displacement = 0.02;
+                     HFraction = 0.001; SlabCells = 2; SlabHFraction = 0.6; }
+    under InteratomicPotential { CutOffDistance = 5.35; Tolerance = 1.0e-5; }
+    under Diffusion { Temperature = 400; AttemptFrequency = 1.0e13; ActivationEnergy = 0.23; }
+    under Time { TimeStep = 1.0e-13; MaxIts = 2; }
+    under Output { Prefix = "%s/"; Solution = "slab"; Frequency = 1; }
+    under Solver { Precision = Mixed; GradientTolerance = 1.0e-8; }
+    """ % tmp_path)
+    r = subprocess.run([os.path.join(root, "a2c_amd", "a2c_dmd"), str(deck)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("step")]
+    assert len(lines) == 2 and all("reason 1" in l for l in lines)
+    npd = nh = 4 * 8 * 3 * 3
+    X, x, g, fh = A.readResults(str(tmp_path / "slab.2.dat"), npd, nh)
+    assert X.size == 4 * (npd + nh) and np.all(x >= 1e-3)
