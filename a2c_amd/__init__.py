@@ -155,6 +155,7 @@ def _load():
         "a2c_macro_results": (I, [C.c_int32, dp, C.c_int32, ip, dp, dp]),
         "a2c_read_results": (I, [C.c_char_p, C.c_int32, C.c_int32, D, D, dp, dp, dp, ip]),
         "a2c_write_results": (I, [C.c_char_p, C.c_int32, C.c_int32, dp, dp, dp, ip, D, C.c_int32]),
+        "a2c_write_xyz": (I, [C.c_char_p, C.c_int32, C.c_int32, dp, dp, dp, dp, dp, ip, D]),
         "a2c_dmd_params_default": (None, [P(DmdParams)]),
         "a2c_dmd_step": (I, [VP, P(DmdParams), C.c_int32, ip, C.c_int32, ip, dp, dp, dp, ip, P(DmdReport)]),
         "a2c_batch_desc_default": (None, [P(BatchDesc)]),
@@ -444,6 +445,16 @@ def writeResults(path, n_pd, n_h, X, x, gamma, full_h, time=0.0, step=0):
     rc = lib.a2c_write_results(path.encode(), n_pd, n_h, _dptr(X), _dptr(x), _dptr(g), _iptr(fh), time, step)
     if rc:
         raise A2CError(rc, "cannot write restart file %s" % path)
+
+
+def writeXYZ(path, n_pd, n_h, X, X0, x, gamma, box, periodic=(1, 1, 1), time=0.0):
+    """Extended-XYZ snapshot (Output.cpp:332-494 layout)."""
+    X, X0, x, g, b = _f64(X), _f64(X0), _f64(x), _f64(gamma), _f64(box, 3)
+    per = _i32(periodic)
+    rc = lib.a2c_write_xyz(path.encode(), n_pd, n_h, _dptr(X), _dptr(X0), _dptr(x), _dptr(g), _dptr(b), _iptr(per),
+                           time)
+    if rc:
+        raise A2CError(rc, "cannot write %s" % path)
 
 
 def dmd_params(**kw):

@@ -82,6 +82,30 @@ int a2c_write_results(const char* path, int32_t n_pd, int32_t n_h, const double*
   return std::fclose(f) == 0 ? A2C_OK : A2C_EINVAL;
 }
 
+int a2c_write_xyz(const char* path, int32_t n_pd, int32_t n_h, const double* X, const double* X0, const double* x,
+                  const double* gamma, const double* box, const int32_t* periodic, double time) {
+  if (!path || !X || !X0 || !box || !periodic || n_pd < 0 || n_h < 0 || (n_h > 0 && (!x || !gamma)))
+    return A2C_EINVAL;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return A2C_EINVAL;
+  const int32_t n = n_pd + n_h;
+  std::fprintf(f, "%d\n", n);
+  std::fprintf(f, "Lattice=\"%e %e %e %e %e %e %e %e %e\" ", box[0], 0.0, 0.0, 0.0, box[1], 0.0, 0.0, 0.0, box[2]);
+  std::fprintf(f, "Origin=\"%e %e %e\" ", 0.0, 0.0, 0.0);
+  std::fprintf(f, "pbc=\"%s %s %s\" ", periodic[0] ? "T" : "F", periodic[1] ? "T" : "F", periodic[2] ? "T" : "F");
+  std::fprintf(f, "Properties=\"species:S:1:pos:R:3:disp:R:3:type:I:1:sigma:R:1:tag:I:1:molar_frac:R:1:"
+                  "chem_potent:R:1\" Time=%e\n", time);
+  for (int32_t s = 0; s < n; ++s) {
+    const double* q = X + 4 * (size_t)s;
+    const double* q0 = X0 + 4 * (size_t)s;
+    const bool h = s >= n_pd;
+    std::fprintf(f, "%16s  %16.10e  %16.10e  %16.10e  %16.10e  %16.10e  %16.10e  %4d  %16.10e  %4d  %16.10e  %16.10e\n",
+                 h ? "H" : "Pd", q[0], q[1], q[2], q[0] - q0[0], q[1] - q0[1], q[2] - q0[2], h ? 1 : 0, q[3], 0,
+                 h ? x[s - n_pd] : 1.0, h ? gamma[s - n_pd] : 0.0);
+  }
+  return std::fclose(f) == 0 ? A2C_OK : A2C_EINVAL;
+}
+
 void a2c_dmd_params_default(a2c_dmd_params* p) {
   std::memset(p, 0, sizeof(*p));
   p->dt = 1e-13;

@@ -72,6 +72,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "ERROR: cannot read the restart file\n");
     return 1;
   }
+  const std::vector<double> X0 = X;  // reference positions for the displacement column
   // reservoir (subsurface) sites: H sites of the first layers of cells along x
   std::vector<int32_t> sub;
   const int layers = (int)deck.integer("Diffusion.SubsurfaceLayers", 0);
@@ -126,6 +127,13 @@ int main(int argc, char** argv) {
       const std::string path = prefix + base + "." + std::to_string(st) + ".dat";
       if (a2c_write_results(path.c_str(), npd, nh, X.data(), x.data(), gamma.data(), full.data(), st * p.dt, st))
         std::fprintf(stderr, "WARNING: cannot write %s\n", path.c_str());
+      char frame[32];
+      std::snprintf(frame, sizeof frame, "_L0_%04d.xyz", st / freq);
+      const std::string xyz = prefix + base + frame;
+      const int32_t perio[3] = {per, per, per};
+      if (a2c_write_xyz(xyz.c_str(), npd, nh, X.data(), X0.data(), x.data(), gamma.data(), box.data(), perio,
+                        st * p.dt))
+        std::fprintf(stderr, "WARNING: cannot write %s\n", xyz.c_str());
     }
   }
   a2c_fe_destroy(ctx);

@@ -237,7 +237,8 @@ def main():
     if args.precision == "mixed":
         fe.set_precision("mixed")
         tm, wall, f, G = run_gpu(fe, X, args.steps, args.warmup)
-        gkern = "grad32"
+        # pair-once default: the owner kernel carries every pair term; full-list mode: grad32
+        gkern = "grad_own" if "grad_own" in tm["kernels_ms"] else "grad32"
     else:
         tm, wall, f, G = tm64, wall64, f64, G64
         args.steps = max(3, args.steps // 10)

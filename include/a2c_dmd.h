@@ -40,6 +40,13 @@ int a2c_read_results(const char* path, int32_t n_pd, int32_t n_h, double xLb, do
 int a2c_write_results(const char* path, int32_t n_pd, int32_t n_h, const double* X, const double* x,
                       const double* gamma, const int32_t* full_h, double time, int32_t step);
 
+/* Extended-XYZ snapshot in the layout of Output::OutputSolutionOnLatticeXYZ
+ * (Output.cpp:332-494): "Lattice", "Origin", "pbc" and "Properties" header,
+ * then per site: species, position q, displacement q - q0, type (0 Pd, 1 H),
+ * sigma, tag, molar fraction x (1 for Pd) and chemical potential gamma. */
+int a2c_write_xyz(const char* path, int32_t n_pd, int32_t n_h, const double* X, const double* X0, const double* x,
+                  const double* gamma, const double* box, const int32_t* periodic, double time);
+
 typedef struct a2c_dmd_params {
   double dt, nu, Qm;        /* time step, attempt frequency, activation energy */
   double xLb, xUb;          /* occupancy clamp (integrate.cpp:79-81) */

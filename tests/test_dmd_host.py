@@ -46,3 +46,19 @@ def test_missing_restart_file_is_an_error(tmp_path):
         assert e.code == A.A2C_EINVAL
     else:
         raise AssertionError("expected an error")
+
+
+def test_xyz_snapshot_layout(tmp_path):
+    X, x, box, npd, nh = A.lattice(2, seed=3, disp_std=0.05)
+    X0 = X.copy()
+    X0[:3] -= 0.01
+    g = np.linspace(-1, 1, nh)
+    p = str(tmp_path / "s_L0_0001.xyz")
+    A.writeXYZ(p, npd, nh, X, X0, x, g, box, (1, 1, 0), time=2e-13)
+    lines = open(p).read().splitlines()
+    assert int(lines[0]) == npd + nh and len(lines) == 2 + npd + nh
+    assert 'pbc="T T F"' in lines[1] and "Properties=\"species:S:1:pos:R:3:disp:R:3" in lines[1]
+    first = lines[2].split()
+    assert first[0] == "Pd" and abs(float(first[4]) - 0.01) < 1e-12 and len(first) == 12
+    last = lines[-1].split()
+    assert last[0] == "H" and float(last[-1]) == pytest.approx(1.0)
