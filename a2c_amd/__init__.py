@@ -590,9 +590,20 @@ class Batch:
         self._check(lib.a2c_batch_lmvm_report(self._b, slot, C.byref(r)))
         return {k: getattr(r, k) for k, _ in LmvmReport._fields_}
 
-    def run(self, max_iters=1 << 20):
-        while self.step(max_iters) > 0:
-            pass
+    def run(self, max_iters=1 << 20, progress=None, every=50):
+        """Step every running slot to completion; progress(iterations, running)
+        is called every `every` iterations when given."""
+        if progress is None:
+            while self.step(max_iters) > 0:
+                pass
+            return
+        done = 0
+        while True:
+            running = self.step(every)
+            done += every
+            progress(done, running)
+            if running == 0:
+                return
 
 
 def exported_symbols():

@@ -109,7 +109,7 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   hipFree(c->demb); hipFree(c->dpart); hipFree(c->dF); hipFree(c->dpdpdnum); hipFree(c->dtab); hipFree(c->daux);
   for (int w = 0; w < 5; ++w) {
     hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt);
-    hipFree(c->ell[w].rev); hipFree(c->ell[w].rb1); hipFree(c->ell[w].rb4);
+    a2c::free_pair_buffers(c->ell[w]);
   }
   hipFree(c->depair);
   hipFree(c->nb_cid); hipFree(c->nb_order); hipFree(c->nb_cc); hipFree(c->nb_cursor); hipFree(c->nb_misc);

@@ -85,7 +85,7 @@ __device__ __forceinline__ void pair_vec32(const LatDev& S, const Site& a, const
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double qj = b.q[c];
-    if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) / S.box[c]);
+    if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) * S.ibox[c]);
     d[c] = (float)(a.q[c] - qj);
   }
 }
@@ -290,11 +290,12 @@ __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, 
 // ================================================================ pair-once
 // Each unordered pair {s, j} is evaluated once, by its owner:
 //   own(s, j) = (s < j) XOR ((s ^ j) & 1),
-// which gives every site about half of each of its lists.  The owner adds its
-// own share and writes the partner's share into the partner's list entry
-// (reverse slot); a gather kernel adds the received shares in FP64.  Each
+// which gives every site about half of each of its lists.  Every row is split
+// at neighbor-build time into a compact owned half and a compact received half
+// (same order as the row), so each lane loops over its own pairs only.  The
+// owner adds its own share and writes the partner's share into the partner's
+// received slot; a gather kernel adds the received shares in FP64.  Each
 // buffer entry is written by exactly one lane, so results are deterministic.
-__device__ __forceinline__ bool owns(int s, int j) { return (s < j) != (((s ^ j) & 1) != 0); }
 
 // 12-point sums of two density tables at the same points
 __device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb, const float d[3], float ai,
@@ -344,19 +345,18 @@ __global__ __launch_bounds__(TB) void k_dens_own(const LatDev* __restrict__ lats
     const int list = pd ? side : 2 + side;
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const Ell& P = partner_of(S, list);
-    const int n = L.cnt[row];
+    const int n = L.ocnt[row];
     for (int t = 0; t < n; ++t) {
       const size_t e = (size_t)t * L.rows + row;
-      const int jl = L.idx[e];
+      const int jl = L.oj[e];
       const int j = side == 0 ? jl : S.npd + jl;
-      if (!owns(s, j)) continue;
       const Site b = load_site(X, j);
       float d[3];
       pair_vec32(S, a, b, d);
       float mine, theirs;  // neighbor-type density (for me), my-type density (for the neighbor)
       pair_density2(tab, side == 0 ? T_FPD : T_FH, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
-      const int rv = L.rev[e];
+      const int rv = L.orv[e];
       if (rv >= 0) P.rb1[(size_t)rv * P.rows + jl] = theirs;
     }
   }
@@ -376,12 +376,10 @@ __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lat
   for (int side = 0; side < 2; ++side) {
     const int list = pd ? side : 2 + side;
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
-    const int n = L.cnt[row];
+    const int n = L.rcnt[row];
     for (int t = 0; t < n; ++t) {
       const size_t e = (size_t)t * L.rows + row;
-      const int jl = L.idx[e];
-      const int j = side == 0 ? jl : S.npd + jl;
-      if (owns(s, j)) continue;
+      const int jl = L.rj[e];
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)L.rb1[e];
     }
   }
@@ -457,12 +455,11 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
     const int list = pd ? side : 2 + side;
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const Ell& P = partner_of(S, list);
-    const int n = L.cnt[row];
+    const int n = L.ocnt[row];
     for (int t = 0; t < n; ++t) {
       const size_t ei = (size_t)t * L.rows + row;
-      const int jl = L.idx[ei];
+      const int jl = L.oj[ei];
       const int j = side == 0 ? jl : S.npd + jl;
-      if (!owns(s, j)) continue;
       const Site b = load_site(X, j);
       const float Fj = aux[j].x;
       float d[3];
@@ -477,7 +474,7 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
       e += w * p.pe;
       g0 += w * p.g[0]; g1 += w * p.g[1]; g2 += w * p.g[2];
       gs += w * p.gsi;
-      const int rv = L.rev[ei];
+      const int rv = L.orv[ei];
       if (rv >= 0) P.rb4[(size_t)rv * P.rows + jl] = make_float4(-p.g[0], -p.g[1], -p.g[2], p.gsj);
     }
   }
@@ -506,12 +503,10 @@ __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lat
     for (int side = 0; side < 2; ++side) {
       const int list = pd ? side : 2 + side;
       const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
-      const int n = L.cnt[row];
+      const int n = L.rcnt[row];
       for (int t = 0; t < n; ++t) {
         const size_t ei = (size_t)t * L.rows + row;
-        const int jl = L.idx[ei];
-        const int j = side == 0 ? jl : S.npd + jl;
-        if (owns(s, j)) continue;
+        const int jl = L.rj[ei];
         const float4 v = L.rb4[ei];
         const double w = side == 0 ? 1.0 : xo[jl];
         g0 += w * v.x; g1 += w * v.y; g2 += w * v.z;

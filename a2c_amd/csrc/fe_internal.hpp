@@ -21,14 +21,20 @@ struct Ell {
   int32_t width = 0;
   int32_t* idx = nullptr;  // width * rows
   int32_t* cnt = nullptr;  // rows
-  // mixed path (each unordered pair evaluated once): slot of this entry's
-  // pair in the partner's list, and the partner-written contributions that
-  // arrive at this list's entries
-  int32_t* rev = nullptr;  // width * rows
-  float* rb1 = nullptr;    // width * rows  (density pass)
-  float4* rb4 = nullptr;   // width * rows  (gradient pass)
-  int64_t cap = 0;         // allocated entries of idx (grow-only across rebuilds)
-  int64_t pcap = 0;        // allocated entries of rev/rb1/rb4
+  // Mixed path (each unordered pair evaluated once, by its owner).  Every
+  // row is split, keeping its order, into the entries this row owns and the
+  // entries it receives; both halves are compact (slot-major, no gaps), so a
+  // wave's lanes step through their halves without masking each other off.
+  int32_t* ocnt = nullptr;  // rows: owned entries
+  int32_t* oj = nullptr;    // width * rows: neighbor of owned entry t
+  int32_t* orv = nullptr;   // width * rows: slot of the pair in the partner's received half
+  int32_t* rcnt = nullptr;  // rows: received entries
+  int32_t* rj = nullptr;    // width * rows: neighbor of received entry t
+  int32_t* rpos = nullptr;  // width * rows: full-row slot -> received slot (-1 if owned)
+  float* rb1 = nullptr;     // width * rows: received density shares
+  float4* rb4 = nullptr;    // width * rows: received gradient shares
+  int64_t cap = 0;          // allocated entries of idx (grow-only across rebuilds)
+  int64_t pcap = 0;         // allocated entries of the split arrays
 };
 
 // Device view of one lattice: what every kernel needs.  The eval kernels
@@ -41,6 +47,7 @@ struct LatDev {
   int32_t periodic[3] = {0, 0, 0};
   int32_t blk0 = 0;          // first block of this lattice in the current launch
   double box[3] = {0, 0, 0};
+  double ibox[3] = {0, 0, 0};  // 1/box: the image shift of a listed pair is L*rint(d/L) with d/L as d*(1/L)
   double T = 0;
   Ell pdpd, pdh, hpd, hh, hh1;
   const double* X = nullptr;  // 4N state (32-byte aligned)
@@ -112,7 +119,11 @@ struct a2c_fe_ctx {
     a2c::DevSys s;
     s.npd = desc.n_pd;
     s.nh = desc.n_h;
-    for (int c = 0; c < 3; ++c) { s.box[c] = desc.box[c]; s.periodic[c] = desc.periodic[c]; }
+    for (int c = 0; c < 3; ++c) {
+      s.box[c] = desc.box[c];
+      s.ibox[c] = desc.box[c] > 0.0 ? 1.0 / desc.box[c] : 0.0;
+      s.periodic[c] = desc.periodic[c];
+    }
     s.T = desc.temperature;
     s.pdpd = ell[0]; s.pdh = ell[1]; s.hpd = ell[2]; s.hh = ell[3]; s.hh1 = ell[4];
     s.x = dx; s.rho = drho; s.Fp = dFp; s.aux = daux; s.emb = demb; s.part = dpart; s.epair = depair;
@@ -133,7 +144,8 @@ int set_err(a2c_fe_ctx* c, int code, const std::string& msg);
 
 // kernels (launchers) ---------------------------------------------------------
 int nbr_build(a2c_fe_ctx* c);                           // nbr.hip
-int ensure_pair_buffers(a2c_fe_ctx* c);                 // reverse slots + contribution buffers
+int ensure_pair_buffers(a2c_fe_ctx* c);                 // owned/received split + contribution buffers
+void free_pair_buffers(Ell& L);                         // nbr.hip
 // Multi-lattice launches: lats/b2l on the device, nblk = total site blocks,
 // nlat = lattices.  The FP64 path fills rho, Fp, emb, G, part, f; the mixed
 // path rho, aux, emb, G, part, f.

@@ -9,8 +9,8 @@
 // d2 < 0.75a^2 (:999-1004).
 //
 // Kernels: k_bin (cell id + histogram), host scan, k_fill (scatter into cell
-// order), k_nbr<false> (row lengths -> ELL widths), k_nbr<true> (sorted ELL
-// rows).  One 64-lane wave per site: lanes test consecutive members of a cell,
+// order), k_nbr<false> (row lengths), k_width (ELL widths), k_nbr<true>
+// (sorted ELL rows).  One 64-lane wave per site: lanes test consecutive members of a cell,
 // a ballot compacts hits into LDS, and each hit's slot is its rank among the
 // hits (an O(n) count per lane over LDS broadcasts), so rows come out sorted
 // whatever order the atomics binned the sites in.
@@ -133,15 +133,12 @@ __global__ __launch_bounds__(64) void k_nbr(NbrArgs a) {
   }
   const int nhn = n - npn;
   if (!FILL) {
-    if (lane == 0) {
+    if (lane == 0) {  // list widths: k_width (one atomic per block, not per site)
       if (pd) {
         a.cnt[0][row] = npn; a.cnt[1][row] = nhn;
-        atomicMax(&a.maxw[0], npn); atomicMax(&a.maxw[1], nhn);
       } else {
         a.cnt[2][row] = npn; a.cnt[3][row] = nhn; a.cnt[4][row] = nh1;
-        atomicMax(&a.maxw[2], npn); atomicMax(&a.maxw[3], nhn); atomicMax(&a.maxw[4], nh1);
       }
-      atomicMax(&a.maxw[5], n);
     }
     return;
   }
@@ -188,12 +185,72 @@ __global__ __launch_bounds__(64) void k_nbr(NbrArgs a) {
 // Reverse slots: for entry (row r, slot t) of list L with neighbor j, the slot
 // of r in row j of the partner list P (PdPd<->PdPd, PdH<->HPd, HH<->HH).
 // Rows are sorted, so a binary search finds it; lists are symmetric.
-__global__ void k_rev(Ell L, Ell P) {
+// Widest row of each list and the largest total count over all sites:
+// block-local maxima, then one atomicMax per block and quantity.
+__global__ __launch_bounds__(256) void k_width(NbrArgs a) {
+  __shared__ int32_t red[6][256];
+  int m[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < a.N; s += gridDim.x * 256) {
+    if (s < a.npd) {
+      const int p = a.cnt[0][s], h = a.cnt[1][s];
+      m[0] = max(m[0], p); m[1] = max(m[1], h); m[5] = max(m[5], p + h);
+    } else {
+      const int r = s - a.npd;
+      const int p = a.cnt[2][r], h = a.cnt[3][r], h1 = a.cnt[4][r];
+      m[2] = max(m[2], p); m[3] = max(m[3], h); m[4] = max(m[4], h1); m[5] = max(m[5], p + h);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = m[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = max(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) atomicMax(&a.maxw[threadIdx.x], red[threadIdx.x][0]);
+}
+
+// Split of each pair-list row into owned and received halves (the mixed
+// path evaluates each unordered pair once).  Ownership of {s, j} (global
+// site indices): (s < j) XOR ((s ^ j) & 1), which hands each site about half
+// of every list.  Both halves keep the row's (sorted) order.
+__device__ __forceinline__ bool pair_owner(int s, int j) { return (s < j) != (((s ^ j) & 1) != 0); }
+
+__global__ void k_split(Ell L, int s_off, int j_off) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= L.rows) return;
   const int n = L.cnt[r];
+  const int s = s_off + r;
+  int no = 0, nr = 0;
   for (int t = 0; t < n; ++t) {
-    const int j = L.idx[(size_t)t * L.rows + r];
+    const size_t e = (size_t)t * L.rows + r;
+    const int j = L.idx[e];
+    if (pair_owner(s, j_off + j)) {
+      L.oj[(size_t)no * L.rows + r] = j;
+      L.rpos[e] = -1;
+      ++no;
+    } else {
+      L.rj[(size_t)nr * L.rows + r] = j;
+      L.rpos[e] = nr;
+      ++nr;
+    }
+  }
+  L.ocnt[r] = no;
+  L.rcnt[r] = nr;
+}
+
+// For every owned entry (r, j): the received slot of r in the partner row j,
+// found by binary search of r in that sorted row (-1 if absent, which the
+// symmetric lists never produce).
+__global__ void k_orv(Ell L, Ell P) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L.rows) return;
+  const int n = L.ocnt[r];
+  for (int t = 0; t < n; ++t) {
+    const size_t e = (size_t)t * L.rows + r;
+    const int j = L.oj[e];
     int lo = 0, hi = P.cnt[j] - 1, k = -1;
     while (lo <= hi) {
       const int mid = (lo + hi) >> 1;
@@ -202,7 +259,7 @@ __global__ void k_rev(Ell L, Ell P) {
       if (v < r) lo = mid + 1;
       else hi = mid - 1;
     }
-    L.rev[(size_t)t * L.rows + r] = k;
+    L.orv[e] = k >= 0 ? P.rpos[(size_t)k * P.rows + j] : -1;
   }
 }
 
@@ -285,6 +342,8 @@ int nbr_build(a2c_fe_ctx* c) {
   if (N > 0) {
     hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, c->stream, a);
     A2C_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_width, dim3(std::min((N + 255) / 256, 1024)), dim3(256), 0, c->stream, a);
+    A2C_HIP(c, hipGetLastError());
   }
   int hm[8];
   A2C_HIP(c, hipMemcpyAsync(hm, dmisc, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -322,26 +381,41 @@ int nbr_build(a2c_fe_ctx* c) {
   return A2C_OK;
 }
 
-// Reverse slots and contribution buffers of the four pair lists (mixed path);
-// the slots are recomputed after every build.
+// Owned/received split and contribution buffers of the four pair lists
+// (mixed path); the split is recomputed after every build.
 int ensure_pair_buffers(a2c_fe_ctx* c) {
   if (!c->have_nbr) return A2C_OK;
   static const int partner[4] = {0, 2, 1, 3};  // PdPd, PdH<->HPd, HH
+  const int npd = c->desc.n_pd;
   for (int w = 0; w < 4; ++w) {
     Ell& L = c->ell[w];
     const int64_t n = (int64_t)std::max(L.width, 1) * std::max(L.rows, 1);
     if (n <= L.pcap) continue;
-    (void)hipFree(L.rev); (void)hipFree(L.rb1); (void)hipFree(L.rb4);
-    L.rev = nullptr; L.rb1 = nullptr; L.rb4 = nullptr;
+    free_pair_buffers(L);
     L.pcap = L.cap;  // same capacity as idx
-    A2C_HIP(c, hipMalloc(&L.rev, sizeof(int32_t) * L.pcap));
+    const size_t rows = (size_t)std::max(L.rows, 1);
+    A2C_HIP(c, hipMalloc(&L.ocnt, sizeof(int32_t) * rows));
+    A2C_HIP(c, hipMalloc(&L.rcnt, sizeof(int32_t) * rows));
+    A2C_HIP(c, hipMalloc(&L.oj, sizeof(int32_t) * L.pcap));
+    A2C_HIP(c, hipMalloc(&L.orv, sizeof(int32_t) * L.pcap));
+    A2C_HIP(c, hipMalloc(&L.rj, sizeof(int32_t) * L.pcap));
+    A2C_HIP(c, hipMalloc(&L.rpos, sizeof(int32_t) * L.pcap));
     A2C_HIP(c, hipMalloc(&L.rb1, sizeof(float) * L.pcap));
     A2C_HIP(c, hipMalloc(&L.rb4, sizeof(float4) * L.pcap));
+  }
+  static const int s_sp[4] = {0, 0, 1, 1}, j_sp[4] = {0, 1, 0, 1};  // species of rows / of neighbors
+  for (int w = 0; w < 4; ++w) {
+    const Ell& L = c->ell[w];
+    if (L.rows > 0) {
+      hipLaunchKernelGGL(k_split, dim3((L.rows + 255) / 256), dim3(256), 0, c->stream, L, s_sp[w] * npd,
+                         j_sp[w] * npd);
+      A2C_HIP(c, hipGetLastError());
+    }
   }
   for (int w = 0; w < 4; ++w) {
     const Ell& L = c->ell[w];
     if (L.rows > 0) {
-      hipLaunchKernelGGL(k_rev, dim3((L.rows + 255) / 256), dim3(256), 0, c->stream, L, c->ell[partner[w]]);
+      hipLaunchKernelGGL(k_orv, dim3((L.rows + 255) / 256), dim3(256), 0, c->stream, L, c->ell[partner[w]]);
       A2C_HIP(c, hipGetLastError());
     }
   }
@@ -349,6 +423,15 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
   A2C_HIP(c, hipStreamSynchronize(c->stream));
   c->lat_dirty = true;
   return A2C_OK;
+}
+
+void free_pair_buffers(Ell& L) {
+  (void)hipFree(L.ocnt); (void)hipFree(L.rcnt); (void)hipFree(L.oj); (void)hipFree(L.orv);
+  (void)hipFree(L.rj); (void)hipFree(L.rpos); (void)hipFree(L.rb1); (void)hipFree(L.rb4);
+  L.ocnt = L.rcnt = L.oj = L.orv = L.rj = L.rpos = nullptr;
+  L.rb1 = nullptr;
+  L.rb4 = nullptr;
+  L.pcap = 0;
 }
 
 }  // namespace a2c

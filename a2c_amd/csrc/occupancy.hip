@@ -46,7 +46,7 @@ __device__ __forceinline__ void pair_vec(const DevSys& S, const Site& a, const S
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double qj = b.q[c];
-    if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) / S.box[c]);
+    if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) * S.ibox[c]);
     d[c] = a.q[c] - qj;
   }
 }

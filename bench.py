@@ -123,7 +123,7 @@ def batched(args):
         for sl in slots:
             B.start(sl, **opts)
         t1 = time.perf_counter()
-        B.run()
+        B.run(progress=lambda it, n: log("batched: %d iterations, %d lattices running" % (it, n)))
         t_relax = time.perf_counter() - t1
         reps = [B.report(sl) for sl in slots]
     wall = t_adm + t_relax

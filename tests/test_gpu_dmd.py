@@ -24,10 +24,13 @@ def test_dmd_steps_conserve_and_match_oracle():
     orc = O.Oracle(npd, nh, box, (1, 1, 1), T=T, threads=8)
     for step in range(3):
         x_before = x.copy()
+        X_start = X.copy()
         rep = A.dmd_step(fe, prm, X, x, g, fh)
         assert rep["lmvm_reason"] > 0
-        # oracle from the relaxed state of this step
-        orc.build_neighbors(X)
+        # oracle at the relaxed state of this step, with the lists of the
+        # step's start (the step builds its lists before relaxing, as
+        # findLocalNeighbors precedes minimizeFreeEntropy in the reference)
+        orc.build_neighbors(X_start)
         go, fo = orc.chem_potential(X, x_before, np.zeros(4), np.zeros(4), 1e300, np.zeros(nh), np.zeros(nh, np.int32))
         assert np.max(np.abs(g - go) / np.maximum(np.abs(go), 1.0)) <= 1e-10
         xo = orc.diffuse(1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16, go, fo, x_before)
