@@ -247,6 +247,12 @@ def main():
     kern = tm["kernels_ms"]
     gms = kern.get(gkern, float("nan"))
     achieved = flop_grad / (gms * 1e-3) / 1e12
+    traffic = None  # HBM bytes per dispatch of gkern from the committed PMC summary (FETCH/WRITE_SIZE, scaled)
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("kernel") == gkern:
+            traffic = tr["bytes_per_dispatch"]
     obj_err = abs(f - f64) / abs(f64)
     grad_err = float(np.linalg.norm(G - G64) / np.linalg.norm(G64))
     result = {
@@ -266,7 +272,7 @@ def main():
                                "x~U(0,0.6), disp N(0,0.03), sigma U(0.08,0.15), T=500K, R12 rule",
                    "cells": args.cells, "sites": N, "precision": args.precision},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[args.precision], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[args.precision], "traffic": None, "kernel": gkern,
+                     "frac": achieved / PEAK[args.precision], "traffic": traffic, "kernel": gkern,
                      "kernel_ms": gms, "algorithmic_flop": flop_grad},
         "kernels_ms": kern,
         "fp64_gpu": {"value": N / (ms64 * 1e-3), "ms_per_step": ms64, "kernels_ms": tm64["kernels_ms"]},

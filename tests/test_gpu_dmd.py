@@ -33,7 +33,10 @@ def test_dmd_steps_conserve_and_match_oracle():
         orc.build_neighbors(X_start)
         go, fo = orc.chem_potential(X, x_before, np.zeros(4), np.zeros(4), 1e300, np.zeros(nh), np.zeros(nh, np.int32))
         assert np.max(np.abs(g - go) / np.maximum(np.abs(go), 1.0)) <= 1e-10
-        xo = orc.diffuse(1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16, go, fo, x_before)
+        assert np.array_equal(fh, fo)
+        # the diffusion step from the GPU's own gamma (the gamma bar above
+        # allows ~1e-8 absolute, which the flux would carry into x)
+        xo = orc.diffuse(1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16, g, fh, x_before)
         assert np.max(np.abs(x - xo)) <= 1e-13
         assert rep["total_h_before"] == pytest.approx(x_before.sum(), rel=1e-15)
         clamped = np.any(x <= 1e-3) and np.any(x_before > 1e-3)
