@@ -485,7 +485,11 @@ int lmvm_init(a2c_batch* b, const std::vector<int>& ids) {
   std::vector<int> sw;
   for (int i : ids) {
     Slot& s = b->slots[i];
-    if (s.mixed && s.opts.mixed_switch && s.gnorm <= s.opts.mixed_gtol) sw.push_back(i);
+    // FP64 from the start when the mixed gradient is already small or the mixed
+    // evaluation is non-finite (a pair closer than its tables reach)
+    if (s.mixed && s.opts.mixed_switch &&
+        (s.gnorm <= s.opts.mixed_gtol || !std::isfinite(s.f) || !std::isfinite(s.gnorm)))
+      sw.push_back(i);
   }
   if (int rc = switch_to_fp64(b, sw)) return rc;
   for (int i : ids) {
@@ -655,8 +659,8 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
       Slot& s = b->slots[i];
       s.f = s.fold;
       if (s.lsstat == -1) { s.rep.reason = -3; s.running = false; }
+      else if (s.mixed && s.opts.mixed_switch) { sw.push_back(i); s.npairs = 0; }  // incl. non-finite mixed trials
       else if (s.lsstat == -2) { s.rep.reason = -4; s.running = false; }
-      else if (s.mixed && s.opts.mixed_switch) { sw.push_back(i); s.npairs = 0; }
       else if (s.npairs == 0) { s.rep.reason = -6; s.running = false; }
       else s.npairs = 0;
     }

@@ -47,29 +47,42 @@ __constant__ float TAIL[NTAB][3] = {
 
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// value and derivative of table w at distance r
-__device__ __forceinline__ void tab_vd(const float4* __restrict__ tab, int w, float r, float& v, float& d) {
-  if (r < R1) {
-    const float xx = fmaxf((r - R0) * INVH, 0.0f);
-    const int k = min((int)xx, NT - 1);
-    const float u = xx - (float)k;
-    const float4 c = tab[w * NT + k];
-    v = fmaf(u, fmaf(u, fmaf(u, c.w, c.z), c.y), c.x);
-    d = fmaf(u, fmaf(u, 3.0f * c.w, 2.0f * c.z), c.y) * INVH;
-  } else {
-    v = TAIL[w][0] * __expf(TAIL[w][1] * (r - TAIL[w][2]));
-    d = TAIL[w][1] * v;
-  }
+// Position of r in the tables: interval k and local coordinate u in [0, 1),
+// computed once per quadrature point and shared by every function evaluated
+// there.  Outside [1, 6) A ("far") k is clamped into the table so the reads
+// stay in bounds and their results are replaced: from 6 A on by the
+// exponential tails, below 1 A by NaN (no physical configuration comes that
+// close -- the nearest Pd-H distance is 1.9 A -- and the non-finite
+// objective makes the caller fall back to the FP64 path).
+struct Pos {
+  int k;
+  float u;
+  bool far;
+};
+__device__ __forceinline__ Pos tab_pos(float r) {
+  const float xx = fmaf(r, INVH, -R0 * INVH);
+  Pos p;
+  p.k = min(max((int)xx, 0), NT - 1);
+  p.u = __builtin_amdgcn_fractf(xx);
+  p.far = !(fabsf(xx - 0.5f * NT) < 0.5f * NT);
+  return p;
 }
-__device__ __forceinline__ float tab_v(const float4* __restrict__ tab, int w, float r) {
-  if (r < R1) {
-    const float xx = fmaxf((r - R0) * INVH, 0.0f);
-    const int k = min((int)xx, NT - 1);
-    const float u = xx - (float)k;
-    const float4 c = tab[w * NT + k];
-    return fmaf(u, fmaf(u, fmaf(u, c.w, c.z), c.y), c.x);
-  }
-  return TAIL[w][0] * __expf(TAIL[w][1] * (r - TAIL[w][2]));
+// value of a cubic row {c0, c1, c2, c3}
+__device__ __forceinline__ float cubic(const float4 c, float u) { return fmaf(u, fmaf(u, fmaf(u, c.w, c.z), c.y), c.x); }
+// value and d/du of a cubic row (Horner with derivative: 5 FMAs)
+__device__ __forceinline__ void cubic_vd(const float4 c, float u, float& v, float& du) {
+  const float v1 = fmaf(u, c.w, c.z);
+  const float v2 = fmaf(u, v1, c.y);
+  const float d2 = fmaf(u, c.w, v1);
+  v = fmaf(u, v2, c.x);
+  du = fmaf(u, d2, v2);
+}
+// value of a derivative row {c1, 2 c2, 3 c3, 0} (d/du of a cubic row)
+__device__ __forceinline__ float quad(const float4 c, float u) { return fmaf(u, fmaf(u, c.z, c.y), c.x); }
+// exponential tail of function w (its slope is TAIL[w][1] times the value);
+// NaN below the table (see tab_pos)
+__device__ __forceinline__ float tail(int w, float r) {
+  return r >= R1 ? TAIL[w][0] * __expf(TAIL[w][1] * (r - TAIL[w][2])) : __builtin_nanf("");
 }
 
 struct Site {
@@ -94,6 +107,15 @@ __device__ __forceinline__ void stage_tables(float4* lds, const float4* __restri
   for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
   __syncthreads();
 }
+// Gradient passes need f only through f': rows 0..2*NT-1 (f_Pd, f_H) are
+// staged as derivative rows {c1, 2 c2, 3 c3, 0}; the pair tables stay cubic.
+__device__ __forceinline__ void stage_grad_tables(float4* lds, const float4* __restrict__ g) {
+  for (int i = threadIdx.x; i < NTAB * NT; i += blockDim.x) {
+    const float4 c = g[i];
+    lds[i] = i < 2 * NT ? make_float4(c.y, 2.0f * c.z, 3.0f * c.w, 0.0f) : c;
+  }
+  __syncthreads();
+}
 
 // ---------------------------------------------------------------- density
 // Sum of the table-w density over the 12 points of one pair.
@@ -108,7 +130,11 @@ __device__ __forceinline__ float pair_density(const float4* tab, int w, const fl
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const float x = fmaxf(r2[q], 1e-12f);
-      acc += tab_v(tab, w, x * rsq(x));
+      const float r = x * rsq(x);
+      const Pos p = tab_pos(r);
+      float v = cubic(tab[w * NT + p.k], p.u);
+      if (p.far) v = tail(w, r);
+      acc += v;
     }
   }
   return acc;
@@ -157,25 +183,38 @@ __global__ __launch_bounds__(TB) void k_density32(const LatDev* __restrict__ lat
 
 // ---------------------------------------------------------------- gradient
 // Pair kinds: 0 Pd-Pd, 1 Pd-H (site Pd), 2 H-Pd (site H), 3 H-H.
+// Per point: the pair energy phi(r) and cu = h dE/dr (h = 0.01 A, the table
+// step; callers scale their per-pair sums by 1/h once) of the pair's share
+// of the objective, dE/dr = F'_i f_j'(r) + F'_j f_i'(r) + phi'(r) (f_k: the
+// density function of site k's species).  One table position serves all
+// three functions.
 template <int KIND>
-__device__ __forceinline__ void point_coef(const float4* tab, float r, float Fi, float Fj, float& phi, float& c) {
-  float v, dv;
-  if (KIND == 0) {
-    tab_vd(tab, T_FPD, r, v, dv);
-    float dphi;
-    tab_vd(tab, T_PHPD, r, phi, dphi);
-    c = fmaf(Fi + Fj, dv, dphi);
-  } else if (KIND == 3) {
-    tab_vd(tab, T_FH, r, v, dv);
-    float dphi;
-    tab_vd(tab, T_PHH, r, phi, dphi);
-    c = fmaf(Fi + Fj, dv, dphi);
+__device__ __forceinline__ void point_coef(const float4* tab, float r, float Fi, float Fj, float& phi, float& cu) {
+  constexpr int WP = KIND == 0 ? T_PHPD : (KIND == 3 ? T_PHH : T_PHPDH);
+  const Pos p = tab_pos(r);
+  float dphi;
+  cubic_vd(tab[WP * NT + p.k], p.u, phi, dphi);
+  if (KIND == 0 || KIND == 3) {
+    constexpr int WF = KIND == 0 ? T_FPD : T_FH;
+    cu = fmaf(Fi + Fj, quad(tab[WF * NT + p.k], p.u), dphi);
   } else {
-    float dpd, dh, dphi;
-    tab_vd(tab, T_FPD, r, v, dpd);
-    tab_vd(tab, T_FH, r, v, dh);
-    tab_vd(tab, T_PHPDH, r, phi, dphi);
-    c = (KIND == 1) ? fmaf(Fi, dh, fmaf(Fj, dpd, dphi)) : fmaf(Fi, dpd, fmaf(Fj, dh, dphi));
+    const float Fpd = KIND == 1 ? Fi : Fj;  // F' of the Pd site multiplies f_H'
+    const float Fh = KIND == 1 ? Fj : Fi;   // F' of the H site multiplies f_Pd'
+    cu = fmaf(Fpd, quad(tab[T_FH * NT + p.k], p.u), fmaf(Fh, quad(tab[T_FPD * NT + p.k], p.u), dphi));
+  }
+  if (p.far) {
+    phi = tail(WP, r);
+    const float dp = TAIL[WP][1] * phi;
+    float c;
+    if (KIND == 0 || KIND == 3) {
+      constexpr int WF = KIND == 0 ? T_FPD : T_FH;
+      c = fmaf(Fi + Fj, TAIL[WF][1] * tail(WF, r), dp);
+    } else {
+      const float Fpd = KIND == 1 ? Fi : Fj;
+      const float Fh = KIND == 1 ? Fj : Fi;
+      c = fmaf(Fpd, TAIL[T_FH][1] * tail(T_FH, r), fmaf(Fh, TAIL[T_FPD][1] * tail(T_FPD, r), dp));
+    }
+    cu = c * (1.0f / INVH);
   }
 }
 
@@ -214,15 +253,15 @@ __device__ __forceinline__ PairSums pair_grad(const float4* tab, const float d[3
   PairSums p;
   p.pe = pe;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) p.g[c] = fmaf(d[c], su, ai * ti[c] - aj * tj[c]);
-  p.gs = fmaf(d[0], ti[0], fmaf(d[1], ti[1], fmaf(d[2], ti[2], ai * ui)));
+  for (int c = 0; c < 3; ++c) p.g[c] = INVH * fmaf(d[c], su, ai * ti[c] - aj * tj[c]);
+  p.gs = INVH * fmaf(d[0], ti[0], fmaf(d[1], ti[1], fmaf(d[2], ti[2], ai * ui)));
   return p;
 }
 
 __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
-  stage_tables(tab, gtab, NTAB * NT);
+  stage_grad_tables(tab, gtab);
   const LatDev& S = lats[b2l[blockIdx.x]];
   const int lb = blockIdx.x - S.blk0;
   const int s = lb * TB + threadIdx.x;
@@ -298,6 +337,7 @@ __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, 
 // buffer entry is written by exactly one lane, so results are deterministic.
 
 // 12-point sums of two density tables at the same points
+template <bool SAME>
 __device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb, const float d[3], float ai,
                                               float aj, float& sa, float& sb) {
   const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -312,11 +352,18 @@ __device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb,
     for (int q = 0; q < 4; ++q) {
       const float x = fmaxf(r2[q], 1e-12f);
       const float r = x * rsq(x);
-      sa += tab_v(tab, wa, r);
-      if (wb != wa) sb += tab_v(tab, wb, r);
+      const Pos p = tab_pos(r);
+      float va = cubic(tab[wa * NT + p.k], p.u);
+      float vb = SAME ? va : cubic(tab[wb * NT + p.k], p.u);
+      if (p.far) {
+        va = tail(wa, r);
+        vb = SAME ? va : tail(wb, r);
+      }
+      sa += va;
+      if (!SAME) sb += vb;
     }
   }
-  if (wb == wa) sb = sa;
+  if (SAME) sb = sa;
 }
 
 // Partner list of each list: PdPd<->PdPd, PdH<->HPd, HH<->HH.
@@ -354,7 +401,9 @@ __global__ __launch_bounds__(TB) void k_dens_own(const LatDev* __restrict__ lats
       float d[3];
       pair_vec32(S, a, b, d);
       float mine, theirs;  // neighbor-type density (for me), my-type density (for the neighbor)
-      pair_density2(tab, side == 0 ? T_FPD : T_FH, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
+      const int wa = side == 0 ? T_FPD : T_FH;
+      if (wa == mytab) pair_density2<true>(tab, wa, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
+      else pair_density2<false>(tab, wa, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
       const int rv = L.orv[e];
       if (rv >= 0) P.rb1[(size_t)rv * P.rows + jl] = theirs;
@@ -427,16 +476,16 @@ __device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d
   PairSums2 p;
   p.pe = pe;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) p.g[c] = fmaf(d[c], su, ai * ti[c] - aj * tj[c]);
-  p.gsi = fmaf(d[0], ti[0], fmaf(d[1], ti[1], fmaf(d[2], ti[2], ai * ui)));
-  p.gsj = fmaf(aj, uj, -fmaf(d[0], tj[0], fmaf(d[1], tj[1], d[2] * tj[2])));
+  for (int c = 0; c < 3; ++c) p.g[c] = INVH * fmaf(d[c], su, ai * ti[c] - aj * tj[c]);
+  p.gsi = INVH * fmaf(d[0], ti[0], fmaf(d[1], ti[1], fmaf(d[2], ti[2], ai * ui)));
+  p.gsj = INVH * fmaf(aj, uj, -fmaf(d[0], tj[0], fmaf(d[1], tj[1], d[2] * tj[2])));
   return p;
 }
 
 __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                  const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
-  stage_tables(tab, gtab, NTAB * NT);
+  stage_grad_tables(tab, gtab);
   const LatDev& S = lats[b2l[blockIdx.x]];
   const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;

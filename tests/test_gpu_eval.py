@@ -182,3 +182,26 @@ def test_mixed_matches_fp64_kernel_at_12_cubed():
     assert np.linalg.norm(G32 - G64) <= MIXED_GRAD_TOL * np.linalg.norm(G64)
     f32b, G32b = fe.eval(X)
     assert f32b == f32 and np.array_equal(G32b, G32)  # deterministic
+
+
+def test_mixed_path_below_its_tables_falls_back_to_fp64():
+    """A pair closer than 1 A (outside the FP32 tables) makes the mixed
+    objective non-finite instead of silently clamping; the relaxation then
+    runs in FP64 from its first evaluation and converges."""
+    X, x, box, per3, npd, nh = specimen(3)
+    X = X.copy()
+    X[4 * npd: 4 * npd + 3] = X[0:3] + np.array([0.9, 0.0, 0.0])  # H site 0 at 0.9 A from Pd site 0
+    fe = A.FreeEntropy(npd, nh, box, per3, precision="mixed")
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    with pytest.raises(A.A2CError) as e:
+        fe.eval(X)
+    assert e.value.code == A.A2C_ENONFINITE
+    fe64 = A.FreeEntropy(npd, nh, box, per3, precision="fp64")
+    fe64.set_occupancy(x)
+    fe64.findNeighbors(X)
+    f64, _ = fe64.eval(X)
+    assert np.isfinite(f64)
+    _, rep = fe.minimizeFreeEntropy(X, gatol=1e-6, grtol=0.0, max_it=400)
+    assert rep["reason"] != -4  # not stopped on the non-finite mixed value
+    assert rep["evals"] - rep["evals_fp64"] <= 1  # only the first evaluation was mixed
