@@ -395,6 +395,7 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   L.fd_eps = b->desc.fd_eps;
   L.tab = b->dtab;
   L.mixed_full = mixed_full_pairs();
+  L.full_pairs64 = fp64_full_pairs();
   int rc = fp64 ? run_eval64(L, nullptr) : run_eval32(L, nullptr);
   if (rc) return berr(b, rc, "fused eval launch failed");
   if (!sync) return A2C_OK;
@@ -809,9 +810,12 @@ const char* a2c_batch_last_error(const a2c_batch* b) { return b ? b->err.c_str()
 
 size_t a2c_batch_footprint(int32_t n_pd, int32_t n_h, int32_t lmvm_m) {
   const size_t N = (size_t)n_pd + n_h;
-  // state/gradient/staging 4N doubles x 4, per-site arrays ~5 doubles, lists
-  // ~90 int32 per site, optimizer (7 + 2m) x 4N doubles
-  return N * (4 * 32 + 40 + 90 * 4 + (7 + 2 * (size_t)std::max(lmvm_m, 1)) * 32) + ((size_t)1 << 20);
+  // state/gradient/staging 4N doubles x 4, per-site arrays ~5 doubles, ELL
+  // lists ~100 int32 per site, the owned/received halves of two pair lists
+  // per site (~28 wide: 2 + 1 int32 per owned/received slot and a 32-byte
+  // received share), optimizer (7 + 2m) x 4N doubles
+  return N * (4 * 32 + 40 + 100 * 4 + 2 * 28 * (3 * 4 + 32) + (7 + 2 * (size_t)std::max(lmvm_m, 1)) * 32) +
+         ((size_t)1 << 20);
 }
 
 size_t a2c_batch_mem_used(const a2c_batch* b) { return b ? b->mem_used : 0; }

@@ -132,8 +132,6 @@ int a2c_fe_set_precision(a2c_fe_ctx* c, int32_t precision) {
     c->lat_dirty = true;
   }
   c->desc.precision = precision;
-  if (precision == A2C_PREC_MIXED)
-    if (int rc = ensure_pair_buffers(c)) return rc;
   return A2C_OK;
 }
 

@@ -406,7 +406,7 @@ __global__ __launch_bounds__(TB) void k_dens_own(const LatDev* __restrict__ lats
       else pair_density2<false>(tab, wa, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
       const int rv = L.orv[e];
-      if (rv >= 0) P.rb1[(size_t)rv * P.rows + jl] = theirs;
+      if (rv >= 0) reinterpret_cast<float*>(P.rbuf)[(size_t)rv * P.rows + jl] = theirs;
     }
   }
   S.rho[s] = rho;
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lat
     for (int t = 0; t < n; ++t) {
       const size_t e = (size_t)t * L.rows + row;
       const int jl = L.rj[e];
-      rho += (side == 0 ? 1.0 : xo[jl]) * (double)L.rb1[e];
+      rho += (side == 0 ? 1.0 : xo[jl]) * (double)reinterpret_cast<const float*>(L.rbuf)[e];
     }
   }
   rho *= QW;
@@ -524,7 +524,8 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
       g0 += w * p.g[0]; g1 += w * p.g[1]; g2 += w * p.g[2];
       gs += w * p.gsi;
       const int rv = L.orv[ei];
-      if (rv >= 0) P.rb4[(size_t)rv * P.rows + jl] = make_float4(-p.g[0], -p.g[1], -p.g[2], p.gsj);
+      if (rv >= 0)
+        reinterpret_cast<float4*>(P.rbuf)[(size_t)rv * P.rows + jl] = make_float4(-p.g[0], -p.g[1], -p.g[2], p.gsj);
     }
   }
   double* __restrict__ G = S.G;
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lat
       for (int t = 0; t < n; ++t) {
         const size_t ei = (size_t)t * L.rows + row;
         const int jl = L.rj[ei];
-        const float4 v = L.rb4[ei];
+        const float4 v = reinterpret_cast<const float4*>(L.rbuf)[ei];
         const double w = side == 0 ? 1.0 : xo[jl];
         g0 += w * v.x; g1 += w * v.y; g2 += w * v.z;
         gs += w * v.w;

@@ -7,6 +7,13 @@
 //                per-site objective share, and a fixed-shape block reduction.
 //   k_final_sum  per lattice, a fixed-order sum of its block partials -> f.
 //
+// Analytic mode evaluates each unordered pair once (k_dens_own64 ->
+// k_dens_recv64 -> k_grad_own64 -> k_grad_recv64): the pair's owner computes
+// both ends' shares and writes the partner's share into the partner's
+// received slot, as the mixed path does.  FD mode keeps the per-site
+// full-list loop nest of the reference (every pair from both ends), since
+// its derivatives are the reference's central differences per site.
+//
 // One lane per site over ELL rows: a wave holds 64 consecutive sites of one
 // species, reads slot t of their rows as one coalesced load, and keeps the
 // per-site sums in registers (no atomics, so results are bitwise
@@ -172,7 +179,7 @@ template <int DM, int KIND>
 __device__ __forceinline__ void pair_accum(const double d[3], double ai, double aj, double Fi, double Fj,
                                            double eps, double& e, double g[3], double& gs) {
   double pe = 0.0, px = 0.0, py = 0.0, pz = 0.0, ps = 0.0;
-#pragma unroll
+#pragma unroll 1  // rolled, as in pair_both
   for (int p = 0; p < 12; ++p) {
     double r[3];
     point_r(p, d, ai, aj, r);
@@ -260,6 +267,212 @@ __global__ __launch_bounds__(TB) void k_grad64(const LatDev* __restrict__ lats, 
   if (threadIdx.x == 0) S.part[lb] = red[0];
 }
 
+// ================================================================ pair-once
+__device__ __forceinline__ const Ell& list_of(const LatDev& S, int list) {
+  return list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
+}
+// partner list: PdPd<->PdPd, PdH<->HPd, HH<->HH
+__device__ __forceinline__ const Ell& partner_list(const LatDev& S, int list) {
+  return list == 0 ? S.pdpd : (list == 1 ? S.hpd : (list == 2 ? S.pdh : S.hh));
+}
+
+__global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const double* __restrict__ X = S.X;
+  const double* __restrict__ xo = S.x;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  const Site a = load_site(X, s);
+  const double ai = SQ6 * a.s;
+  double rho = 0.0;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {  // Pd neighbors, then H neighbors
+    const int list = pd ? side : 2 + side;
+    const Ell& L = list_of(S, list);
+    const Ell& P = partner_list(S, list);
+    double* __restrict__ share = reinterpret_cast<double*>(P.rbuf);
+    const int n = L.ocnt[row];
+    for (int t = 0; t < n; ++t) {
+      const size_t e = (size_t)t * L.rows + row;
+      const int jl = L.oj[e];
+      const Site b = load_site(X, side == 0 ? jl : S.npd + jl);
+      double d[3];
+      pair_vec(S, a, b, d);
+      const double aj = SQ6 * b.s;
+      double mine = 0.0, theirs = 0.0;  // neighbor-species density at s, s-species density at the neighbor
+#pragma unroll 1
+      for (int p = 0; p < 12; ++p) {
+        double r[3];
+        point_r(p, d, ai, aj, r);
+        const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
+        const double fh = (side == 1 || !pd) ? eam::f_H(rn) : 0.0;
+        mine += side == 0 ? fpd : fh;
+        theirs += pd ? fpd : fh;
+      }
+      rho += (side == 0 ? 1.0 : xo[jl]) * mine;
+      const int rv = L.orv[e];
+      if (rv >= 0) share[(size_t)rv * P.rows + jl] = theirs;
+    }
+  }
+  S.rho[s] = rho;
+}
+
+__global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const double* __restrict__ xo = S.x;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  double rho = S.rho[s];
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const Ell& L = list_of(S, pd ? side : 2 + side);
+    const double* __restrict__ share = reinterpret_cast<const double*>(L.rbuf);
+    const int n = L.rcnt[row];
+    for (int t = 0; t < n; ++t) {
+      const size_t e = (size_t)t * L.rows + row;
+      rho += (side == 0 ? 1.0 : xo[L.rj[e]]) * share[e];
+    }
+  }
+  rho *= QW;
+  S.rho[s] = rho;
+  const double xs = pd ? 1.0 : xo[row];
+  const eam::VD F = pd ? eam::F_Pd_vd(rho) : eam::F_H_vd(rho);
+  S.Fp[s] = F.d;
+  S.emb[s] = xs * F.v;
+}
+
+// One owned pair: phi summed over the 12 points, the q-gradient g of site s
+// (the partner's is -g), and the sigma terms of both ends (site s: its own
+// six points; the partner: the six points that displace it, seen from its
+// side with r -> -r).
+template <int KIND>
+__device__ __forceinline__ void pair_both(const double d[3], double ai, double aj, double Fi, double Fj,
+                                          double& pe, double g[3], double& gsi, double& gsj) {
+  pe = 0.0;
+  g[0] = g[1] = g[2] = 0.0;
+  gsi = gsj = 0.0;
+#pragma unroll 1  // rolled: the FP64 closed forms keep ~140 VGPRs live (3 waves per SIMD) instead of ~260 (1)
+  for (int p = 0; p < 12; ++p) {
+    double r[3];
+    point_r(p, d, ai, aj, r);
+    const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double phi, c;
+    pair_point<A2C_DERIV_ANALYTIC, KIND>(rn, Fi, Fj, 0.0, phi, c);
+    pe += phi;
+    const double u = c / rn;
+    g[0] += u * r[0];
+    g[1] += u * r[1];
+    g[2] += u * r[2];
+    const int ax = (p >> 1) % 3;
+    if (p < 6) gsi += ((p & 1) ? -u : u) * r[ax];
+    else gsj += ((p & 1) ? u : -u) * r[ax];
+  }
+}
+
+__global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  if (s >= N) return;
+  const double* __restrict__ X = S.X;
+  const double* __restrict__ xo = S.x;
+  const double* __restrict__ Fp = S.Fp;
+  const bool pd = s < S.npd;
+  const int row = pd ? s : s - S.npd;
+  const Site a = load_site(X, s);
+  const double ai = SQ6 * a.s;
+  const double Fi = Fp[s];
+  double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, gs = 0.0;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int list = pd ? side : 2 + side;
+    const Ell& L = list_of(S, list);
+    const Ell& P = partner_list(S, list);
+    double4* __restrict__ share = reinterpret_cast<double4*>(P.rbuf);
+    const int n = L.ocnt[row];
+    for (int t = 0; t < n; ++t) {
+      const size_t ei = (size_t)t * L.rows + row;
+      const int jl = L.oj[ei];
+      const int j = side == 0 ? jl : S.npd + jl;
+      const Site b = load_site(X, j);
+      double d[3];
+      pair_vec(S, a, b, d);
+      const double aj = SQ6 * b.s, Fj = Fp[j];
+      double pe, pg[3], psi, psj;
+      if (list == 0) pair_both<0>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
+      else if (list == 1) pair_both<1>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
+      else if (list == 2) pair_both<2>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
+      else pair_both<3>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
+      const double w = side == 0 ? 1.0 : xo[jl];
+      e += w * pe;
+      g0 += w * pg[0]; g1 += w * pg[1]; g2 += w * pg[2];
+      gs += w * psi;
+      const int rv = L.orv[ei];
+      if (rv >= 0) share[(size_t)rv * P.rows + jl] = make_double4(-pg[0], -pg[1], -pg[2], psj);
+    }
+  }
+  double* __restrict__ G = S.G;
+  G[4 * (size_t)s + 0] = g0;
+  G[4 * (size_t)s + 1] = g1;
+  G[4 * (size_t)s + 2] = g2;
+  G[4 * (size_t)s + 3] = gs;
+  S.epair[s] = e;
+}
+
+__global__ __launch_bounds__(TB) void k_grad_recv64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
+  __shared__ double red[TB];
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int lb = blockIdx.x - S.blk0;
+  const int s = lb * TB + threadIdx.x;
+  const int N = S.npd + S.nh;
+  double fs = 0.0;
+  if (s < N) {
+    const double* __restrict__ xo = S.x;
+    const bool pd = s < S.npd;
+    const int row = pd ? s : s - S.npd;
+    double* __restrict__ G = S.G;
+    double g0 = G[4 * (size_t)s], g1 = G[4 * (size_t)s + 1], g2 = G[4 * (size_t)s + 2], gs = G[4 * (size_t)s + 3];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const Ell& L = list_of(S, pd ? side : 2 + side);
+      const double4* __restrict__ share = reinterpret_cast<const double4*>(L.rbuf);
+      const int n = L.rcnt[row];
+      for (int t = 0; t < n; ++t) {
+        const size_t ei = (size_t)t * L.rows + row;
+        const double4 v = share[ei];
+        const double w = side == 0 ? 1.0 : xo[L.rj[ei]];
+        g0 += w * v.x; g1 += w * v.y; g2 += w * v.z;
+        gs += w * v.w;
+      }
+    }
+    const double xi = pd ? 1.0 : xo[row];
+    const double sg = S.X[4 * (size_t)s + 3];
+    const double T = S.T;
+    const double V = S.emb[s] + QW * xi * S.epair[s];  // owned pairs carry their full energy
+    const double sc = QW * xi / T;
+    G[4 * (size_t)s + 0] = sc * g0;
+    G[4 * (size_t)s + 1] = sc * g1;
+    G[4 * (size_t)s + 2] = sc * g2;
+    G[4 * (size_t)s + 3] = SQ6 * sc * gs - 3.0 * KB * xi / sg;
+    fs = V / T - 3.0 * KB * xi * log(sg);
+  }
+  red[threadIdx.x] = fs;
+  __syncthreads();
+#pragma unroll
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S.part[lb] = red[0];
+}
+
 }  // namespace
 
 // One block per lattice: fixed-order sum of its block partials.
@@ -289,6 +502,26 @@ int run_density64(const Launch& L) {
 }
 
 int run_eval64(const Launch& L, Timing* tm) {
+  if (L.deriv_mode == A2C_DERIV_ANALYTIC && !L.full_pairs64) {
+    if (L.nblk > 0) {
+      if (tm) tm->mark(L.stream, "dens_own64");
+      hipLaunchKernelGGL(k_dens_own64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
+      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+      if (tm) tm->mark(L.stream, "dens_recv64");
+      hipLaunchKernelGGL(k_dens_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
+      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+      if (tm) tm->mark(L.stream, "grad_own64");
+      hipLaunchKernelGGL(k_grad_own64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
+      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+      if (tm) tm->mark(L.stream, "grad_recv64");
+      hipLaunchKernelGGL(k_grad_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
+      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+    }
+    if (tm) tm->mark(L.stream, "final_sum");
+    if (int rc = run_final_sum(L)) return rc;
+    if (tm) tm->mark(L.stream, nullptr);
+    return A2C_OK;
+  }
   if (tm) tm->mark(L.stream, "density64");
   if (int rc = run_density64(L)) return rc;
   if (tm) tm->mark(L.stream, "grad64");
@@ -319,10 +552,18 @@ bool mixed_full_pairs() {
   }();
   return full;
 }
+bool fp64_full_pairs() {
+  static const bool full = [] {
+    const char* v = getenv("A2C_FP64_PAIRS");
+    return v && std::string(v) == "full";
+  }();
+  return full;
+}
 
 static Launch single_launch(a2c_fe_ctx* c) {
   Launch L;
   L.mixed_full = mixed_full_pairs();
+  L.full_pairs64 = fp64_full_pairs();
   L.lats = c->dlat;
   L.b2l = c->db2l;
   L.nblk = site_blocks(c->N);

@@ -21,20 +21,26 @@ struct Ell {
   int32_t width = 0;
   int32_t* idx = nullptr;  // width * rows
   int32_t* cnt = nullptr;  // rows
-  // Mixed path (each unordered pair evaluated once, by its owner).  Every
-  // row is split, keeping its order, into the entries this row owns and the
-  // entries it receives; both halves are compact (slot-major, no gaps), so a
-  // wave's lanes step through their halves without masking each other off.
+  // Pair-once evaluation (each unordered pair evaluated once, by its owner).
+  // Every row is split, keeping its order, into the entries this row owns
+  // and the entries it receives; both halves are compact (slot-major, no
+  // gaps), so a wave's lanes step through their halves without masking each
+  // other off.  Widths are the longest half over all rows.
+  int32_t owidth = 0, rwidth = 0;
   int32_t* ocnt = nullptr;  // rows: owned entries
-  int32_t* oj = nullptr;    // width * rows: neighbor of owned entry t
-  int32_t* orv = nullptr;   // width * rows: slot of the pair in the partner's received half
+  int32_t* oj = nullptr;    // owidth * rows: neighbor of owned entry t
+  int32_t* orv = nullptr;   // owidth * rows: slot of the pair in the partner's received half
   int32_t* rcnt = nullptr;  // rows: received entries
-  int32_t* rj = nullptr;    // width * rows: neighbor of received entry t
-  int32_t* rpos = nullptr;  // width * rows: full-row slot -> received slot (-1 if owned)
-  float* rb1 = nullptr;     // width * rows: received density shares
-  float4* rb4 = nullptr;    // width * rows: received gradient shares
-  int64_t cap = 0;          // allocated entries of idx (grow-only across rebuilds)
-  int64_t pcap = 0;         // allocated entries of the split arrays
+  int32_t* rj = nullptr;    // rwidth * rows: neighbor of received entry t
+  // Received shares, written by the partner's owner pass and read by this
+  // row's receive pass of the same evaluation: rwidth * rows entries of a
+  // float (density) or float4 (gradient) in the mixed passes, a double or
+  // double4 in the FP64 passes -- one buffer serves all four.  During the
+  // split it holds the map full-row slot -> received slot.
+  void* rbuf = nullptr;
+  int64_t ocap = 0, rcap = 0;  // allocated entries of oj/orv and of rj
+  int64_t bcap = 0;            // allocated bytes of rbuf
+  int64_t cap = 0;             // allocated entries of idx (grow-only across rebuilds)
 };
 
 // Device view of one lattice: what every kernel needs.  The eval kernels
@@ -157,9 +163,11 @@ struct Launch {
   int deriv_mode;
   double fd_eps;
   const float4* tab;  // mixed path
-  bool mixed_full = false;  // mixed path: every pair from both ends (A/B reference)
+  bool mixed_full = false;    // mixed path: every pair from both ends (A/B reference)
+  bool full_pairs64 = false;  // FP64 analytic path: every pair from both ends (A/B reference)
 };
 bool mixed_full_pairs();  // A2C_MIXED_PAIRS=full selects the both-ends kernels
+bool fp64_full_pairs();   // A2C_FP64_PAIRS=full likewise for the FP64 kernels
 int run_density64(const Launch& L);
 int run_eval64(const Launch& L, Timing* tm);
 int run_eval32(const Launch& L, Timing* tm);

@@ -212,15 +212,41 @@ __global__ __launch_bounds__(256) void k_width(NbrArgs a) {
   if (threadIdx.x < 6) atomicMax(&a.maxw[threadIdx.x], red[threadIdx.x][0]);
 }
 
-// Split of each pair-list row into owned and received halves (the mixed
-// path evaluates each unordered pair once).  Ownership of {s, j} (global
-// site indices): (s < j) XOR ((s ^ j) & 1), which hands each site about half
-// of every list.  Both halves keep the row's (sorted) order.
+// Split of each pair-list row into owned and received halves (pair-once
+// evaluation).  Ownership of {s, j} (global site indices):
+// (s < j) XOR ((s ^ j) & 1), which hands each site about half of every list.
+// Both halves keep the row's (sorted) order.
 __device__ __forceinline__ bool pair_owner(int s, int j) { return (s < j) != (((s ^ j) & 1) != 0); }
+
+__global__ void k_own_count(Ell L, int s_off, int j_off) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L.rows) return;
+  const int n = L.cnt[r];
+  const int s = s_off + r;
+  int no = 0;
+  for (int t = 0; t < n; ++t) no += pair_owner(s, j_off + L.idx[(size_t)t * L.rows + r]);
+  L.ocnt[r] = no;
+  L.rcnt[r] = n - no;
+}
+
+// max over rows of a[r], one atomicMax per block
+__global__ __launch_bounds__(256) void k_rowmax(const int32_t* __restrict__ a, int n, int32_t* out) {
+  __shared__ int32_t red[256];
+  int m = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = max(m, a[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
 
 __global__ void k_split(Ell L, int s_off, int j_off) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= L.rows) return;
+  int32_t* rpos = reinterpret_cast<int32_t*>(L.rbuf);  // full-row slot -> received slot (-1 if owned)
   const int n = L.cnt[r];
   const int s = s_off + r;
   int no = 0, nr = 0;
@@ -229,16 +255,14 @@ __global__ void k_split(Ell L, int s_off, int j_off) {
     const int j = L.idx[e];
     if (pair_owner(s, j_off + j)) {
       L.oj[(size_t)no * L.rows + r] = j;
-      L.rpos[e] = -1;
+      rpos[e] = -1;
       ++no;
     } else {
       L.rj[(size_t)nr * L.rows + r] = j;
-      L.rpos[e] = nr;
+      rpos[e] = nr;
       ++nr;
     }
   }
-  L.ocnt[r] = no;
-  L.rcnt[r] = nr;
 }
 
 // For every owned entry (r, j): the received slot of r in the partner row j,
@@ -247,6 +271,7 @@ __global__ void k_split(Ell L, int s_off, int j_off) {
 __global__ void k_orv(Ell L, Ell P) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= L.rows) return;
+  const int32_t* prpos = reinterpret_cast<const int32_t*>(P.rbuf);
   const int n = L.ocnt[r];
   for (int t = 0; t < n; ++t) {
     const size_t e = (size_t)t * L.rows + r;
@@ -259,7 +284,7 @@ __global__ void k_orv(Ell L, Ell P) {
       if (v < r) lo = mid + 1;
       else hi = mid - 1;
     }
-    L.orv[e] = k >= 0 ? P.rpos[(size_t)k * P.rows + j] : -1;
+    L.orv[e] = k >= 0 ? prpos[(size_t)k * P.rows + j] : -1;
   }
 }
 
@@ -376,48 +401,82 @@ int nbr_build(a2c_fe_ctx* c) {
   c->stats.max_neib = hm[5];
   for (int w = 0; w < 5; ++w) c->stats.width[w] = hm[w];
   c->have_nbr = true;
-  if (c->desc.precision == A2C_PREC_MIXED)
-    if (int rc = ensure_pair_buffers(c)) return rc;
-  return A2C_OK;
+  return ensure_pair_buffers(c);
 }
 
-// Owned/received split and contribution buffers of the four pair lists
-// (mixed path); the split is recomputed after every build.
+// Owned/received split and the received-share buffers of the four pair
+// lists; recomputed after every build.  Arrays grow only.
 int ensure_pair_buffers(a2c_fe_ctx* c) {
   if (!c->have_nbr) return A2C_OK;
   static const int partner[4] = {0, 2, 1, 3};  // PdPd, PdH<->HPd, HH
+  static const int s_sp[4] = {0, 0, 1, 1}, j_sp[4] = {0, 1, 0, 1};  // species of rows / of neighbors
   const int npd = c->desc.n_pd;
+  const int TB = 256;
+  // 1. per-row owned/received counts and their maxima
   for (int w = 0; w < 4; ++w) {
     Ell& L = c->ell[w];
-    const int64_t n = (int64_t)std::max(L.width, 1) * std::max(L.rows, 1);
-    if (n <= L.pcap) continue;
-    free_pair_buffers(L);
-    L.pcap = L.cap;  // same capacity as idx
-    const size_t rows = (size_t)std::max(L.rows, 1);
-    A2C_HIP(c, hipMalloc(&L.ocnt, sizeof(int32_t) * rows));
-    A2C_HIP(c, hipMalloc(&L.rcnt, sizeof(int32_t) * rows));
-    A2C_HIP(c, hipMalloc(&L.oj, sizeof(int32_t) * L.pcap));
-    A2C_HIP(c, hipMalloc(&L.orv, sizeof(int32_t) * L.pcap));
-    A2C_HIP(c, hipMalloc(&L.rj, sizeof(int32_t) * L.pcap));
-    A2C_HIP(c, hipMalloc(&L.rpos, sizeof(int32_t) * L.pcap));
-    A2C_HIP(c, hipMalloc(&L.rb1, sizeof(float) * L.pcap));
-    A2C_HIP(c, hipMalloc(&L.rb4, sizeof(float4) * L.pcap));
-  }
-  static const int s_sp[4] = {0, 0, 1, 1}, j_sp[4] = {0, 1, 0, 1};  // species of rows / of neighbors
-  for (int w = 0; w < 4; ++w) {
-    const Ell& L = c->ell[w];
-    if (L.rows > 0) {
-      hipLaunchKernelGGL(k_split, dim3((L.rows + 255) / 256), dim3(256), 0, c->stream, L, s_sp[w] * npd,
-                         j_sp[w] * npd);
-      A2C_HIP(c, hipGetLastError());
+    if (!L.ocnt) {
+      A2C_HIP(c, hipMalloc(&L.ocnt, sizeof(int32_t) * std::max(L.rows, 1)));
+      A2C_HIP(c, hipMalloc(&L.rcnt, sizeof(int32_t) * std::max(L.rows, 1)));
     }
   }
+  A2C_HIP(c, hipMemsetAsync(c->nb_misc, 0, sizeof(int) * 8, c->stream));
   for (int w = 0; w < 4; ++w) {
     const Ell& L = c->ell[w];
-    if (L.rows > 0) {
-      hipLaunchKernelGGL(k_orv, dim3((L.rows + 255) / 256), dim3(256), 0, c->stream, L, c->ell[partner[w]]);
-      A2C_HIP(c, hipGetLastError());
+    if (L.rows == 0) continue;
+    const int g = (L.rows + TB - 1) / TB;
+    hipLaunchKernelGGL(k_own_count, dim3(g), dim3(TB), 0, c->stream, L, s_sp[w] * npd, j_sp[w] * npd);
+    A2C_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_rowmax, dim3(std::min(g, 1024)), dim3(TB), 0, c->stream, L.ocnt, L.rows, c->nb_misc + 2 * w);
+    A2C_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_rowmax, dim3(std::min(g, 1024)), dim3(TB), 0, c->stream, L.rcnt, L.rows,
+                       c->nb_misc + 2 * w + 1);
+    A2C_HIP(c, hipGetLastError());
+  }
+  int wm[8];
+  A2C_HIP(c, hipMemcpyAsync(wm, c->nb_misc, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+  A2C_HIP(c, hipStreamSynchronize(c->stream));
+  // 2. storage
+  for (int w = 0; w < 4; ++w) {
+    Ell& L = c->ell[w];
+    const int64_t rows = std::max(L.rows, 1);
+    L.owidth = wm[2 * w];
+    L.rwidth = wm[2 * w + 1];
+    const int64_t on = (int64_t)std::max(L.owidth, 1) * rows, rn = (int64_t)std::max(L.rwidth, 1) * rows;
+    const int64_t bn = std::max<int64_t>(32 * rn, 4 * (int64_t)std::max(L.width, 1) * rows);
+    if (on > L.ocap) {
+      (void)hipFree(L.oj); (void)hipFree(L.orv);
+      L.oj = L.orv = nullptr;
+      L.ocap = on + 4 * rows;  // slack: small changes of the widest half reuse the arrays
+      A2C_HIP(c, hipMalloc(&L.oj, sizeof(int32_t) * L.ocap));
+      A2C_HIP(c, hipMalloc(&L.orv, sizeof(int32_t) * L.ocap));
     }
+    if (rn > L.rcap) {
+      (void)hipFree(L.rj);
+      L.rj = nullptr;
+      L.rcap = rn + 4 * rows;
+      A2C_HIP(c, hipMalloc(&L.rj, sizeof(int32_t) * L.rcap));
+    }
+    if (bn > L.bcap) {
+      (void)hipFree(L.rbuf);
+      L.rbuf = nullptr;
+      L.bcap = bn + 32 * 4 * rows;
+      A2C_HIP(c, hipMalloc(&L.rbuf, L.bcap));
+    }
+  }
+  // 3. split, then the partner slots (needs every list's split)
+  for (int w = 0; w < 4; ++w) {
+    const Ell& L = c->ell[w];
+    if (L.rows == 0) continue;
+    hipLaunchKernelGGL(k_split, dim3((L.rows + TB - 1) / TB), dim3(TB), 0, c->stream, L, s_sp[w] * npd,
+                       j_sp[w] * npd);
+    A2C_HIP(c, hipGetLastError());
+  }
+  for (int w = 0; w < 4; ++w) {
+    const Ell& L = c->ell[w];
+    if (L.rows == 0) continue;
+    hipLaunchKernelGGL(k_orv, dim3((L.rows + TB - 1) / TB), dim3(TB), 0, c->stream, L, c->ell[partner[w]]);
+    A2C_HIP(c, hipGetLastError());
   }
   if (!c->depair) A2C_HIP(c, hipMalloc(&c->depair, sizeof(double) * std::max(c->N, 1)));
   A2C_HIP(c, hipStreamSynchronize(c->stream));
@@ -427,11 +486,11 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
 
 void free_pair_buffers(Ell& L) {
   (void)hipFree(L.ocnt); (void)hipFree(L.rcnt); (void)hipFree(L.oj); (void)hipFree(L.orv);
-  (void)hipFree(L.rj); (void)hipFree(L.rpos); (void)hipFree(L.rb1); (void)hipFree(L.rb4);
-  L.ocnt = L.rcnt = L.oj = L.orv = L.rj = L.rpos = nullptr;
-  L.rb1 = nullptr;
-  L.rb4 = nullptr;
-  L.pcap = 0;
+  (void)hipFree(L.rj); (void)hipFree(L.rbuf);
+  L.ocnt = L.rcnt = L.oj = L.orv = L.rj = nullptr;
+  L.rbuf = nullptr;
+  L.ocap = L.rcap = L.bcap = 0;
+  L.owidth = L.rwidth = 0;
 }
 
 }  // namespace a2c

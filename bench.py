@@ -249,7 +249,7 @@ def main():
     else:
         tm, wall, f, G = tm64, wall64, f64, G64
         args.steps = max(3, args.steps // 10)
-        gkern = "grad64"
+        gkern = "grad_own64" if "grad_own64" in tm["kernels_ms"] else "grad64"
     ms = tm["total_ms"] / args.steps
     kern = tm["kernels_ms"]
     gms = kern.get(gkern, float("nan"))
