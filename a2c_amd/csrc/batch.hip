@@ -55,7 +55,7 @@ struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 ze
   double* z;        // x (0), gp (1), d (2)
   const double* x;  // x (1, 2, 3)
   const double* y;  // g (1), d (3)
-  double lb;
+  double lb, ub;
   double* out;      // step max (3)
   int32_t nsite, blk0;
 };
@@ -137,18 +137,20 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
   if (s < o.nsite) {
     const size_t i = 4 * (size_t)s;
     if (MODE == 0) {
-      o.z[i + 3] = fmax(o.z[i + 3], o.lb);
+      o.z[i + 3] = fmin(fmax(o.z[i + 3], o.lb), o.ub);
     } else if (MODE == 1) {
       o.z[i] = o.y[i];
       o.z[i + 1] = o.y[i + 1];
       o.z[i + 2] = o.y[i + 2];
-      const double gs = o.y[i + 3];
-      o.z[i + 3] = (o.x[i + 3] <= o.lb && gs > 0.0) ? 0.0 : gs;
+      const double gs = o.y[i + 3], xs = o.x[i + 3];
+      o.z[i + 3] = ((xs <= o.lb && gs > 0.0) || (xs >= o.ub && gs < 0.0)) ? 0.0 : gs;
     } else if (MODE == 2) {
-      if (o.x[i + 3] <= o.lb && o.z[i + 3] < 0.0) o.z[i + 3] = 0.0;
+      const double xs = o.x[i + 3], ds = o.z[i + 3];
+      if ((xs <= o.lb && ds < 0.0) || (xs >= o.ub && ds > 0.0)) o.z[i + 3] = 0.0;
     } else {
       const double ds = o.y[i + 3];
       if (ds < 0.0) m = (o.x[i + 3] - o.lb) / (-ds);
+      else if (ds > 0.0 && o.ub < HUGE_VAL) m = (o.ub - o.x[i + 3]) / ds;
     }
   }
   if (MODE != 3) return;
@@ -460,7 +462,7 @@ int eval_slots(a2c_batch* b, const std::vector<int>& ids) {
 
 int proj_gnorm(a2c_batch* b, const std::vector<int>& ids) {
   if (int rc = seg_bound<1>(b, ids, [](SegBound& o, Slot& s) {
-        o.z = s.gp; o.x = s.x; o.y = s.g; o.lb = s.opts.sigma_lb;
+        o.z = s.gp; o.x = s.x; o.y = s.g; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
       }))
     return rc;
   std::vector<double> v;
@@ -480,7 +482,10 @@ int switch_to_fp64(a2c_batch* b, const std::vector<int>& ids) {
 // first evaluation of newly started slots
 int lmvm_init(a2c_batch* b, const std::vector<int>& ids) {
   if (ids.empty()) return A2C_OK;
-  if (int rc = seg_bound<0>(b, ids, [](SegBound& o, Slot& s) { o.z = s.x; o.lb = s.opts.sigma_lb; })) return rc;
+  if (int rc = seg_bound<0>(b, ids, [](SegBound& o, Slot& s) {
+        o.z = s.x; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+      }))
+    return rc;
   if (int rc = eval_slots(b, ids)) return rc;
   if (int rc = proj_gnorm(b, ids)) return rc;
   std::vector<int> sw;
@@ -582,7 +587,9 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
     if (int rc = seg_axpby(b, sd, [&](SegAx& o, Slot& s) { o.z = s.d; o.x = s.g; o.y = s.g; o.a = -del[kk++]; o.b = 0.0; }))
       return rc;
   }
-  if (int rc = seg_bound<2>(b, ids, [](SegBound& o, Slot& s) { o.z = s.d; o.x = s.x; o.lb = s.opts.sigma_lb; }))
+  if (int rc = seg_bound<2>(b, ids, [](SegBound& o, Slot& s) {
+        o.z = s.d; o.x = s.x; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+      }))
     return rc;
   if (int rc = seg_dot(b, ids, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
   for (size_t k = 0; k < ids.size(); ++k) b->slots[ids[k]].gd = v[2 * k];
@@ -609,7 +616,9 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
   if (go.empty()) return A2C_OK;
   // line search setup
   std::vector<double> smax;
-  if (int rc = seg_bound<3>(b, go, [](SegBound& o, Slot& s) { o.x = s.x; o.y = s.d; o.lb = s.opts.sigma_lb; }, &smax))
+  if (int rc = seg_bound<3>(b, go, [](SegBound& o, Slot& s) {
+        o.x = s.x; o.y = s.d; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+      }, &smax))
     return rc;
   if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.xo; o.x = s.x; o.y = s.x; o.a = 1.0; o.b = 0.0; })) return rc;
   if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.go; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; })) return rc;
@@ -900,7 +909,8 @@ int a2c_batch_lmvm_start(a2c_batch* b, int32_t slot, const a2c_lmvm_opts* o) {
   if (!b || !o) return A2C_EINVAL;
   b->err.clear();
   if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return berr(b, A2C_EINVAL, "bad slot");
-  if (o->m < 1 || o->m > 64 || o->max_it < 0 || o->max_funcs < 1 || !(o->sigma_lb >= 0))
+  if (o->m < 1 || o->m > 64 || o->max_it < 0 || o->max_funcs < 1 || !(o->sigma_lb >= 0) ||
+      !(o->sigma_ub > o->sigma_lb))
     return berr(b, A2C_EINVAL, "bad options");
   Slot& s = b->slots[slot];
   s.opts = *o;
@@ -975,6 +985,7 @@ extern "C" void a2c_lmvm_opts_default(a2c_lmvm_opts* o) {
   o->sigma_lb = 1e-6;  // formVariableBounds, minimizer.cpp:815,822
   o->mixed_switch = 1;
   o->mixed_gtol = 1e-5;
+  o->sigma_ub = HUGE_VAL;  // formVariableBounds leaves sigma unbounded above
 }
 
 extern "C" int a2c_fe_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* opts, double* X_out,
@@ -983,7 +994,8 @@ extern "C" int a2c_fe_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_o
   c->err.clear();
   if (!X0 || !X_out || !opts || !report) return a2c::set_err(c, A2C_EINVAL, "null argument");
   if (!c->have_nbr || !c->have_x) return a2c::set_err(c, A2C_ESTATE, "neighbors/occupancy not set");
-  if (opts->m < 1 || opts->max_it < 0 || opts->max_funcs < 1 || !(opts->sigma_lb >= 0))
+  if (opts->m < 1 || opts->max_it < 0 || opts->max_funcs < 1 || !(opts->sigma_lb >= 0) ||
+      !(opts->sigma_ub > opts->sigma_lb))
     return a2c::set_err(c, A2C_EINVAL, "bad options");
   return a2c::lmvm_minimize(c, X0, opts, X_out, report);
 }

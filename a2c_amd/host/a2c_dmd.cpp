@@ -20,7 +20,7 @@
 //   Diffusion.GammaBoundary, Diffusion.SubsurfaceLayers (reservoir sites:
 //   H sites in the first layers of cells along x), Diffusion.xLb, .xUb
 //   Solver.Precision (Mixed/FP64), Solver.Derivative (Analytic/FD),
-//   Solver.GradientTolerance, Solver.MaxIts
+//   Solver.GradientTolerance, Solver.MaxIts, Solver.SigmaUpperBound
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -107,6 +107,7 @@ int main(int argc, char** argv) {
   p.lmvm.gatol = deck.num("Solver.GradientTolerance", 1e-8);
   p.lmvm.grtol = 0.0;
   p.lmvm.max_it = (int)deck.integer("Solver.MaxIts", 2000);
+  p.lmvm.sigma_ub = deck.num("Solver.SigmaUpperBound", p.lmvm.sigma_ub);
   const int steps = (int)deck.integer("Time.MaxIts", 10);
   const std::string prefix = deck.str("Output.Prefix", "");
   const std::string base = deck.str("Output.Solution", "solution");

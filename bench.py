@@ -172,9 +172,13 @@ def dmd(args):
     # means for the 6^3-cell lattice (6,912 DOF), i.e. both tolerances scaled
     # by sqrt(DOF / 6912); an absolute 1e-8 over 8.4 M DOF sits at the FP64
     # evaluation's own noise floor.
+    # Widths are capped at 0.5 A (relaxed widths are 0.07-0.16 A): the
+    # functional decreases without bound as one width grows past a barrier
+    # near 1 A (-3 kB x ln sigma), and over hundreds of thousands of packets
+    # an uncapped line search eventually steps across it.
     scale = np.sqrt(4.0 * (npd + nh) / 6912.0)
     prm = A.dmd_params(dt=5e-13, nu=1e13, Qm=0.23, xLb=1e-3, xUb=1 - 1e-16, gamma_bd=1e300,
-                       lmvm={"gatol": 1e-8 * scale, "grtol": 0.0, "mixed_gtol": 1e-5 * scale})
+                       lmvm={"gatol": 1e-8 * scale, "grtol": 0.0, "mixed_gtol": 1e-5 * scale, "sigma_ub": 0.5})
     g = np.zeros(nh)
     fh = np.zeros(nh, np.int32)
     h0 = x.sum()
@@ -182,8 +186,10 @@ def dmd(args):
     t0 = time.perf_counter()
     for st in range(args.dmd_steps):
         reps.append(A.dmd_step(fe, prm, X, x, g, fh))
-        log("dmd step %d: f=%.12g it=%d dH=%.3e" % (st, reps[-1]["f"], reps[-1]["lmvm_iterations"],
-                                                    reps[-1]["total_h_after"] - reps[-1]["total_h_before"]))
+        sg = X.reshape(-1, 4)[:, 3]
+        log("dmd step %d: f=%.12g it=%d reason=%d dH=%.3e sigma %.4f..%.4f" % (
+            st, reps[-1]["f"], reps[-1]["lmvm_iterations"], reps[-1]["lmvm_reason"],
+            reps[-1]["total_h_after"] - reps[-1]["total_h_before"], sg.min(), sg.max()))
     wall = time.perf_counter() - t0
     fe.close()
     drift = max(abs(r["total_h_after"] - r["total_h_before"]) for r in reps)
@@ -195,7 +201,7 @@ def dmd(args):
         "config": {"workload": "coupled DMD, %d^3 cells, slab x=0.6 in 8 layers, T=400K" % n, "sites": npd + nh},
         "max_step_abs_dH": drift, "total_H_start": h0, "total_H_end": float(x.sum()),
         "dH_note": "total H changes only where the xLb clamp (integrate.cpp:79-81) lifts sites that lose H",
-        "gatol": 1e-8 * scale,
+        "gatol": 1e-8 * scale, "sigma_ub": 0.5,
         "lmvm_iterations": [r["lmvm_iterations"] for r in reps],
     }
 

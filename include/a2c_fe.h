@@ -134,6 +134,12 @@ typedef struct a2c_lmvm_opts {
   int32_t mixed_switch; /* MIXED precision: switch to FP64 evals once
                            ||g|| <= mixed_gtol (1) */
   double mixed_gtol;    /* (1e-5) */
+  double sigma_ub;      /* upper bound on sigma (+inf, as formVariableBounds,
+                           minimizer.cpp:804-830).  The functional is
+                           unbounded below as sigma -> inf (the entropy term
+                           -3 kB x ln sigma outgrows the vanishing pair terms
+                           beyond a few A), so long relaxations of large
+                           lattices may set a finite cap. */
 } a2c_lmvm_opts;
 
 typedef struct a2c_lmvm_report {
@@ -148,7 +154,7 @@ typedef struct a2c_lmvm_report {
 void a2c_lmvm_opts_default(a2c_lmvm_opts* o);
 
 /* Replaces Minimizer::minimizeFreeEntropy (minimizer.cpp:548-586): minimizes
- * f over X (sigma >= sigma_lb), starting from X0; neighbor lists and x0 stay
+ * f over X (sigma_lb <= sigma <= sigma_ub), starting from X0; neighbor lists and x0 stay
  * frozen.  X_out may alias X0 (host arrays of 4N doubles). */
 int a2c_fe_minimize(a2c_fe_ctx* ctx, const double* X0, const a2c_lmvm_opts* opts,
                     double* X_out, a2c_lmvm_report* report);

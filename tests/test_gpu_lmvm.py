@@ -94,3 +94,23 @@ def test_sigma_bound_respected():
     Xr, rep = fe.minimizeFreeEntropy(X, max_it=20)
     assert np.all(Xr[3::4] >= 1e-6)
     assert rep["f"] < fe.eval(X)[0]
+
+
+def test_sigma_upper_bound_caps_and_is_kkt():
+    """sigma_ub: widths stay in [lb, ub]; at the cap the sigma gradient points
+    outward (df/dsigma < 0) and the projected gradient meets gatol."""
+    X, x, box, per, npd, nh = specimen(3, seed=6, sigma=(0.10, 0.10))[:6]
+    fe = A.FreeEntropy(npd, nh, box, per, T=300.0)
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    ub = 0.09  # below the relaxed H widths (~0.15), above the Pd widths (~0.08)
+    Xr, rep = fe.minimizeFreeEntropy(X, sigma_ub=ub, gatol=1e-8, grtol=0.0)
+    assert rep["reason"] > 0
+    s = Xr[3::4]
+    assert np.all(s <= ub) and np.all(s >= 1e-6)
+    at_cap = s >= ub
+    assert at_cap.any() and not at_cap.all()
+    _, G = fe.eval(Xr)
+    assert np.all(G[3::4][at_cap] < 0.0)
+    with pytest.raises(A.A2CError):
+        fe.minimizeFreeEntropy(X, sigma_ub=1e-6)  # ub must exceed lb

@@ -114,6 +114,27 @@ def test_sigma_gradient_entropy_dominates_at_small_sigma():
     assert (gs[: orc.n_pd] < 0).all()  # -3 kB / sigma wins: f decreases as sigma grows
 
 
+def test_objective_is_unbounded_as_one_width_grows():
+    """Spreading one Pd packet: past a barrier near 1 A its pair terms vanish
+    and only the entropy -3 kB x ln(sigma) / the embedding at rho -> 0 remain,
+    so df/dsigma -> -3 kB / sigma exactly and f decreases without bound (the
+    reason for the optimizer's optional sigma_ub)."""
+    orc, X, x = make(n=4, sigma=(0.10, 0.10), x=(1e-3, 1e-3), T=400.0)
+    f0, _ = orc.eval(X, x)
+    fs, gs = [], []
+    for s in (1.0, 30.0, 100.0, 300.0):
+        Y = X.copy()
+        Y[3] = s
+        f, G = orc.eval(Y, x)
+        fs.append(f)
+        gs.append(G[3])
+    assert fs[0] > f0  # the barrier
+    assert fs[1] > fs[2] > fs[3]  # then downhill for good
+    for s, g in zip((30.0, 100.0, 300.0), gs[1:]):
+        assert g == pytest.approx(-3.0 * KB / s, rel=1e-9)
+    assert fs[2] - fs[3] == pytest.approx(3.0 * KB * np.log(3.0), rel=1e-9)
+
+
 def test_fd_mode_matches_analytic():
     """Reference FD derivatives (eps 1e-5) agree with the analytic ones up to
     the FD round-off of the reference's f_Pd polynomial form (~1e-6 rel)."""
