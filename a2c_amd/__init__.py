@@ -64,7 +64,7 @@ class LmvmOpts(C.Structure):
         ("m", C.c_int32), ("max_it", C.c_int32), ("max_funcs", C.c_int32),
         ("gatol", C.c_double), ("grtol", C.c_double), ("gttol", C.c_double),
         ("sigma_lb", C.c_double), ("mixed_switch", C.c_int32), ("mixed_gtol", C.c_double),
-        ("sigma_ub", C.c_double),
+        ("sigma_ub", C.c_double), ("precond", C.c_int32),
     ]
 
 

@@ -36,27 +36,42 @@ namespace {
 constexpr int TB = SITE_BLOCK;
 
 // ---------------------------------------------------------------- kernels
-struct SegAx {  // z = a x + b y on 4N doubles
+// Diagonal metric of the optimizer: every (q, sigma) component of a site is
+// scaled by 1/w, w = 1 for Pd sites and the occupancy x (floored at 1e-6) for
+// H sites.  Every term of the objective that involves an H packet carries
+// its occupancy, so its block of the Hessian scales with x; dividing by x
+// equalises the curvature of nearly empty and full sites.
+struct Metric {
+  const double* occ = nullptr;  // null: identity
+  int32_t npd = 0;
+};
+__device__ __forceinline__ double metric_inv(const Metric& m, int s) {
+  return (m.occ && s >= m.npd) ? 1.0 / fmax(m.occ[s - m.npd], 1e-6) : 1.0;
+}
+struct SegAx {  // z = a D^k x + b y on 4N doubles (k = 0, or 1 when scaled)
   double* z;
   const double* x;
   const double* y;
   double a, b;
+  Metric met;
+  int32_t scaled;
   int32_t nsite, blk0;
 };
-struct SegDot {  // out[0] = a1 . b1, out[1] = a2 . b2 (a2 may be null)
+struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null)
   const double* a1;
   const double* b1;
   const double* a2;
   const double* b2;
+  Metric met;
+  int32_t k1, k2;
   double* out;
   int32_t nsite, blk0;
 };
-struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero active d, 3 step max
+struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero the active part of d
   double* z;        // x (0), gp (1), d (2)
-  const double* x;  // x (1, 2, 3)
-  const double* y;  // g (1), d (3)
+  const double* x;  // x (1, 2)
+  const double* y;  // g (1)
   double lb, ub;
-  double* out;      // step max (3)
   int32_t nsite, blk0;
 };
 
@@ -65,8 +80,9 @@ __global__ __launch_bounds__(TB) void k_seg_axpby(const SegAx* __restrict__ ops,
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   if (s >= o.nsite) return;
   const size_t i = 4 * (size_t)s;
+  const double a = o.scaled ? o.a * metric_inv(o.met, s) : o.a;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) o.z[i + c] = o.a * o.x[i + c] + o.b * o.y[i + c];
+  for (int c = 0; c < 4; ++c) o.z[i + c] = a * o.x[i + c] + o.b * o.y[i + c];
 }
 
 __global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, const int32_t* __restrict__ b2l,
@@ -77,11 +93,14 @@ __global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, 
   double v1 = 0.0, v2 = 0.0;
   if (s < o.nsite) {
     const size_t i = 4 * (size_t)s;
+    const double di = metric_inv(o.met, s);
 #pragma unroll
     for (int c = 0; c < 4; ++c) v1 += o.a1[i + c] * o.b1[i + c];
+    if (o.k1) v1 *= o.k1 == 1 ? di : di * di;
     if (o.a2) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) v2 += o.a2[i + c] * o.b2[i + c];
+      if (o.k2) v2 *= o.k2 == 1 ? di : di * di;
     }
   }
   r1[threadIdx.x] = v1;
@@ -128,55 +147,23 @@ __global__ __launch_bounds__(1024) void k_seg_dot_final(const SegDot* __restrict
 }
 
 template <int MODE>
-__global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ ops, const int32_t* __restrict__ b2l,
-                                                  double* __restrict__ part) {
-  __shared__ double red[TB];
+__global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ ops, const int32_t* __restrict__ b2l) {
   const SegBound& o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
-  double m = 1e300;
-  if (s < o.nsite) {
-    const size_t i = 4 * (size_t)s;
-    if (MODE == 0) {
-      o.z[i + 3] = fmin(fmax(o.z[i + 3], o.lb), o.ub);
-    } else if (MODE == 1) {
-      o.z[i] = o.y[i];
-      o.z[i + 1] = o.y[i + 1];
-      o.z[i + 2] = o.y[i + 2];
-      const double gs = o.y[i + 3], xs = o.x[i + 3];
-      o.z[i + 3] = ((xs <= o.lb && gs > 0.0) || (xs >= o.ub && gs < 0.0)) ? 0.0 : gs;
-    } else if (MODE == 2) {
-      const double xs = o.x[i + 3], ds = o.z[i + 3];
-      if ((xs <= o.lb && ds < 0.0) || (xs >= o.ub && ds > 0.0)) o.z[i + 3] = 0.0;
-    } else {
-      const double ds = o.y[i + 3];
-      if (ds < 0.0) m = (o.x[i + 3] - o.lb) / (-ds);
-      else if (ds > 0.0 && o.ub < HUGE_VAL) m = (o.ub - o.x[i + 3]) / ds;
-    }
+  if (s >= o.nsite) return;
+  const size_t i = 4 * (size_t)s;
+  if (MODE == 0) {
+    o.z[i + 3] = fmin(fmax(o.z[i + 3], o.lb), o.ub);
+  } else if (MODE == 1) {
+    o.z[i] = o.y[i];
+    o.z[i + 1] = o.y[i + 1];
+    o.z[i + 2] = o.y[i + 2];
+    const double gs = o.y[i + 3], xs = o.x[i + 3];
+    o.z[i + 3] = ((xs <= o.lb && gs > 0.0) || (xs >= o.ub && gs < 0.0)) ? 0.0 : gs;
+  } else {
+    const double xs = o.x[i + 3], ds = o.z[i + 3];
+    if ((xs <= o.lb && ds < 0.0) || (xs >= o.ub && ds > 0.0)) o.z[i + 3] = 0.0;
   }
-  if (MODE != 3) return;
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = TB / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(1024) void k_seg_min_final(const SegBound* __restrict__ ops,
-                                                        const double* __restrict__ part) {
-  __shared__ double red[1024];
-  const SegBound& o = ops[blockIdx.x];
-  const int n = site_blocks(o.nsite);
-  double m = 1e300;
-  for (int i = threadIdx.x; i < n; i += 1024) m = fmin(m, part[o.blk0 + i]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *o.out = red[0];
 }
 
 }  // namespace
@@ -212,11 +199,16 @@ struct Slot {
   double *x = nullptr, *g = nullptr, *gp = nullptr, *d = nullptr, *xo = nullptr, *go = nullptr, *q = nullptr;
   std::vector<double*> S, Y;
   std::vector<double> rho, alpha;
-  double f = 0, fold = 0, gnorm = 0, gnorm0 = 0, gd = 0, stp = 0, smax = 0, sy = 0, yy = 0;
+  double f = 0, fold = 0, gnorm = 0, gnorm0 = 0, gd = 0, stp = 0;
   bool steepest = false;
   int lsstat = 0, nfev = 0;
   MoreThuente ls;
   int N() const { return fe->N; }
+  Metric metric() const {  // the optimizer's diagonal metric (identity unless opts.precond)
+    Metric m;
+    if (opts.precond) { m.occ = fe->dx; m.npd = fe->desc.n_pd; }
+    return m;
+  }
 };
 
 }  // namespace a2c
@@ -296,10 +288,15 @@ int seg_launch_prep(a2c_batch* b, const std::vector<int>& ids, Fill fill, Op** d
 template <class Fill>
 int seg_axpby(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   if (ids.empty()) return A2C_OK;
+  auto fill0 = [&](SegAx& o, Slot& s) {
+    o.met = Metric{};
+    o.scaled = 0;
+    fill(o, s);
+  };
   SegAx* ops;
   int32_t* map;
   int nblk;
-  if (int rc = seg_launch_prep<SegAx>(b, ids, fill, &ops, &map, &nblk)) return rc;
+  if (int rc = seg_launch_prep<SegAx>(b, ids, fill0, &ops, &map, &nblk)) return rc;
   if (nblk) hipLaunchKernelGGL(k_seg_axpby, dim3(nblk), dim3(TB), 0, b->stream, ops, map);
   B_HIP(b, hipGetLastError());
   return A2C_OK;
@@ -312,6 +309,8 @@ int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<do
   if (ids.empty()) return A2C_OK;
   auto fill2 = [&](SegDot& o, Slot& s) {
     o.a2 = o.b2 = nullptr;
+    o.met = Metric{};
+    o.k1 = o.k2 = 0;
     fill(o, s);
     o.out = b->dres + 2 * (&s - b->slots.data());
   };
@@ -333,27 +332,14 @@ int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<do
 }
 
 template <int MODE, class Fill>
-int seg_bound(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<double>* out = nullptr) {
+int seg_bound(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   if (ids.empty()) return A2C_OK;
-  auto fill2 = [&](SegBound& o, Slot& s) {
-    fill(o, s);
-    o.out = b->dres + 2 * (&s - b->slots.data());
-  };
   SegBound* ops;
   int32_t* map;
   int nblk;
-  if (int rc = seg_launch_prep<SegBound>(b, ids, fill2, &ops, &map, &nblk)) return rc;
-  if (nblk) hipLaunchKernelGGL(k_seg_bound<MODE>, dim3(nblk), dim3(TB), 0, b->stream, ops, map, b->dpart);
+  if (int rc = seg_launch_prep<SegBound>(b, ids, fill, &ops, &map, &nblk)) return rc;
+  if (nblk) hipLaunchKernelGGL(k_seg_bound<MODE>, dim3(nblk), dim3(TB), 0, b->stream, ops, map);
   B_HIP(b, hipGetLastError());
-  if (MODE == 3) {
-    hipLaunchKernelGGL(k_seg_min_final, dim3(ids.size()), dim3(1024), 0, b->stream, ops, b->dpart);
-    B_HIP(b, hipGetLastError());
-    B_HIP(b, hipMemcpyAsync(b->hres, b->dres, sizeof(double) * 2 * b->slots.size(), hipMemcpyDeviceToHost,
-                            b->stream));
-    if (int rc = bsync(b)) return rc;
-    out->resize(ids.size());
-    for (size_t k = 0; k < ids.size(); ++k) (*out)[k] = b->hres[2 * ids[k]];
-  }
   return A2C_OK;
 }
 
@@ -544,13 +530,17 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
           }))
         return rc;
     }
+    // H0 = gamma D with gamma = s.y / y.D y of the newest pair (D: the metric)
     if (int rc = seg_dot(b, tl, [](SegDot& o, Slot& s) {
           const int nw = (s.head - 1 + s.m) % s.m;
           o.a1 = s.S[nw]; o.b1 = s.Y[nw]; o.a2 = s.Y[nw]; o.b2 = s.Y[nw];
+          o.met = s.metric(); o.k2 = 1;
         }, v))
       return rc;
-    for (size_t k = 0; k < tl.size(); ++k) b->slots[tl[k]].gd = v[2 * k] / v[2 * k + 1];  // H0 scale (temporary)
-    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.q; o.y = s.q; o.a = s.gd; o.b = 0.0; }))
+    for (size_t k = 0; k < tl.size(); ++k) b->slots[tl[k]].gd = v[2 * k] / v[2 * k + 1];  // gamma (temporary)
+    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) {
+          o.z = s.q; o.x = s.q; o.y = s.q; o.a = s.gd; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+        }))
       return rc;
     for (int t = maxp - 1; t >= 0; --t) {  // oldest to newest
       std::vector<int> sub;
@@ -573,18 +563,24 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
       return rc;
   }
   if (!sd.empty()) {
-    // scaled steepest descent: TAO's initial scale 2|f|/||g||^2, capped so
-    // that the first trial moves no coordinate by more than 0.05 A
-    if (int rc = seg_dot(b, sd, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.g; }, v)) return rc;
+    // scaled steepest descent d = -delta D g: TAO's initial scale
+    // 2|f| / g.D g, capped so that the first trial moves no coordinate
+    // group by more than 0.05 A in the 2-norm
+    if (int rc = seg_dot(b, sd, [](SegDot& o, Slot& s) {
+          o.a1 = s.g; o.b1 = s.g; o.a2 = s.g; o.b2 = s.g; o.met = s.metric(); o.k1 = 1; o.k2 = 2;
+        }, v))
+      return rc;
     std::vector<double> del(sd.size());
     for (size_t k = 0; k < sd.size(); ++k) {
       Slot& s = b->slots[sd[k]];
-      const double gg = std::max(v[2 * k], 1e-300);
-      double delta = (s.f != 0.0 ? 2.0 * std::fabs(s.f) : 2.0) / gg;
-      del[k] = std::min(delta, 0.05 / std::sqrt(gg));
+      const double gdg = std::max(v[2 * k], 1e-300), dgdg = std::max(v[2 * k + 1], 1e-300);
+      double delta = (s.f != 0.0 ? 2.0 * std::fabs(s.f) : 2.0) / gdg;
+      del[k] = std::min(delta, 0.05 / std::sqrt(dgdg));
     }
     size_t kk = 0;
-    if (int rc = seg_axpby(b, sd, [&](SegAx& o, Slot& s) { o.z = s.d; o.x = s.g; o.y = s.g; o.a = -del[kk++]; o.b = 0.0; }))
+    if (int rc = seg_axpby(b, sd, [&](SegAx& o, Slot& s) {
+          o.z = s.d; o.x = s.g; o.y = s.g; o.a = -del[kk++]; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+        }))
       return rc;
   }
   if (int rc = seg_bound<2>(b, ids, [](SegBound& o, Slot& s) {
@@ -614,21 +610,17 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
   }
   std::sort(go.begin(), go.end());
   if (go.empty()) return A2C_OK;
-  // line search setup
-  std::vector<double> smax;
-  if (int rc = seg_bound<3>(b, go, [](SegBound& o, Slot& s) {
-        o.x = s.x; o.y = s.d; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
-      }, &smax))
-    return rc;
+  // line search setup (projected search: trial points are clipped onto the
+  // sigma bounds, and the slope is taken along the projected step)
   if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.xo; o.x = s.x; o.y = s.x; o.a = 1.0; o.b = 0.0; })) return rc;
   if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.go; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; })) return rc;
   for (size_t k = 0; k < go.size(); ++k) {
     Slot& s = b->slots[go[k]];
     s.fold = s.f;
     s.ls = MoreThuente();
-    s.ls.stpmax = std::min(1e20, smax[k]);
+    s.ls.stpmax = 1e20;
     s.ls.epsf = s.mixed ? 0.0 : 1e-11 * std::fabs(s.f);
-    s.stp = std::min(1.0, s.ls.stpmax);
+    s.stp = 1.0;
     s.ls.start(s.f, s.gd, s.stp);
     s.nfev = 0;
     s.lsstat = 0;
@@ -642,14 +634,20 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
     if (run.empty()) break;
     if (int rc = seg_axpby(b, run, [](SegAx& o, Slot& s) { o.z = s.x; o.x = s.xo; o.y = s.d; o.a = 1.0; o.b = s.stp; }))
       return rc;
+    if (int rc = seg_bound<0>(b, run, [](SegBound& o, Slot& s) {
+          o.z = s.x; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+        }))
+      return rc;
+    if (int rc = seg_axpby(b, run, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.x; o.y = s.xo; o.a = 1.0; o.b = -1.0; }))
+      return rc;
     if (int rc = eval_slots(b, run)) return rc;
-    if (int rc = seg_dot(b, run, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
+    if (int rc = seg_dot(b, run, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.q; }, v)) return rc;
     std::vector<int> next;
     for (size_t k = 0; k < run.size(); ++k) {
       Slot& s = b->slots[run[k]];
       s.nfev++;
       if (!std::isfinite(s.f)) { s.lsstat = -2; continue; }
-      s.lsstat = s.ls.iterate(s.f, v[2 * k], s.stp);
+      s.lsstat = s.ls.iterate(s.f, v[2 * k] / s.stp, s.stp);  // slope along the projected step
       if (s.lsstat == 0 && s.nfev >= 30) s.lsstat = 2;
       if (s.lsstat == 0) next.push_back(run[k]);
     }
@@ -910,7 +908,7 @@ int a2c_batch_lmvm_start(a2c_batch* b, int32_t slot, const a2c_lmvm_opts* o) {
   b->err.clear();
   if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return berr(b, A2C_EINVAL, "bad slot");
   if (o->m < 1 || o->m > 64 || o->max_it < 0 || o->max_funcs < 1 || !(o->sigma_lb >= 0) ||
-      !(o->sigma_ub > o->sigma_lb))
+      !(o->sigma_ub > o->sigma_lb) || (o->precond != 0 && o->precond != 1))
     return berr(b, A2C_EINVAL, "bad options");
   Slot& s = b->slots[slot];
   s.opts = *o;
@@ -986,6 +984,7 @@ extern "C" void a2c_lmvm_opts_default(a2c_lmvm_opts* o) {
   o->mixed_switch = 1;
   o->mixed_gtol = 1e-5;
   o->sigma_ub = HUGE_VAL;  // formVariableBounds leaves sigma unbounded above
+  o->precond = 1;
 }
 
 extern "C" int a2c_fe_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* opts, double* X_out,
@@ -995,7 +994,7 @@ extern "C" int a2c_fe_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_o
   if (!X0 || !X_out || !opts || !report) return a2c::set_err(c, A2C_EINVAL, "null argument");
   if (!c->have_nbr || !c->have_x) return a2c::set_err(c, A2C_ESTATE, "neighbors/occupancy not set");
   if (opts->m < 1 || opts->max_it < 0 || opts->max_funcs < 1 || !(opts->sigma_lb >= 0) ||
-      !(opts->sigma_ub > opts->sigma_lb))
+      !(opts->sigma_ub > opts->sigma_lb) || (opts->precond != 0 && opts->precond != 1))
     return a2c::set_err(c, A2C_EINVAL, "bad options");
   return a2c::lmvm_minimize(c, X0, opts, X_out, report);
 }

@@ -140,6 +140,10 @@ typedef struct a2c_lmvm_opts {
                            -3 kB x ln sigma outgrows the vanishing pair terms
                            beyond a few A), so long relaxations of large
                            lattices may set a finite cap. */
+  int32_t precond;      /* 1: initial inverse Hessian gamma * diag(1/w), w = 1
+                           for Pd sites and the occupancy x for H sites (every
+                           term of an H packet carries x, so its curvature
+                           does); 0: gamma * I as TAO's BLMVM (1) */
 } a2c_lmvm_opts;
 
 typedef struct a2c_lmvm_report {

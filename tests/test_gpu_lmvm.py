@@ -114,3 +114,22 @@ def test_sigma_upper_bound_caps_and_is_kkt():
     assert np.all(G[3::4][at_cap] < 0.0)
     with pytest.raises(A.A2CError):
         fe.minimizeFreeEntropy(X, sigma_ub=1e-6)  # ub must exceed lb
+
+
+def test_preconditioned_and_plain_metrics_reach_the_same_state():
+    """precond=1 (diag(1/x) initial inverse Hessian) and precond=0 (TAO's
+    scalar form) converge to the same relaxed state on a lattice whose H
+    occupancies span three decades."""
+    X, x, box, per, npd, nh = specimen(3, seed=9, sigma=(0.08, 0.15), x=(1e-3, 0.6))[:6]
+    fe = A.FreeEntropy(npd, nh, box, per, T=400.0, precision="fp64")
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    X1, r1 = fe.minimizeFreeEntropy(X, gatol=1e-10, grtol=0.0, precond=1, max_it=20000, max_funcs=40000)
+    X0, r0 = fe.minimizeFreeEntropy(X, gatol=1e-10, grtol=0.0, precond=0, max_it=20000, max_funcs=40000)
+    assert r1["reason"] > 0 and r0["reason"] > 0, (r1, r0)
+    d = (X1 - X0).reshape(-1, 4)
+    d[:, :3] -= d[:, :3].mean(axis=0)  # uniform translation is an exact zero mode
+    # the packets of nearly empty sites are determined only to ~gatol / (x * curvature)
+    w = np.concatenate([np.ones(npd), x])
+    assert np.max(np.abs(d) * w[:, None]) <= 1e-6, np.max(np.abs(d) * w[:, None])
+    print("iterations: precond %d, plain %d" % (r1["iterations"], r0["iterations"]))
