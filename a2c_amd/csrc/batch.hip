@@ -67,11 +67,15 @@ struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null
   double* out;
   int32_t nsite, blk0;
 };
-struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero the active part of d
-  double* z;        // x (0), gp (1), d (2)
-  const double* x;  // x (1, 2)
-  const double* y;  // g (1)
-  double lb, ub;
+constexpr double SIGMA_STEP = 4.0;  // largest factor by which one line-search trial may change a width
+
+struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero the active part of d,
+                   // 3 project a trial point and form the step direction it realises
+  double* z;        // x (0), gp (1), d (2), trial x (3)
+  const double* x;  // x (1, 2), x_old (3)
+  const double* y;  // g (1), d (3)
+  double* w;        // realised direction (3)
+  double lb, ub, stp;
   int32_t nsite, blk0;
 };
 
@@ -160,9 +164,24 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
     o.z[i + 2] = o.y[i + 2];
     const double gs = o.y[i + 3], xs = o.x[i + 3];
     o.z[i + 3] = ((xs <= o.lb && gs > 0.0) || (xs >= o.ub && gs < 0.0)) ? 0.0 : gs;
-  } else {
+  } else if (MODE == 2) {
     const double xs = o.x[i + 3], ds = o.z[i + 3];
     if ((xs <= o.lb && ds < 0.0) || (xs >= o.ub && ds > 0.0)) o.z[i + 3] = 0.0;
+  } else {
+    // Position components are never clipped: their direction is d itself (a
+    // difference x - x_old would lose digits to the ~100 A coordinates once
+    // steps are small).  A width is clipped onto the bounds and onto a factor
+    // SIGMA_STEP of its previous value -- a width is a scale, and one step
+    // must not carry it into the -3 kB x ln(sigma) wall near zero -- and then
+    // moves (clipped - x_old) / stp.
+    o.w[i] = o.y[i];
+    o.w[i + 1] = o.y[i + 1];
+    o.w[i + 2] = o.y[i + 2];
+    const double so = o.x[i + 3];
+    const double lo = fmax(o.lb, so * (1.0 / SIGMA_STEP)), hi = fmin(o.ub, so * SIGMA_STEP);
+    const double xs = o.z[i + 3], cs = fmin(fmax(xs, lo), hi);
+    o.z[i + 3] = cs;
+    o.w[i + 3] = cs == xs ? o.y[i + 3] : (cs - so) / o.stp;
   }
 }
 
@@ -634,11 +653,10 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
     if (run.empty()) break;
     if (int rc = seg_axpby(b, run, [](SegAx& o, Slot& s) { o.z = s.x; o.x = s.xo; o.y = s.d; o.a = 1.0; o.b = s.stp; }))
       return rc;
-    if (int rc = seg_bound<0>(b, run, [](SegBound& o, Slot& s) {
-          o.z = s.x; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+    if (int rc = seg_bound<3>(b, run, [](SegBound& o, Slot& s) {
+          o.z = s.x; o.x = s.xo; o.y = s.d; o.w = s.q;
+          o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub; o.stp = s.stp;
         }))
-      return rc;
-    if (int rc = seg_axpby(b, run, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.x; o.y = s.xo; o.a = 1.0; o.b = -1.0; }))
       return rc;
     if (int rc = eval_slots(b, run)) return rc;
     if (int rc = seg_dot(b, run, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.q; }, v)) return rc;
@@ -647,7 +665,7 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
       Slot& s = b->slots[run[k]];
       s.nfev++;
       if (!std::isfinite(s.f)) { s.lsstat = -2; continue; }
-      s.lsstat = s.ls.iterate(s.f, v[2 * k] / s.stp, s.stp);  // slope along the projected step
+      s.lsstat = s.ls.iterate(s.f, v[2 * k], s.stp);  // slope along the realised (projected) direction
       if (s.lsstat == 0 && s.nfev >= 30) s.lsstat = 2;
       if (s.lsstat == 0) next.push_back(run[k]);
     }
