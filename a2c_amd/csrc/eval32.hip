@@ -85,6 +85,13 @@ __device__ __forceinline__ float tail(int w, float r) {
   return r >= R1 ? TAIL[w][0] * __expf(TAIL[w][1] * (r - TAIL[w][2])) : __builtin_nanf("");
 }
 
+// value of table w at r (tails beyond the table; NaN below it)
+__device__ __forceinline__ float tab_v(const float4* __restrict__ tab, int w, float r) {
+  const Pos p = tab_pos(r);
+  if (!p.far) return cubic(tab[w * NT + p.k], p.u);
+  return tail(w, r);
+}
+
 struct Site {
   double q[3];
   double s;
@@ -130,11 +137,7 @@ __device__ __forceinline__ float pair_density(const float4* tab, int w, const fl
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const float x = fmaxf(r2[q], 1e-12f);
-      const float r = x * rsq(x);
-      const Pos p = tab_pos(r);
-      float v = cubic(tab[w * NT + p.k], p.u);
-      if (p.far) v = tail(w, r);
-      acc += v;
+      acc += tab_v(tab, w, x * rsq(x));
     }
   }
   return acc;
@@ -336,15 +339,16 @@ __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, 
 // received slot; a gather kernel adds the received shares in FP64.  Each
 // buffer entry is written by exactly one lane, so results are deterministic.
 
-// 12-point sums of two density tables at the same points
-template <bool SAME>
+// 12-point sums of two density tables at the same points.  One branch per
+// lookup keeps the live range short (~63 VGPRs, 8 waves per SIMD), which
+// this latency-bound pass prefers over hoisting all 12 points' loads.
 __device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb, const float d[3], float ai,
                                               float aj, float& sa, float& sb) {
   const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   const float bi = D2 + ai * ai, bj = D2 + aj * aj;
   sa = 0.0f;
   sb = 0.0f;
-#pragma unroll 1  // rolled over the axes: ~66 VGPRs (7 waves per SIMD) instead of ~108 (4)
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     const float ti = 2.0f * ai * d[c], tj = 2.0f * aj * d[c];
     const float r2[4] = {bi + ti, bi - ti, bj - tj, bj + tj};
@@ -352,18 +356,11 @@ __device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb,
     for (int q = 0; q < 4; ++q) {
       const float x = fmaxf(r2[q], 1e-12f);
       const float r = x * rsq(x);
-      const Pos p = tab_pos(r);
-      float va = cubic(tab[wa * NT + p.k], p.u);
-      float vb = SAME ? va : cubic(tab[wb * NT + p.k], p.u);
-      if (p.far) {
-        va = tail(wa, r);
-        vb = SAME ? va : tail(wb, r);
-      }
-      sa += va;
-      if (!SAME) sb += vb;
+      sa += tab_v(tab, wa, r);
+      if (wb != wa) sb += tab_v(tab, wb, r);
     }
   }
-  if (SAME) sb = sa;
+  if (wb == wa) sb = sa;
 }
 
 // Partner list of each list: PdPd<->PdPd, PdH<->HPd, HH<->HH.
@@ -371,7 +368,9 @@ __device__ __forceinline__ const Ell& partner_of(const LatDev& S, int list) {
   return list == 0 ? S.pdpd : (list == 1 ? S.hpd : (list == 2 ? S.pdh : S.hh));
 }
 
-__global__ __launch_bounds__(TB) void k_dens_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+// 8 waves per SIMD: the lookups are latency-bound, 64 VGPRs hold this pass without spills
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_dens_own(
+    const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                  const float4* __restrict__ gtab) {
   __shared__ float4 tab[2 * NT];  // f_Pd, f_H
   stage_tables(tab, gtab, 2 * NT);
@@ -401,9 +400,7 @@ __global__ __launch_bounds__(TB) void k_dens_own(const LatDev* __restrict__ lats
       float d[3];
       pair_vec32(S, a, b, d);
       float mine, theirs;  // neighbor-type density (for me), my-type density (for the neighbor)
-      const int wa = side == 0 ? T_FPD : T_FH;
-      if (wa == mytab) pair_density2<true>(tab, wa, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
-      else pair_density2<false>(tab, wa, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
+      pair_density2(tab, side == 0 ? T_FPD : T_FH, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
       const int rv = L.orv[e];
       if (rv >= 0) reinterpret_cast<float*>(P.rbuf)[(size_t)rv * P.rows + jl] = theirs;
