@@ -370,8 +370,7 @@ __device__ __forceinline__ const Ell& partner_of(const LatDev& S, int list) {
 
 // 8 waves per SIMD: the lookups are latency-bound, 64 VGPRs hold this pass without spills
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_dens_own(
-    const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
-                                                 const float4* __restrict__ gtab) {
+    const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l, const float4* __restrict__ gtab) {
   __shared__ float4 tab[2 * NT];  // f_Pd, f_H
   stage_tables(tab, gtab, 2 * NT);
   const LatDev& S = lats[b2l[blockIdx.x]];
