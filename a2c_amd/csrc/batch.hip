@@ -532,7 +532,9 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
   }
   std::vector<double> v;
   if (!tl.empty()) {
-    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; }))
+    // the recursion acts on the projected gradient (bounded limited-memory
+    // variable metric: components held at a bound do not steer the others)
+    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.gp; o.y = s.gp; o.a = 1.0; o.b = 0.0; }))
       return rc;
     for (int t = 0; t < maxp; ++t) {  // newest to oldest
       std::vector<int> sub;
@@ -586,7 +588,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
     // 2|f| / g.D g, capped so that the first trial moves no coordinate
     // group by more than 0.05 A in the 2-norm
     if (int rc = seg_dot(b, sd, [](SegDot& o, Slot& s) {
-          o.a1 = s.g; o.b1 = s.g; o.a2 = s.g; o.b2 = s.g; o.met = s.metric(); o.k1 = 1; o.k2 = 2;
+          o.a1 = s.gp; o.b1 = s.gp; o.a2 = s.gp; o.b2 = s.gp; o.met = s.metric(); o.k1 = 1; o.k2 = 2;
         }, v))
       return rc;
     std::vector<double> del(sd.size());
@@ -598,7 +600,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
     }
     size_t kk = 0;
     if (int rc = seg_axpby(b, sd, [&](SegAx& o, Slot& s) {
-          o.z = s.d; o.x = s.g; o.y = s.g; o.a = -del[kk++]; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+          o.z = s.d; o.x = s.gp; o.y = s.gp; o.a = -del[kk++]; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
         }))
       return rc;
   }
