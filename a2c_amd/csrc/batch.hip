@@ -68,6 +68,7 @@ struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null
   int32_t nsite, blk0;
 };
 constexpr double SIGMA_STEP = 4.0;  // largest factor by which one line-search trial may change a width
+constexpr double Q_STEP = 0.25;     // largest displacement of a site in one line-search trial (A)
 
 struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero the active part of d,
                    // 3 project a trial point and form the step direction it realises
@@ -168,15 +169,25 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
     const double xs = o.x[i + 3], ds = o.z[i + 3];
     if ((xs <= o.lb && ds < 0.0) || (xs >= o.ub && ds > 0.0)) o.z[i + 3] = 0.0;
   } else {
-    // Position components are never clipped: their direction is d itself (a
-    // difference x - x_old would lose digits to the ~100 A coordinates once
-    // steps are small).  A width is clipped onto the bounds and onto a factor
-    // SIGMA_STEP of its previous value -- a width is a scale, and one step
-    // must not carry it into the -3 kB x ln(sigma) wall near zero -- and then
-    // moves (clipped - x_old) / stp.
-    o.w[i] = o.y[i];
-    o.w[i + 1] = o.y[i + 1];
-    o.w[i + 2] = o.y[i + 2];
+    // Trial safeguards (they bend the search path, not the fixed point): a
+    // site's position moves at most Q_STEP, so that a packet whose metric
+    // weight is large (nearly empty H) cannot be thrown out of its cage; a
+    // width is clipped onto the bounds and onto a factor SIGMA_STEP of its
+    // previous value (a width is a scale, and one step must not carry it into
+    // the -3 kB x ln(sigma) wall near zero).  The realised direction is built
+    // from d and the scale factors, never as x - x_old, which would lose
+    // digits to the ~100 A coordinates once steps are small.
+    const double d0 = o.y[i], d1 = o.y[i + 1], d2 = o.y[i + 2];
+    const double len = o.stp * sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double sc = len > Q_STEP ? Q_STEP / len : 1.0;  // a site moves at most Q_STEP per trial
+    o.w[i] = sc * d0;
+    o.w[i + 1] = sc * d1;
+    o.w[i + 2] = sc * d2;
+    if (sc < 1.0) {
+      o.z[i] = o.x[i] + o.stp * o.w[i];
+      o.z[i + 1] = o.x[i + 1] + o.stp * o.w[i + 1];
+      o.z[i + 2] = o.x[i + 2] + o.stp * o.w[i + 2];
+    }
     const double so = o.x[i + 3];
     const double lo = fmax(o.lb, so * (1.0 / SIGMA_STEP)), hi = fmin(o.ub, so * SIGMA_STEP);
     const double xs = o.z[i + 3], cs = fmin(fmax(xs, lo), hi);
@@ -532,9 +543,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
   }
   std::vector<double> v;
   if (!tl.empty()) {
-    // the recursion acts on the projected gradient (bounded limited-memory
-    // variable metric: components held at a bound do not steer the others)
-    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.gp; o.y = s.gp; o.a = 1.0; o.b = 0.0; }))
+    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; }))
       return rc;
     for (int t = 0; t < maxp; ++t) {  // newest to oldest
       std::vector<int> sub;
@@ -588,7 +597,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
     // 2|f| / g.D g, capped so that the first trial moves no coordinate
     // group by more than 0.05 A in the 2-norm
     if (int rc = seg_dot(b, sd, [](SegDot& o, Slot& s) {
-          o.a1 = s.gp; o.b1 = s.gp; o.a2 = s.gp; o.b2 = s.gp; o.met = s.metric(); o.k1 = 1; o.k2 = 2;
+          o.a1 = s.g; o.b1 = s.g; o.a2 = s.g; o.b2 = s.g; o.met = s.metric(); o.k1 = 1; o.k2 = 2;
         }, v))
       return rc;
     std::vector<double> del(sd.size());
@@ -600,7 +609,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
     }
     size_t kk = 0;
     if (int rc = seg_axpby(b, sd, [&](SegAx& o, Slot& s) {
-          o.z = s.d; o.x = s.gp; o.y = s.gp; o.a = -del[kk++]; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+          o.z = s.d; o.x = s.g; o.y = s.g; o.a = -del[kk++]; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
         }))
       return rc;
   }

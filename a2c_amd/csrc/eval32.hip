@@ -480,17 +480,12 @@ __device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d
   return p;
 }
 
-// BS threads per workgroup = BS/256 site blocks sharing one staged copy of
-// the tables (LDS per wave halves at BS = 512).
-template <int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 512 ? 5 : 1))) void k_grad_own(
-    const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l, const float4* __restrict__ gtab, int nblk) {
+__global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+                                                 const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
   stage_grad_tables(tab, gtab);
-  const int sb = blockIdx.x * (BS / TB) + (int)threadIdx.x / TB;
-  if (sb >= nblk) return;
-  const LatDev& S = lats[b2l[sb]];
-  const int s = (sb - S.blk0) * TB + (int)threadIdx.x % TB;
+  const LatDev& S = lats[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ X = S.X;
@@ -625,15 +620,6 @@ int build_tables(a2c_fe_ctx* c) {
   return A2C_OK;
 }
 
-// A2C_GRAD_BLOCK=512 selects the 512-thread owner kernel (A/B of occupancy)
-static bool grad_block_512() {
-  static const bool on = [] {
-    const char* v = getenv("A2C_GRAD_BLOCK");
-    return v && std::string(v) == "512";
-  }();
-  return on;
-}
-
 int run_eval32(const Launch& L, Timing* tm) {
   if (L.mixed_full) {  // every pair from both ends (no reverse buffers)
     if (tm) tm->mark(L.stream, "density32");
@@ -654,11 +640,7 @@ int run_eval32(const Launch& L, Timing* tm) {
     hipLaunchKernelGGL(k_dens_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "grad_own");
-    if (grad_block_512())
-      hipLaunchKernelGGL(k_grad_own<512>, dim3((L.nblk + 1) / 2), dim3(512), 0, L.stream, L.lats, L.b2l, L.tab,
-                         L.nblk);
-    else
-      hipLaunchKernelGGL(k_grad_own<TB>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab, L.nblk);
+    hipLaunchKernelGGL(k_grad_own, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "grad_recv");
     hipLaunchKernelGGL(k_grad_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
