@@ -149,8 +149,9 @@ __global__ __launch_bounds__(TB) void k_density32(const LatDev* __restrict__ lat
                                                   const float4* __restrict__ gtab) {
   __shared__ float4 tab[2 * NT];  // f_Pd, f_H
   stage_tables(tab, gtab, 2 * NT);
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ X = S.X;
@@ -267,8 +268,9 @@ __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, 
                                                const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
   stage_grad_tables(tab, gtab);
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int lb = blockIdx.x - S.blk0;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int lb = blk - S.blk0;
   const int s = lb * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   const double* __restrict__ X = S.X;
@@ -375,8 +377,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l, const float4* __restrict__ gtab) {
   __shared__ float4 tab[2 * NT];  // f_Pd, f_H
   stage_tables(tab, gtab, 2 * NT);
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ X = S.X;
@@ -411,8 +414,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_
 }
 
 __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ xo = S.x;
@@ -484,8 +488,9 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
                                                  const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
   stage_grad_tables(tab, gtab);
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ X = S.X;
@@ -536,8 +541,9 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
 
 __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
   __shared__ double red[TB];
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int lb = blockIdx.x - S.blk0;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int lb = blk - S.blk0;
   const int s = lb * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   double fs = 0.0;

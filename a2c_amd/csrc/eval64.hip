@@ -80,8 +80,9 @@ __device__ __forceinline__ void point_r(int p, const double d[3], double ai, dou
 template <int DM>
 __global__ __launch_bounds__(TB) void k_density64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                   double eps) {
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ X = S.X;
@@ -205,8 +206,9 @@ template <int DM>
 __global__ __launch_bounds__(TB) void k_grad64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                double eps) {
   __shared__ double red[TB];
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int lb = blockIdx.x - S.blk0;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int lb = blk - S.blk0;
   const int s = lb * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   double fs = 0.0;
@@ -277,8 +279,9 @@ __device__ __forceinline__ const Ell& partner_list(const LatDev& S, int list) {
 }
 
 __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ X = S.X;
@@ -322,8 +325,9 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
 }
 
 __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ xo = S.x;
@@ -377,8 +381,9 @@ __device__ __forceinline__ void pair_both(const double d[3], double ai, double a
 }
 
 __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int s = (blockIdx.x - S.blk0) * TB + threadIdx.x;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
   const double* __restrict__ X = S.X;
@@ -428,8 +433,9 @@ __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ la
 
 __global__ __launch_bounds__(TB) void k_grad_recv64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
   __shared__ double red[TB];
-  const LatDev& S = lats[b2l[blockIdx.x]];
-  const int lb = blockIdx.x - S.blk0;
+  const int blk = xcd_block();
+  const LatDev& S = lats[b2l[blk]];
+  const int lb = blk - S.blk0;
   const int s = lb * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   double fs = 0.0;

@@ -71,6 +71,17 @@ using DevSys = LatDev;
 constexpr int SITE_BLOCK = 256;  // sites per block of every site-parallel kernel
 __host__ __device__ inline int site_blocks(int n) { return (n + SITE_BLOCK - 1) / SITE_BLOCK; }
 
+// Logical site block of a workgroup.  Workgroups are dealt to the 8 XCDs
+// round-robin by index; this bijection gives XCD k one contiguous range of
+// site blocks, so the neighbor data its waves gather stays within its own L2
+// instead of every XCD touching the whole lattice.
+constexpr int NUM_XCD = 8;
+__device__ inline int xcd_block() {
+  const int b = blockIdx.x, n = gridDim.x;
+  const int x = b % NUM_XCD, q = n / NUM_XCD, r = n % NUM_XCD;
+  return x * q + (x < r ? x : r) + b / NUM_XCD;
+}
+
 struct Timing;
 
 }  // namespace a2c
