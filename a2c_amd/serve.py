@@ -17,11 +17,18 @@ id.  The comparator relaxes the same requests one at a time in FP64.
 import argparse
 import json
 import math
+import sys
 import time
 
 import numpy as np
 
-from . import Batch, FreeEntropy, lattice
+if __package__:
+    from . import Batch, FreeEntropy, lattice
+else:  # run as a script: python a2c_amd/serve.py
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from a2c_amd import Batch, FreeEntropy, lattice
 
 
 def make_requests(n, rate, seed=4, nmin=6, nmax=48):
@@ -124,7 +131,10 @@ def main():
     out = []
     for rate in [float(r) for r in a.rates.split(",")]:
         reqs = make_requests(a.requests, rate, nmax=a.nmax)
-        done, wall = serve([dict(r) for r in reqs])
+        note = lambda r: print("rate %g: request %d (n=%d) done, latency %.3f s, %d iterations"
+                               % (rate, r["id"], r["n"], r["latency"], r["report"]["iterations"]),
+                               file=sys.stderr, flush=True)
+        done, wall = serve([dict(r) for r in reqs], log=note)
         row = {"rate": rate, "batched_mixed": summarize(done, wall)}
         if not a.no_baseline:
             d1, w1 = serve_one_at_a_time([dict(r) for r in reqs])
