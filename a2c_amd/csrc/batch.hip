@@ -48,11 +48,19 @@ struct Metric {
 __device__ __forceinline__ double metric_inv(const Metric& m, int s) {
   return (m.occ && s >= m.npd) ? 1.0 / fmax(m.occ[s - m.npd], 1e-6) : 1.0;
 }
-struct SegAx {  // z = a D^k x + b y on 4N doubles (k = 0, or 1 when scaled)
+// z = A D^k x + B y on 4N doubles (k = 0, or 1 when scaled).  The
+// coefficients may come from device scalars, so optimizer steps that depend
+// on dot products need no host round trip:
+//   amode 0: A = a;  1: A = a * p1;  2: A = a * p1 / p2;
+//         3: A = -min(a / p1, 0.05 / sqrt(p2))   (scaled steepest descent)
+//   bmode 0: B = b;  1: B = b * q1;  2: B = b * (q1 - q2)
+struct SegAx {
   double* z;
   const double* x;
   const double* y;
   double a, b;
+  const double *p1, *p2, *q1, *q2;
+  int32_t amode, bmode;
   Metric met;
   int32_t scaled;
   int32_t nsite, blk0;
@@ -67,6 +75,10 @@ struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null
   double* out;
   int32_t nsite, blk0;
 };
+// Device scalars per slot: [0, m) the first-loop dots s_i.q of the two-loop
+// recursion, then s.y and y.D y of the newest pair, the second-loop dot, and
+// g.D g, g.D^2 g of scaled steepest descent.
+constexpr int SCAL_PER_SLOT = 72;  // >= m + 5 for m <= 64
 constexpr double SIGMA_STEP = 4.0;  // largest factor by which one line-search trial may change a width
 constexpr double Q_STEP = 0.25;     // largest displacement of a site in one line-search trial (A)
 
@@ -85,9 +97,15 @@ __global__ __launch_bounds__(TB) void k_seg_axpby(const SegAx* __restrict__ ops,
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   if (s >= o.nsite) return;
   const size_t i = 4 * (size_t)s;
-  const double a = o.scaled ? o.a * metric_inv(o.met, s) : o.a;
+  double a = o.a, bb = o.b;
+  if (o.amode == 1) a *= *o.p1;
+  else if (o.amode == 2) a *= *o.p1 / *o.p2;
+  else if (o.amode == 3) a = -fmin(o.a / fmax(*o.p1, 1e-300), 0.05 / sqrt(fmax(*o.p2, 1e-300)));
+  if (o.bmode == 1) bb *= *o.q1;
+  else if (o.bmode == 2) bb *= *o.q1 - *o.q2;
+  if (o.scaled) a *= metric_inv(o.met, s);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) o.z[i + c] = a * o.x[i + c] + o.b * o.y[i + c];
+  for (int c = 0; c < 4; ++c) o.z[i + c] = a * o.x[i + c] + bb * o.y[i + c];
 }
 
 __global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, const int32_t* __restrict__ b2l,
@@ -147,7 +165,7 @@ __global__ __launch_bounds__(1024) void k_seg_dot_final(const SegDot* __restrict
   }
   if (threadIdx.x == 0) {
     o.out[0] = r1[0];
-    o.out[1] = r2[0];
+    if (o.a2) o.out[1] = r2[0];  // single dots write one scalar (device scalars are packed)
   }
 }
 
@@ -251,8 +269,9 @@ struct a2c_batch {
   a2c::Arena arena;
   double* dF = nullptr;    // objective per slot (device)
   double* hF = nullptr;    // pinned mirror
-  double* dres = nullptr;  // 2 doubles per slot (device dot/min results)
+  double* dres = nullptr;  // 2 doubles per slot (device dot results read by the host)
   double* hres = nullptr;  // pinned mirror
+  double* dscal = nullptr;  // SCAL_PER_SLOT doubles per slot: optimizer scalars that stay on the device
   double* dpart = nullptr;  // 2 * max blocks
   size_t part_cap = 0;      // blocks
   float4* dtab = nullptr;
@@ -321,6 +340,8 @@ int seg_axpby(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   auto fill0 = [&](SegAx& o, Slot& s) {
     o.met = Metric{};
     o.scaled = 0;
+    o.amode = o.bmode = 0;
+    o.p1 = o.p2 = o.q1 = o.q2 = nullptr;
     fill(o, s);
   };
   SegAx* ops;
@@ -356,8 +377,29 @@ int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<do
   if (int rc = bsync(b)) return rc;
   for (size_t k = 0; k < ids.size(); ++k) {
     out[2 * k] = b->hres[2 * ids[k]];
-    out[2 * k + 1] = b->hres[2 * ids[k] + 1];
+    out[2 * k + 1] = b->hres[2 * ids[k] + 1];  // meaningful only for paired dots (a2 set)
   }
+  return A2C_OK;
+}
+
+// dot products per slot left on the device (fill sets o.out); no host sync
+template <class Fill>
+int seg_dot_dev(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
+  if (ids.empty()) return A2C_OK;
+  auto fill2 = [&](SegDot& o, Slot& s) {
+    o.a2 = o.b2 = nullptr;
+    o.met = Metric{};
+    o.k1 = o.k2 = 0;
+    fill(o, s);
+  };
+  SegDot* ops;
+  int32_t* map;
+  int nblk;
+  if (int rc = seg_launch_prep<SegDot>(b, ids, fill2, &ops, &map, &nblk)) return rc;
+  if (nblk) hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, b->stream, ops, map, b->dpart, nblk);
+  B_HIP(b, hipGetLastError());
+  hipLaunchKernelGGL(k_seg_dot_final, dim3(ids.size()), dim3(1024), 0, b->stream, ops, b->dpart, nblk);
+  B_HIP(b, hipGetLastError());
   return A2C_OK;
 }
 
@@ -459,8 +501,10 @@ int converged_reason(const Slot& s) {
   return 0;
 }
 
-// evaluate f, g at x for the listed slots, each in its own precision
-int eval_slots(a2c_batch* b, const std::vector<int>& ids) {
+// evaluate f, g at x for the listed slots, each in its own precision; with
+// sync = false the objective copy is only enqueued (the caller's next
+// synchronising step completes it, then take_f reads it)
+int eval_slots(a2c_batch* b, const std::vector<int>& ids, bool sync = true) {
   std::vector<int> m32, m64;
   for (int i : ids) {
     Slot& s = b->slots[i];
@@ -471,9 +515,13 @@ int eval_slots(a2c_batch* b, const std::vector<int>& ids) {
   if (int rc = fused_eval(b, m32, false, lmvm_state, false)) return rc;
   if (int rc = fused_eval(b, m64, true, lmvm_state, false)) return rc;
   B_HIP(b, hipMemcpyAsync(b->hF, b->dF, sizeof(double) * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
+  if (!sync) return A2C_OK;
   if (int rc = bsync(b)) return rc;
   for (int i : ids) b->slots[i].f = b->hF[i];
   return A2C_OK;
+}
+void take_f(a2c_batch* b, const std::vector<int>& ids) {
+  for (int i : ids) b->slots[i].f = b->hF[i];
 }
 
 int proj_gnorm(a2c_batch* b, const std::vector<int>& ids) {
@@ -542,50 +590,51 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
     }
   }
   std::vector<double> v;
+  auto scal = [b](const Slot& s) { return b->dscal + SCAL_PER_SLOT * (&s - b->slots.data()); };
   if (!tl.empty()) {
+    // Two-loop recursion with every dot product and coefficient on the
+    // device (no host round trip inside the recursion).
     if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; }))
       return rc;
-    for (int t = 0; t < maxp; ++t) {  // newest to oldest
+    for (int t = 0; t < maxp; ++t) {  // newest to oldest: A_i = s_i.q, q -= rho_i A_i y_i
       std::vector<int> sub;
       for (int i : tl)
         if (t < b->slots[i].npairs) sub.push_back(i);
       auto pidx = [t](const Slot& s) { return (s.head - 1 - t + s.m) % s.m; };
-      if (int rc = seg_dot(b, sub, [&](SegDot& o, Slot& s) { o.a1 = s.S[pidx(s)]; o.b1 = s.q; }, v)) return rc;
-      for (size_t k = 0; k < sub.size(); ++k) {
-        Slot& s = b->slots[sub[k]];
-        s.alpha[pidx(s)] = s.rho[pidx(s)] * v[2 * k];
-      }
+      if (int rc = seg_dot_dev(b, sub, [&](SegDot& o, Slot& s) {
+            o.a1 = s.S[pidx(s)]; o.b1 = s.q; o.out = scal(s) + pidx(s);
+          }))
+        return rc;
       if (int rc = seg_axpby(b, sub, [&](SegAx& o, Slot& s) {
-            o.z = s.q; o.x = s.q; o.y = s.Y[pidx(s)]; o.a = 1.0; o.b = -s.alpha[pidx(s)];
+            o.z = s.q; o.x = s.q; o.y = s.Y[pidx(s)]; o.a = 1.0;
+            o.b = -s.rho[pidx(s)]; o.bmode = 1; o.q1 = scal(s) + pidx(s);
           }))
         return rc;
     }
-    // H0 = gamma D with gamma = s.y / y.D y of the newest pair (D: the metric)
-    if (int rc = seg_dot(b, tl, [](SegDot& o, Slot& s) {
+    // H0 = gamma D, gamma = s.y / y.D y of the newest pair
+    if (int rc = seg_dot_dev(b, tl, [&](SegDot& o, Slot& s) {
           const int nw = (s.head - 1 + s.m) % s.m;
           o.a1 = s.S[nw]; o.b1 = s.Y[nw]; o.a2 = s.Y[nw]; o.b2 = s.Y[nw];
-          o.met = s.metric(); o.k2 = 1;
-        }, v))
-      return rc;
-    for (size_t k = 0; k < tl.size(); ++k) b->slots[tl[k]].gd = v[2 * k] / v[2 * k + 1];  // gamma (temporary)
-    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) {
-          o.z = s.q; o.x = s.q; o.y = s.q; o.a = s.gd; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+          o.met = s.metric(); o.k2 = 1; o.out = scal(s) + s.m;
         }))
       return rc;
-    for (int t = maxp - 1; t >= 0; --t) {  // oldest to newest
+    if (int rc = seg_axpby(b, tl, [&](SegAx& o, Slot& s) {
+          o.z = s.q; o.x = s.q; o.y = s.q; o.a = 1.0; o.amode = 2; o.p1 = scal(s) + s.m; o.p2 = scal(s) + s.m + 1;
+          o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+        }))
+      return rc;
+    for (int t = maxp - 1; t >= 0; --t) {  // oldest to newest: q += rho_i (A_i - y_i.q) s_i
       std::vector<int> sub;
       for (int i : tl)
         if (t < b->slots[i].npairs) sub.push_back(i);
       auto pidx = [t](const Slot& s) { return (s.head - 1 - t + s.m) % s.m; };
-      if (int rc = seg_dot(b, sub, [&](SegDot& o, Slot& s) { o.a1 = s.Y[pidx(s)]; o.b1 = s.q; }, v)) return rc;
-      std::vector<double> coef(sub.size());
-      for (size_t k = 0; k < sub.size(); ++k) {
-        Slot& s = b->slots[sub[k]];
-        coef[k] = s.alpha[pidx(s)] - s.rho[pidx(s)] * v[2 * k];
-      }
-      size_t kk = 0;
+      if (int rc = seg_dot_dev(b, sub, [&](SegDot& o, Slot& s) {
+            o.a1 = s.Y[pidx(s)]; o.b1 = s.q; o.out = scal(s) + s.m + 2;
+          }))
+        return rc;
       if (int rc = seg_axpby(b, sub, [&](SegAx& o, Slot& s) {
-            o.z = s.q; o.x = s.q; o.y = s.S[pidx(s)]; o.a = 1.0; o.b = coef[kk++];
+            o.z = s.q; o.x = s.q; o.y = s.S[pidx(s)]; o.a = 1.0;
+            o.b = s.rho[pidx(s)]; o.bmode = 2; o.q1 = scal(s) + pidx(s); o.q2 = scal(s) + s.m + 2;
           }))
         return rc;
     }
@@ -595,21 +644,15 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
   if (!sd.empty()) {
     // scaled steepest descent d = -delta D g: TAO's initial scale
     // 2|f| / g.D g, capped so that the first trial moves no coordinate
-    // group by more than 0.05 A in the 2-norm
-    if (int rc = seg_dot(b, sd, [](SegDot& o, Slot& s) {
+    // group by more than 0.05 A in the 2-norm (both on the device)
+    if (int rc = seg_dot_dev(b, sd, [&](SegDot& o, Slot& s) {
           o.a1 = s.g; o.b1 = s.g; o.a2 = s.g; o.b2 = s.g; o.met = s.metric(); o.k1 = 1; o.k2 = 2;
-        }, v))
+          o.out = scal(s) + s.m + 3;
+        }))
       return rc;
-    std::vector<double> del(sd.size());
-    for (size_t k = 0; k < sd.size(); ++k) {
-      Slot& s = b->slots[sd[k]];
-      const double gdg = std::max(v[2 * k], 1e-300), dgdg = std::max(v[2 * k + 1], 1e-300);
-      double delta = (s.f != 0.0 ? 2.0 * std::fabs(s.f) : 2.0) / gdg;
-      del[k] = std::min(delta, 0.05 / std::sqrt(dgdg));
-    }
-    size_t kk = 0;
     if (int rc = seg_axpby(b, sd, [&](SegAx& o, Slot& s) {
-          o.z = s.d; o.x = s.g; o.y = s.g; o.a = -del[kk++]; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+          o.z = s.d; o.x = s.g; o.y = s.g; o.a = s.f != 0.0 ? 2.0 * std::fabs(s.f) : 2.0; o.amode = 3;
+          o.p1 = scal(s) + s.m + 3; o.p2 = scal(s) + s.m + 4; o.b = 0.0; o.met = s.metric(); o.scaled = 1;
         }))
       return rc;
   }
@@ -669,8 +712,9 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
           o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub; o.stp = s.stp;
         }))
       return rc;
-    if (int rc = eval_slots(b, run)) return rc;
+    if (int rc = eval_slots(b, run, false)) return rc;  // f and the slope come back in one round trip
     if (int rc = seg_dot(b, run, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.q; }, v)) return rc;
+    take_f(b, run);
     std::vector<int> next;
     for (size_t k = 0; k < run.size(); ++k) {
       Slot& s = b->slots[run[k]];
@@ -713,16 +757,24 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
       return rc;
     if (int rc = seg_axpby(b, ok, [](SegAx& o, Slot& s) { o.z = s.Y[s.head]; o.x = s.g; o.y = s.go; o.a = 1.0; o.b = -1.0; }))
       return rc;
-    if (int rc = seg_dot(b, ok, [](SegDot& o, Slot& s) { o.a1 = s.S[s.head]; o.b1 = s.Y[s.head]; }, v)) return rc;
+    // s.y of the new pair and the projected-gradient norm in one round trip
+    if (int rc = seg_bound<1>(b, ok, [](SegBound& o, Slot& s) {
+          o.z = s.gp; o.x = s.x; o.y = s.g; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+        }))
+      return rc;
+    if (int rc = seg_dot(b, ok, [](SegDot& o, Slot& s) {
+          o.a1 = s.S[s.head]; o.b1 = s.Y[s.head]; o.a2 = s.gp; o.b2 = s.gp;
+        }, v))
+      return rc;
     for (size_t k = 0; k < ok.size(); ++k) {
       Slot& s = b->slots[ok[k]];
+      s.gnorm = std::sqrt(v[2 * k + 1]);
       if (v[2 * k] > 1e-300) {  // keep only pairs with positive curvature
         s.rho[s.head] = 1.0 / v[2 * k];
         s.head = (s.head + 1) % s.m;
         s.npairs = std::min(s.npairs + 1, s.m);
       }
     }
-    if (int rc = proj_gnorm(b, ok)) return rc;
     std::vector<int> sw2;
     for (int i : ok) {
       Slot& s = b->slots[i];
@@ -813,7 +865,8 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
   b->arena.cap = (size_t)32 << 20;
   bool ok = hipHostMalloc(&b->arena.h, b->arena.cap) == hipSuccess && hipMalloc(&b->arena.d, b->arena.cap) == hipSuccess &&
             hipMalloc(&b->dF, 8 * cap) == hipSuccess && hipHostMalloc(&b->hF, 8 * cap) == hipSuccess &&
-            hipMalloc(&b->dres, 16 * cap) == hipSuccess && hipHostMalloc(&b->hres, 16 * cap) == hipSuccess;
+            hipMalloc(&b->dres, 16 * cap) == hipSuccess && hipHostMalloc(&b->hres, 16 * cap) == hipSuccess &&
+            hipMalloc(&b->dscal, sizeof(double) * SCAL_PER_SLOT * cap) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
   std::vector<float4> t;
   host_tables(t);
@@ -835,6 +888,7 @@ void a2c_batch_destroy(a2c_batch* b) {
   (void)hipFree(b->dF);
   if (b->hF) (void)hipHostFree(b->hF);
   (void)hipFree(b->dres);
+  (void)hipFree(b->dscal);
   if (b->hres) (void)hipHostFree(b->hres);
   (void)hipFree(b->dpart);
   (void)hipFree(b->dtab);
