@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstring>
 #include <new>
 #include <string>
@@ -219,16 +220,20 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
 // ---------------------------------------------------------------- host side
 // Pinned staging for per-launch descriptor arrays.  Space is handed out
 // bump-style and reclaimed only after the stream has been synchronised.
+// Launches that read staged descriptors are queued (q) and issued by flush()
+// after one copy of everything staged since the last flush: an optimizer
+// iteration is ~50 small launches, and one host-to-device copy per launch
+// would cost more than the kernels of a small lattice.
 struct Arena {
   char* h = nullptr;
   char* d = nullptr;
-  size_t cap = 0, used = 0;
+  size_t cap = 0, used = 0, copied = 0;
+  std::vector<std::function<void(hipStream_t)>> q;
   template <class T>
-  T* push(const T* src, size_t n, hipStream_t st) {
+  T* push(const T* src, size_t n) {
     const size_t bytes = ((n * sizeof(T)) + 255) & ~(size_t)255;
     if (used + bytes > cap) return nullptr;
     std::memcpy(h + used, src, n * sizeof(T));
-    if (hipMemcpyAsync(d + used, h + used, n * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess) return nullptr;
     T* out = reinterpret_cast<T*>(d + used);
     used += bytes;
     return out;
@@ -291,9 +296,23 @@ int berr(a2c_batch* b, int code, const std::string& m) {
     if (e_ != hipSuccess) return berr((b), A2C_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// copy the staged descriptors, then issue the queued launches in order
+int flush(a2c_batch* b) {
+  Arena& a = b->arena;
+  if (a.used > a.copied) {
+    B_HIP(b, hipMemcpyAsync(a.d + a.copied, a.h + a.copied, a.used - a.copied, hipMemcpyHostToDevice, b->stream));
+    a.copied = a.used;
+  }
+  for (auto& f : a.q) f(b->stream);
+  a.q.clear();
+  B_HIP(b, hipGetLastError());
+  return A2C_OK;
+}
+
 int bsync(a2c_batch* b) {
+  if (int rc = flush(b)) return rc;
   B_HIP(b, hipStreamSynchronize(b->stream));
-  b->arena.used = 0;
+  b->arena.used = b->arena.copied = 0;
   return A2C_OK;
 }
 
@@ -328,8 +347,8 @@ int seg_launch_prep(a2c_batch* b, const std::vector<int>& ids, Fill fill, Op** d
   size_t need = ((ops.size() * sizeof(Op) + 255) & ~(size_t)255) + ((map.size() * 4 + 255) & ~(size_t)255);
   if (b->arena.used + need > b->arena.cap)
     if (int rc = bsync(b)) return rc;
-  *dops = b->arena.push(ops.data(), ops.size(), b->stream);
-  *db2l = map.empty() ? nullptr : b->arena.push(map.data(), map.size(), b->stream);
+  *dops = b->arena.push(ops.data(), ops.size());
+  *db2l = map.empty() ? nullptr : b->arena.push(map.data(), map.size());
   if (!*dops || (!map.empty() && !*db2l)) return berr(b, A2C_ENOMEM, "staging arena too small");
   return A2C_OK;
 }
@@ -348,8 +367,8 @@ int seg_axpby(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   int32_t* map;
   int nblk;
   if (int rc = seg_launch_prep<SegAx>(b, ids, fill0, &ops, &map, &nblk)) return rc;
-  if (nblk) hipLaunchKernelGGL(k_seg_axpby, dim3(nblk), dim3(TB), 0, b->stream, ops, map);
-  B_HIP(b, hipGetLastError());
+  if (nblk)
+    b->arena.q.emplace_back([=](hipStream_t st) { hipLaunchKernelGGL(k_seg_axpby, dim3(nblk), dim3(TB), 0, st, ops, map); });
   return A2C_OK;
 }
 
@@ -369,10 +388,14 @@ int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<do
   int32_t* map;
   int nblk;
   if (int rc = seg_launch_prep<SegDot>(b, ids, fill2, &ops, &map, &nblk)) return rc;
-  if (nblk) hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, b->stream, ops, map, b->dpart, nblk);
-  B_HIP(b, hipGetLastError());
-  hipLaunchKernelGGL(k_seg_dot_final, dim3(ids.size()), dim3(1024), 0, b->stream, ops, b->dpart, nblk);
-  B_HIP(b, hipGetLastError());
+  double* part = b->dpart;
+  const unsigned nops = (unsigned)ids.size();
+  if (nblk)
+    b->arena.q.emplace_back(
+        [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk); });
+  b->arena.q.emplace_back(
+      [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot_final, dim3(nops), dim3(1024), 0, st, ops, part, nblk); });
+  if (int rc = flush(b)) return rc;
   B_HIP(b, hipMemcpyAsync(b->hres, b->dres, sizeof(double) * 2 * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
   if (int rc = bsync(b)) return rc;
   for (size_t k = 0; k < ids.size(); ++k) {
@@ -396,10 +419,13 @@ int seg_dot_dev(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   int32_t* map;
   int nblk;
   if (int rc = seg_launch_prep<SegDot>(b, ids, fill2, &ops, &map, &nblk)) return rc;
-  if (nblk) hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, b->stream, ops, map, b->dpart, nblk);
-  B_HIP(b, hipGetLastError());
-  hipLaunchKernelGGL(k_seg_dot_final, dim3(ids.size()), dim3(1024), 0, b->stream, ops, b->dpart, nblk);
-  B_HIP(b, hipGetLastError());
+  double* part = b->dpart;
+  const unsigned nops = (unsigned)ids.size();
+  if (nblk)
+    b->arena.q.emplace_back(
+        [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk); });
+  b->arena.q.emplace_back(
+      [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot_final, dim3(nops), dim3(1024), 0, st, ops, part, nblk); });
   return A2C_OK;
 }
 
@@ -410,8 +436,9 @@ int seg_bound(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   int32_t* map;
   int nblk;
   if (int rc = seg_launch_prep<SegBound>(b, ids, fill, &ops, &map, &nblk)) return rc;
-  if (nblk) hipLaunchKernelGGL(k_seg_bound<MODE>, dim3(nblk), dim3(TB), 0, b->stream, ops, map);
-  B_HIP(b, hipGetLastError());
+  if (nblk)
+    b->arena.q.emplace_back(
+        [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_bound<MODE>, dim3(nblk), dim3(TB), 0, st, ops, map); });
   return A2C_OK;
 }
 
@@ -442,9 +469,10 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   size_t need = ((lats.size() * sizeof(LatDev) + 255) & ~(size_t)255) + ((map.size() * 4 + 255) & ~(size_t)255);
   if (b->arena.used + need > b->arena.cap)
     if (int rc = bsync(b)) return rc;
-  LatDev* dl = b->arena.push(lats.data(), lats.size(), b->stream);
-  int32_t* dm = map.empty() ? nullptr : b->arena.push(map.data(), map.size(), b->stream);
+  LatDev* dl = b->arena.push(lats.data(), lats.size());
+  int32_t* dm = map.empty() ? nullptr : b->arena.push(map.data(), map.size());
   if (!dl || (!map.empty() && !dm)) return berr(b, A2C_ENOMEM, "staging arena too small");
+  if (int rc = flush(b)) return rc;  // the evaluation kernels are launched directly
   Launch L;
   L.lats = dl;
   L.b2l = dm;
@@ -994,6 +1022,7 @@ int a2c_batch_lmvm_start(a2c_batch* b, int32_t slot, const a2c_lmvm_opts* o) {
       !(o->sigma_ub > o->sigma_lb) || (o->precond != 0 && o->precond != 1))
     return berr(b, A2C_EINVAL, "bad options");
   Slot& s = b->slots[slot];
+  if (int rc = bsync(b)) return rc;  // queued launches may still use the buffers lmvm_alloc replaces
   s.opts = *o;
   s.m = o->m;
   if (int rc = lmvm_alloc(b, s)) return rc;
