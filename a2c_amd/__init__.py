@@ -228,12 +228,13 @@ def lattice(n, a=3.89, seed=1, lattice_id=0, disp_std=0.02, sigma=(0.10, 0.10),
 
 
 def eam_tables():
-    """FP32 cubic tables of the mixed path: array (5, 500, 4); table order
-    f_Pd, f_H, phi_Pd, phi_PdH, phi_H; interval k covers [1+0.01k, 1+0.01(k+1))."""
+    """FP32 cubic tables of the mixed path: array (5, NT, 4); table order
+    f_Pd, f_H, phi_Pd, phi_PdH, phi_H; NT intervals of h = 5/NT A cover
+    [1, 6) A, interval k being [1 + h k, 1 + h (k+1))."""
     n = lib.a2c_eam_tables(None, 0)
     out = np.zeros(n, dtype=np.float32)
     lib.a2c_eam_tables(out.ctypes.data_as(C.POINTER(C.c_float)), n)
-    return out.reshape(5, 500, 4)
+    return out.reshape(5, n // 20, 4)
 
 
 def philox4x32_10(ctr, key):

@@ -31,7 +31,8 @@ namespace a2c {
 namespace {
 
 constexpr int TB = SITE_BLOCK;
-constexpr int NT = 500;               // intervals per table
+constexpr int NT = 500;               // intervals per table (h = 0.0125 would fit 32 KB of LDS, but the
+                                      // f_Pd derivative error then reaches 1.4e-5 of its maximum)
 constexpr float R0 = 1.0f, R1 = 6.0f, INVH = 100.0f;
 constexpr double H64 = 0.01;
 constexpr double QW = 1.0 / 12.0;     // 0.5/dim, quadrature.cpp:39

@@ -80,15 +80,16 @@ def test_mixed_tables_reproduce_the_closed_forms():
     closed forms and analytic derivatives, away from the C1 switch radii."""
     import oracle as O
     tab = A.eam_tables().astype(np.float64)
+    h = 5.0 / tab.shape[1]
     fns = [O.RHO_PD, O.RHO_H, O.PHI_PD, O.PHI_PDH, O.PHI_H]
     r = np.linspace(1.6, 5.99, 3001)
     r = r[np.min(np.abs(r[:, None] - np.array([5.0, 5.05, 5.35])[None]), axis=1) > 0.02]
-    k = np.floor((r - 1.0) / 0.01).astype(int)
-    u = (r - 1.0) / 0.01 - k
+    k = np.floor((r - 1.0) / h).astype(int)
+    u = (r - 1.0) / h - k
     for t, fn in enumerate(fns):
         c = tab[t, k]
         v = c[:, 0] + u * (c[:, 1] + u * (c[:, 2] + u * c[:, 3]))
-        dv = (c[:, 1] + u * (2 * c[:, 2] + 3 * u * c[:, 3])) / 0.01
+        dv = (c[:, 1] + u * (2 * c[:, 2] + 3 * u * c[:, 3])) / h
         ev, ed = O.eam(fn, r), O.eam_d(fn, r)
         assert np.max(np.abs(v - ev)) <= 2e-7 * np.abs(ev).max(), t
         assert abs(np.mean(v - ev)) <= 1e-9 * np.abs(ev).max(), t
