@@ -449,8 +449,8 @@ template <class StateOf>
 int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf state_of, bool sync = true) {
   if (ids.empty()) return A2C_OK;
   std::vector<LatDev> lats(ids.size());
-  std::vector<int32_t> map;
-  int cum = 0;
+  std::vector<int32_t> map, omap;  // site block -> lattice; pair-owner workgroup -> lattice
+  int cum = 0, ocum = 0;
   for (size_t k = 0; k < ids.size(); ++k) {
     Slot& s = b->slots[ids[k]];
     LatDev L = s.fe->sys();
@@ -461,22 +461,30 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
     L.G = G;
     L.f = b->dF + ids[k];
     L.blk0 = cum;
+    L.kl = lanes_per_site(s.N());
+    L.owg0 = ocum;
     lats[k] = L;
     const int nb = site_blocks(s.N());
     map.insert(map.end(), nb, (int32_t)k);
+    omap.insert(omap.end(), (size_t)nb * L.kl, (int32_t)k);
     cum += nb;
+    ocum += nb * L.kl;
   }
-  size_t need = ((lats.size() * sizeof(LatDev) + 255) & ~(size_t)255) + ((map.size() * 4 + 255) & ~(size_t)255);
+  size_t need = ((lats.size() * sizeof(LatDev) + 255) & ~(size_t)255) + ((map.size() * 4 + 255) & ~(size_t)255) +
+                ((omap.size() * 4 + 255) & ~(size_t)255);
   if (b->arena.used + need > b->arena.cap)
     if (int rc = bsync(b)) return rc;
   LatDev* dl = b->arena.push(lats.data(), lats.size());
   int32_t* dm = map.empty() ? nullptr : b->arena.push(map.data(), map.size());
-  if (!dl || (!map.empty() && !dm)) return berr(b, A2C_ENOMEM, "staging arena too small");
+  int32_t* dom = omap.empty() ? nullptr : b->arena.push(omap.data(), omap.size());
+  if (!dl || (!map.empty() && (!dm || !dom))) return berr(b, A2C_ENOMEM, "staging arena too small");
   if (int rc = flush(b)) return rc;  // the evaluation kernels are launched directly
   Launch L;
   L.lats = dl;
   L.b2l = dm;
   L.nblk = cum;
+  L.ob2l = dom;
+  L.onblk = ocum;
   L.nlat = (int)ids.size();
   L.stream = b->stream;
   L.deriv_mode = b->desc.deriv_mode;

@@ -92,10 +92,11 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
             hipMalloc(&c->dpart, 8 * (size_t)c->nblk_red) == hipSuccess && hipMalloc(&c->dF, 64) == hipSuccess &&
             hipHostMalloc(&c->hX, 32 * N) == hipSuccess && hipHostMalloc(&c->hG, 32 * N) == hipSuccess &&
             hipHostMalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && hipMalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
-            hipMalloc(&c->db2l, 4 * (size_t)c->nblk_red) == hipSuccess;
+            hipMalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
   new (c->hlat) LatDev();
-  if (hipMemset(c->db2l, 0, 4 * (size_t)c->nblk_red) != hipSuccess) return fail(A2C_EHIP);
+  // all-zero block -> lattice map, long enough for the pair-owner launches too
+  if (hipMemset(c->db2l, 0, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) != hipSuccess) return fail(A2C_EHIP);
   c->lat_dirty = true;
   if (desc->precision == A2C_PREC_MIXED && build_tables(c) != A2C_OK) return fail(A2C_EHIP);
   *out = c;

@@ -375,14 +375,16 @@ __device__ __forceinline__ const Ell& partner_of(const LatDev& S, int list) {
 
 // 8 waves per SIMD: the lookups are latency-bound, 64 VGPRs hold this pass without spills
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_dens_own(
-    const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l, const float4* __restrict__ gtab) {
+    const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l, const float4* __restrict__ gtab) {
   __shared__ float4 tab[2 * NT];  // f_Pd, f_H
   stage_tables(tab, gtab, 2 * NT);
-  const int blk = xcd_block();
-  const LatDev& S = lats[b2l[blk]];
-  const int s = (blk - S.blk0) * TB + threadIdx.x;
+  const int w = xcd_block();
+  const LatDev& S = lats[ob2l[w]];
+  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
-  if (s >= N) return;
+  if (s >= N) return;  // whole lane groups
   const double* __restrict__ X = S.X;
   const double* __restrict__ xo = S.x;
   const bool pd = s < S.npd;
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const Ell& P = partner_of(S, list);
     const int n = L.ocnt[row];
-    for (int t = 0; t < n; ++t) {
+    for (int t = ph; t < n; t += K) {
       const size_t e = (size_t)t * L.rows + row;
       const int jl = L.oj[e];
       const int j = side == 0 ? jl : S.npd + jl;
@@ -411,7 +413,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_
       if (rv >= 0) reinterpret_cast<float*>(P.rbuf)[(size_t)rv * P.rows + jl] = theirs;
     }
   }
-  S.rho[s] = rho;
+  rho = site_sum(rho, K);
+  if (ph == 0) S.rho[s] = rho;
 }
 
 __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
@@ -485,15 +488,17 @@ __device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d
   return p;
 }
 
-__global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
+__global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l,
                                                  const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
   stage_grad_tables(tab, gtab);
-  const int blk = xcd_block();
-  const LatDev& S = lats[b2l[blk]];
-  const int s = (blk - S.blk0) * TB + threadIdx.x;
+  const int w = xcd_block();
+  const LatDev& S = lats[ob2l[w]];
+  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
-  if (s >= N) return;
+  if (s >= N) return;  // whole lane groups
   const double* __restrict__ X = S.X;
   const double* __restrict__ xo = S.x;
   const float2* __restrict__ aux = S.aux;
@@ -509,7 +514,7 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const Ell& P = partner_of(S, list);
     const int n = L.ocnt[row];
-    for (int t = 0; t < n; ++t) {
+    for (int t = ph; t < n; t += K) {
       const size_t ei = (size_t)t * L.rows + row;
       const int jl = L.oj[ei];
       const int j = side == 0 ? jl : S.npd + jl;
@@ -532,6 +537,12 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
         reinterpret_cast<float4*>(P.rbuf)[(size_t)rv * P.rows + jl] = make_float4(-p.g[0], -p.g[1], -p.g[2], p.gsj);
     }
   }
+  e = site_sum(e, K);
+  g0 = site_sum(g0, K);
+  g1 = site_sum(g1, K);
+  g2 = site_sum(g2, K);
+  gs = site_sum(gs, K);
+  if (ph != 0) return;
   double* __restrict__ G = S.G;
   G[4 * (size_t)s + 0] = g0;
   G[4 * (size_t)s + 1] = g1;
@@ -641,13 +652,13 @@ int run_eval32(const Launch& L, Timing* tm) {
     }
   } else if (L.nblk > 0) {
     if (tm) tm->mark(L.stream, "dens_own");
-    hipLaunchKernelGGL(k_dens_own, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+    hipLaunchKernelGGL(k_dens_own, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "dens_recv");
     hipLaunchKernelGGL(k_dens_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "grad_own");
-    hipLaunchKernelGGL(k_grad_own, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
+    hipLaunchKernelGGL(k_grad_own, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "grad_recv");
     hipLaunchKernelGGL(k_grad_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);

@@ -278,12 +278,14 @@ __device__ __forceinline__ const Ell& partner_list(const LatDev& S, int list) {
   return list == 0 ? S.pdpd : (list == 1 ? S.hpd : (list == 2 ? S.pdh : S.hh));
 }
 
-__global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
-  const int blk = xcd_block();
-  const LatDev& S = lats[b2l[blk]];
-  const int s = (blk - S.blk0) * TB + threadIdx.x;
+__global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
+  const int w = xcd_block();
+  const LatDev& S = lats[ob2l[w]];
+  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
-  if (s >= N) return;
+  if (s >= N) return;  // whole lane groups
   const double* __restrict__ X = S.X;
   const double* __restrict__ xo = S.x;
   const bool pd = s < S.npd;
@@ -298,7 +300,7 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
     const Ell& P = partner_list(S, list);
     double* __restrict__ share = reinterpret_cast<double*>(P.rbuf);
     const int n = L.ocnt[row];
-    for (int t = 0; t < n; ++t) {
+    for (int t = ph; t < n; t += K) {
       const size_t e = (size_t)t * L.rows + row;
       const int jl = L.oj[e];
       const Site b = load_site(X, side == 0 ? jl : S.npd + jl);
@@ -321,7 +323,8 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
       if (rv >= 0) share[(size_t)rv * P.rows + jl] = theirs;
     }
   }
-  S.rho[s] = rho;
+  rho = site_sum(rho, K);
+  if (ph == 0) S.rho[s] = rho;
 }
 
 __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
@@ -380,12 +383,14 @@ __device__ __forceinline__ void pair_both(const double d[3], double ai, double a
   }
 }
 
-__global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
-  const int blk = xcd_block();
-  const LatDev& S = lats[b2l[blk]];
-  const int s = (blk - S.blk0) * TB + threadIdx.x;
+__global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
+  const int w = xcd_block();
+  const LatDev& S = lats[ob2l[w]];
+  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
-  if (s >= N) return;
+  if (s >= N) return;  // whole lane groups
   const double* __restrict__ X = S.X;
   const double* __restrict__ xo = S.x;
   const double* __restrict__ Fp = S.Fp;
@@ -402,7 +407,7 @@ __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ la
     const Ell& P = partner_list(S, list);
     double4* __restrict__ share = reinterpret_cast<double4*>(P.rbuf);
     const int n = L.ocnt[row];
-    for (int t = 0; t < n; ++t) {
+    for (int t = ph; t < n; t += K) {
       const size_t ei = (size_t)t * L.rows + row;
       const int jl = L.oj[ei];
       const int j = side == 0 ? jl : S.npd + jl;
@@ -423,6 +428,12 @@ __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ la
       if (rv >= 0) share[(size_t)rv * P.rows + jl] = make_double4(-pg[0], -pg[1], -pg[2], psj);
     }
   }
+  e = site_sum(e, K);
+  g0 = site_sum(g0, K);
+  g1 = site_sum(g1, K);
+  g2 = site_sum(g2, K);
+  gs = site_sum(gs, K);
+  if (ph != 0) return;
   double* __restrict__ G = S.G;
   G[4 * (size_t)s + 0] = g0;
   G[4 * (size_t)s + 1] = g1;
@@ -511,13 +522,13 @@ int run_eval64(const Launch& L, Timing* tm) {
   if (L.deriv_mode == A2C_DERIV_ANALYTIC && !L.full_pairs64) {
     if (L.nblk > 0) {
       if (tm) tm->mark(L.stream, "dens_own64");
-      hipLaunchKernelGGL(k_dens_own64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
+      hipLaunchKernelGGL(k_dens_own64, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l);
       if (hipGetLastError() != hipSuccess) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "dens_recv64");
       hipLaunchKernelGGL(k_dens_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
       if (hipGetLastError() != hipSuccess) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "grad_own64");
-      hipLaunchKernelGGL(k_grad_own64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
+      hipLaunchKernelGGL(k_grad_own64, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l);
       if (hipGetLastError() != hipSuccess) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "grad_recv64");
       hipLaunchKernelGGL(k_grad_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
@@ -572,6 +583,8 @@ static Launch single_launch(a2c_fe_ctx* c) {
   L.full_pairs64 = fp64_full_pairs();
   L.lats = c->dlat;
   L.b2l = c->db2l;
+  L.ob2l = c->db2l;
+  L.onblk = site_blocks(c->N) * lanes_per_site(c->N);
   L.nblk = site_blocks(c->N);
   L.nlat = 1;
   L.stream = c->stream;
@@ -608,6 +621,8 @@ int sync_lattice(a2c_fe_ctx* c, const double* dX, double* dG, double* dF) {
   A2C_HIP(c, hipStreamSynchronize(c->stream));  // the pinned copy may still be in flight
   h = c->sys();
   h.blk0 = 0;
+  h.kl = lanes_per_site(c->N);
+  h.owg0 = 0;
   h.X = dX;
   h.G = dG;
   h.f = dF;

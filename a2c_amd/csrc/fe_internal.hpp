@@ -52,6 +52,8 @@ struct LatDev {
   int32_t npd = 0, nh = 0;
   int32_t periodic[3] = {0, 0, 0};
   int32_t blk0 = 0;          // first block of this lattice in the current launch
+  int32_t kl = 1;            // lanes per site in the pair-owner passes (lanes_per_site)
+  int32_t owg0 = 0;          // first workgroup of this lattice in the pair-owner launches
   double box[3] = {0, 0, 0};
   double ibox[3] = {0, 0, 0};  // 1/box: the image shift of a listed pair is L*rint(d/L) with d/L as d*(1/L)
   double T = 0;
@@ -75,6 +77,22 @@ __host__ __device__ inline int site_blocks(int n) { return (n + SITE_BLOCK - 1) 
 // round-robin by index; this bijection gives XCD k one contiguous range of
 // site blocks, so the neighbor data its waves gather stays within its own L2
 // instead of every XCD touching the whole lattice.
+// Lanes per site in the pair-owner passes.  A small lattice has too few
+// sites to fill the GPU with one lane per site, and each lane would walk its
+// ~40 owned pairs x 12 points serially; splitting a site's pairs over 8 lanes
+// (combined in a fixed shuffle order) cuts that latency 8x.  The choice
+// depends only on the lattice, so its results are the same in any batch.
+inline int lanes_per_site(int n_sites) { return n_sites < 65536 ? 8 : 1; }
+constexpr int MAX_LANES_PER_SITE = 8;
+
+// Fixed-order sum over the K lanes of a site (K a power of two <= 64, lanes
+// contiguous within a wave); the total lands in the group's first lane.
+template <typename T>
+__device__ inline T site_sum(T v, int K) {
+  for (int off = K >> 1; off > 0; off >>= 1) v += __shfl_down(v, off, K);
+  return v;
+}
+
 constexpr int NUM_XCD = 8;
 __device__ inline int xcd_block() {
   const int b = blockIdx.x, n = gridDim.x;
@@ -174,6 +192,8 @@ struct Launch {
   int deriv_mode;
   double fd_eps;
   const float4* tab;  // mixed path
+  const int32_t* ob2l = nullptr;  // pair-owner workgroup -> lattice
+  int onblk = 0;                  // pair-owner workgroups
   bool mixed_full = false;    // mixed path: every pair from both ends (A/B reference)
   bool full_pairs64 = false;  // FP64 analytic path: every pair from both ends (A/B reference)
 };
