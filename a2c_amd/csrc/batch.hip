@@ -461,14 +461,16 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
     L.G = G;
     L.f = b->dF + ids[k];
     L.blk0 = cum;
-    L.kl = lanes_per_site(s.N());
-    L.owg0 = ocum;
+    L.kl32 = lanes_per_site(s.N(), false);
+    L.kl64 = lanes_per_site(s.N(), true);
+    const int kl = fp64 ? L.kl64 : L.kl32;
+    L.owg0_32 = L.owg0_64 = ocum;
     lats[k] = L;
     const int nb = site_blocks(s.N());
     map.insert(map.end(), nb, (int32_t)k);
-    omap.insert(omap.end(), (size_t)nb * L.kl, (int32_t)k);
+    omap.insert(omap.end(), (size_t)nb * kl, (int32_t)k);
     cum += nb;
-    ocum += nb * L.kl;
+    ocum += nb * kl;
   }
   size_t need = ((lats.size() * sizeof(LatDev) + 255) & ~(size_t)255) + ((map.size() * 4 + 255) & ~(size_t)255) +
                 ((omap.size() * 4 + 255) & ~(size_t)255);

@@ -380,8 +380,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_
   stage_tables(tab, gtab, 2 * NT);
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
-  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
-  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int K = S.kl32;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0_32) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
   if (s >= N) return;  // whole lane groups
@@ -494,8 +494,8 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
   stage_grad_tables(tab, gtab);
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
-  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
-  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int K = S.kl32;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0_32) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
   if (s >= N) return;  // whole lane groups

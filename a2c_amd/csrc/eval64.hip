@@ -281,8 +281,8 @@ __device__ __forceinline__ const Ell& partner_list(const LatDev& S, int list) {
 __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
-  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
-  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
   if (s >= N) return;  // whole lane groups
@@ -386,8 +386,8 @@ __device__ __forceinline__ void pair_both(const double d[3], double ai, double a
 __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
-  const int K = S.kl;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
-  const int s = (w - S.owg0) * (TB / K) + (int)threadIdx.x / K;
+  const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
+  const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
   const int N = S.npd + S.nh;
   if (s >= N) return;  // whole lane groups
@@ -577,14 +577,14 @@ bool fp64_full_pairs() {
   return full;
 }
 
-static Launch single_launch(a2c_fe_ctx* c) {
+static Launch single_launch(a2c_fe_ctx* c, bool fp64) {
   Launch L;
   L.mixed_full = mixed_full_pairs();
   L.full_pairs64 = fp64_full_pairs();
   L.lats = c->dlat;
   L.b2l = c->db2l;
   L.ob2l = c->db2l;
-  L.onblk = site_blocks(c->N) * lanes_per_site(c->N);
+  L.onblk = site_blocks(c->N) * lanes_per_site(c->N, fp64);
   L.nblk = site_blocks(c->N);
   L.nlat = 1;
   L.stream = c->stream;
@@ -596,20 +596,20 @@ static Launch single_launch(a2c_fe_ctx* c) {
 
 int launch_density64(a2c_fe_ctx* c, const double* dX) {
   if (int rc = sync_lattice(c, dX, c->dG, c->dF)) return rc;
-  if (run_density64(single_launch(c))) return set_err(c, A2C_EHIP, "density64 launch failed");
+  if (run_density64(single_launch(c, true))) return set_err(c, A2C_EHIP, "density64 launch failed");
   return A2C_OK;
 }
 
 int launch_eval64(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
   if (int rc = sync_lattice(c, dX, dG, dF)) return rc;
-  if (run_eval64(single_launch(c), timed ? tm : nullptr)) return set_err(c, A2C_EHIP, "eval64 launch failed");
+  if (run_eval64(single_launch(c, true), timed ? tm : nullptr)) return set_err(c, A2C_EHIP, "eval64 launch failed");
   return A2C_OK;
 }
 
 int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
   if (!c->dtab) return set_err(c, A2C_ESTATE, "tables not built");
   if (int rc = sync_lattice(c, dX, dG, dF)) return rc;
-  if (run_eval32(single_launch(c), timed ? tm : nullptr)) return set_err(c, A2C_EHIP, "eval32 launch failed");
+  if (run_eval32(single_launch(c, false), timed ? tm : nullptr)) return set_err(c, A2C_EHIP, "eval32 launch failed");
   return A2C_OK;
 }
 
@@ -621,8 +621,9 @@ int sync_lattice(a2c_fe_ctx* c, const double* dX, double* dG, double* dF) {
   A2C_HIP(c, hipStreamSynchronize(c->stream));  // the pinned copy may still be in flight
   h = c->sys();
   h.blk0 = 0;
-  h.kl = lanes_per_site(c->N);
-  h.owg0 = 0;
+  h.kl32 = lanes_per_site(c->N, false);
+  h.kl64 = lanes_per_site(c->N, true);
+  h.owg0_32 = h.owg0_64 = 0;
   h.X = dX;
   h.G = dG;
   h.f = dF;

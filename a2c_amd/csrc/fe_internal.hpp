@@ -52,8 +52,8 @@ struct LatDev {
   int32_t npd = 0, nh = 0;
   int32_t periodic[3] = {0, 0, 0};
   int32_t blk0 = 0;          // first block of this lattice in the current launch
-  int32_t kl = 1;            // lanes per site in the pair-owner passes (lanes_per_site)
-  int32_t owg0 = 0;          // first workgroup of this lattice in the pair-owner launches
+  int32_t kl32 = 1, kl64 = 1;    // lanes per site in the mixed / FP64 pair-owner passes
+  int32_t owg0_32 = 0, owg0_64 = 0;  // first workgroup of this lattice in those launches
   double box[3] = {0, 0, 0};
   double ibox[3] = {0, 0, 0};  // 1/box: the image shift of a listed pair is L*rint(d/L) with d/L as d*(1/L)
   double T = 0;
@@ -81,8 +81,11 @@ __host__ __device__ inline int site_blocks(int n) { return (n + SITE_BLOCK - 1) 
 // sites to fill the GPU with one lane per site, and each lane would walk its
 // ~40 owned pairs x 12 points serially; splitting a site's pairs over 8 lanes
 // (combined in a fixed shuffle order) cuts that latency 8x.  The choice
-// depends only on the lattice, so its results are the same in any batch.
-inline int lanes_per_site(int n_sites) { return n_sites < 65536 ? 8 : 1; }
+// depends only on the lattice (and the precision), so its results are the
+// same in any batch.  The mixed passes stage 40 KB of tables per workgroup,
+// so they split only lattices small enough that the extra workgroups stay
+// few; the FP64 passes split up to 65,536 sites.
+inline int lanes_per_site(int n_sites, bool fp64) { return n_sites < (fp64 ? 65536 : 16384) ? 8 : 1; }
 constexpr int MAX_LANES_PER_SITE = 8;
 
 // Fixed-order sum over the K lanes of a site (K a power of two <= 64, lanes
