@@ -22,7 +22,8 @@ REF_QUAD = os.path.join(HERE, "_ref", "quadrature_ref")
 F_PD, F_H, RHO_PD, RHO_H, PHI_PD, PHI_PDH, PHI_H = range(7)
 FUNC_NAMES = ["F_Pd", "F_H", "f_Pd", "f_H", "phi_Pd", "phi_PdH", "phi_H"]
 DERIV = {"analytic": 0, "fd": 1}
-POT = {"eam": 0, "test": 1}
+POT = {"eam": 0, "test": 1, "test4": 2}
+QUAD = {"r12": 0, "gh3": 1}
 
 
 def build(reference=None):
@@ -37,7 +38,7 @@ class Params(C.Structure):
     _fields_ = [
         ("n_pd", C.c_int32), ("n_h", C.c_int32), ("box", C.c_double * 3), ("periodic", C.c_int32 * 3),
         ("rc", C.c_double), ("a_pd", C.c_double), ("T", C.c_double), ("deriv_mode", C.c_int32),
-        ("eps", C.c_double), ("potential", C.c_int32), ("threads", C.c_int32),
+        ("eps", C.c_double), ("potential", C.c_int32), ("threads", C.c_int32), ("quad_rule", C.c_int32),
     ]
 
 
@@ -105,7 +106,7 @@ class Oracle:
     potential and diffusion, in FP64 on the CPU."""
 
     def __init__(self, n_pd, n_h, box=(0, 0, 0), periodic=(0, 0, 0), rc=5.35, a_pd=3.89, T=300.0,
-                 deriv="analytic", eps=1e-5, potential="eam", threads=1):
+                 deriv="analytic", eps=1e-5, potential="eam", threads=1, quad="r12"):
         p = Params()
         p.n_pd, p.n_h = n_pd, n_h
         for c in range(3):
@@ -113,6 +114,7 @@ class Oracle:
             p.periodic[c] = int(bool(periodic[c]))
         p.rc, p.a_pd, p.T = rc, a_pd, T
         p.deriv_mode, p.eps, p.potential, p.threads = DERIV[deriv], eps, POT[potential], threads
+        p.quad_rule = QUAD[quad]
         self.p = p
         self.n_pd, self.n_h, self.N = n_pd, n_h, n_pd + n_h
         self._nb = None

@@ -229,6 +229,7 @@ int g_potential = OR_POT_EAM;
 
 double eam_val(int fn, double x) {
   if (g_potential == OR_POT_TEST) return (fn == OR_F_PD || fn == OR_F_H) ? x : x * x;
+  if (g_potential == OR_POT_TEST4) return (fn == OR_F_PD || fn == OR_F_H) ? x : x * x * x * x;
   switch (fn) {
     case OR_F_PD: return F_Pd(x);
     case OR_F_H: return F_H(x);
@@ -246,6 +247,7 @@ double eam_val(int fn, double x) {
 double eam_der(int fn, double x, int mode, double eps) {
   if (mode == OR_DERIV_FD) return (eam_val(fn, x + eps) - eam_val(fn, x - eps)) / (2.0 * eps);
   if (g_potential == OR_POT_TEST) return (fn == OR_F_PD || fn == OR_F_H) ? 1.0 : 2.0 * x;
+  if (g_potential == OR_POT_TEST4) return (fn == OR_F_PD || fn == OR_F_H) ? 1.0 : 4.0 * x * x * x;
   switch (fn) {
     case OR_F_PD: return F_Pd_a(x).d;
     case OR_F_H: return F_H_a(x).d;
@@ -289,6 +291,45 @@ Quad make_quad() {
   return q;
 }
 const Quad QUAD = make_quad();
+
+/* Nodes of one pair under either rule: relative vector r_k, weight w_k and
+ * dsi_k, the derivative of r_k w.r.t. sigma_i up to the factor sigfac (the
+ * 12-point rule keeps the reference's sqrt(2) as a final factor on the
+ * summed sigma gradient, minimizer.cpp:221,283). */
+struct Nodes {
+  int K;
+  V3 r[27];
+  double w[27];
+  V3 dsi[27];
+};
+const double SQ3 = 1.7320508075688772935;
+inline void pair_nodes(int rule, const V3& qi, double si, const V3& qj, double sj, Nodes& n) {
+  const double sq2 = std::sqrt(2.0);
+  if (rule != OR_QUAD_GH3) {
+    n.K = QK;
+    for (int k = 0; k < QK; ++k) {
+      n.r[k] = vsub(vsub(vadd(qi, vscale(sq2 * si, QUAD.xi1[k])), qj), vscale(sq2 * sj, QUAD.xi2[k]));
+      n.w[k] = QUAD.w;
+      n.dsi[k] = QUAD.xi1[k];
+    }
+    return;
+  }
+  static const double z1[3] = {-SQ3, 0.0, SQ3}, w1[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+  const V3 d = vsub(qi, qj);
+  const double s = std::sqrt(si * si + sj * sj);
+  const double ds = s > 0.0 ? si / s : 0.0;
+  n.K = 27;
+  int k = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c, ++k) {
+        const V3 z = {{z1[a], z1[b], z1[c]}};
+        n.r[k] = vadd(d, vscale(s, z));
+        n.w[k] = w1[a] * w1[b] * w1[c];
+        n.dsi[k] = vscale(ds, z);
+      }
+}
+inline double sigma_factor(int rule) { return rule == OR_QUAD_GH3 ? 1.0 : std::sqrt(2.0); }
 
 const double KB = 8.6173324e-5; /* minimizer.cpp:92 */
 
@@ -502,30 +543,25 @@ Lists lists_of(const or_nbr* nb) {
 }
 
 double site_density(const Sys& S, const Lists& L, const double* x, int s) {
-  const double sq2 = std::sqrt(2.0), QWp = QUAD.w;
+  const int rule = S.p->quad_rule;
   double rho = 0.0;
   const bool pd = s < S.npd;
   const int i = pd ? s : s - S.npd;
   const int wl0 = pd ? 0 : 2, wl1 = pd ? 1 : 3;
   const V3 qi = S.q(s);
   const double si = S.sig(s);
+  Nodes n;
   for (int t = 0; t < L.n(wl0, i); ++t) { /* Pd neighbors */
     int j = L.at(wl0, i, t);
     V3 qj = image_of(S.p, qi, S.q(j));
-    double sj = S.sig(j);
-    for (int k = 0; k < QK; ++k) {
-      V3 r = vsub(vsub(vadd(qi, vscale(sq2 * si, QUAD.xi1[k])), qj), vscale(sq2 * sj, QUAD.xi2[k]));
-      rho += eam_val(OR_RHO_PD, vnorm(r)) * QWp;
-    }
+    pair_nodes(rule, qi, si, qj, S.sig(j), n);
+    for (int k = 0; k < n.K; ++k) rho += eam_val(OR_RHO_PD, vnorm(n.r[k])) * n.w[k];
   }
   for (int t = 0; t < L.n(wl1, i); ++t) { /* H neighbors, weighted by occupancy */
     int a = L.at(wl1, i, t), j = S.npd + a;
     V3 qj = image_of(S.p, qi, S.q(j));
-    double sj = S.sig(j);
-    for (int k = 0; k < QK; ++k) {
-      V3 r = vsub(vsub(vadd(qi, vscale(sq2 * si, QUAD.xi1[k])), qj), vscale(sq2 * sj, QUAD.xi2[k]));
-      rho += x[a] * eam_val(OR_RHO_H, vnorm(r)) * QWp;
-    }
+    pair_nodes(rule, qi, si, qj, S.sig(j), n);
+    for (int k = 0; k < n.K; ++k) rho += x[a] * eam_val(OR_RHO_H, vnorm(n.r[k])) * n.w[k];
   }
   return rho;
 }
@@ -555,7 +591,7 @@ extern "C" int or_eval(const or_params* p, const or_nbr* nb, const double* X,
   g_potential = p->potential;
   const int npd = p->n_pd, nh = p->n_h, N = npd + nh;
   const int md = p->deriv_mode;
-  const double eps = p->eps, T = p->T, QWp = QUAD.w, sq2 = std::sqrt(2.0);
+  const double eps = p->eps, T = p->T, sq2 = sigma_factor(p->quad_rule);
   Sys S{p, X, npd, nh};
   Lists L = lists_of(nb);
   std::vector<double> rho(N), sv(N), ss(N);
@@ -575,19 +611,21 @@ extern "C" int or_eval(const or_params* p, const or_nbr* nb, const double* X,
     double sum_V = 0.0;
     V3 Fq = {{0, 0, 0}};
     double Fs = 0.0;
-    auto acc = [&](double c, const V3& r, int k) { /* Fq += c*r; Fsigma += c*r . xi1 */
+    Nodes n;
+    auto acc = [&](double c, const V3& r, int k) { /* Fq += c*r; Fsigma += c*r . dr/dsigma_i */
       V3 cr = vscale(c, r);
       Fq = vadd(Fq, cr);
-      Fs += vdot(cr, QUAD.xi1[k]);
+      Fs += vdot(cr, n.dsi[k]);
     };
     if (pd) {
       sum_V += eam_val(OR_F_PD, rPd[i]);
       for (int t = 0; t < L.n(0, i); ++t) { /* Pd-Pd pairs :172-194 */
         int jj = L.at(0, i, t);
         V3 qj = image_of(p, qi, S.q(jj));
-        double sigmaj = S.sig(jj);
-        for (int k = 0; k < QK; ++k) {
-          V3 r = vsub(vsub(vadd(qi, vscale(sq2 * sigmai, QUAD.xi1[k])), qj), vscale(sq2 * sigmaj, QUAD.xi2[k]));
+        pair_nodes(p->quad_rule, qi, sigmai, qj, S.sig(jj), n);
+        for (int k = 0; k < n.K; ++k) {
+          const V3& r = n.r[k];
+          const double QWp = n.w[k];
           acc(QWp * eam_der(OR_F_PD, rPd[i], md, eps) * eam_der(OR_RHO_PD, vnorm(r), md, eps) / vnorm(r), r, k);
           acc(QWp * eam_der(OR_F_PD, rPd[jj], md, eps) * eam_der(OR_RHO_PD, vnorm(r), md, eps) / vnorm(r), r, k);
           sum_V += 0.5 * QWp * eam_val(OR_PHI_PD, vnorm(r));
@@ -597,9 +635,10 @@ extern "C" int or_eval(const or_params* p, const or_nbr* nb, const double* X,
       for (int t = 0; t < L.n(1, i); ++t) { /* Pd-H pairs :197-219 */
         int jj = L.at(1, i, t);
         V3 qj = image_of(p, qi, S.q(npd + jj));
-        double sigmaj = S.sig(npd + jj);
-        for (int k = 0; k < QK; ++k) {
-          V3 r = vsub(vsub(vadd(qi, vscale(sq2 * sigmai, QUAD.xi1[k])), qj), vscale(sq2 * sigmaj, QUAD.xi2[k]));
+        pair_nodes(p->quad_rule, qi, sigmai, qj, S.sig(npd + jj), n);
+        for (int k = 0; k < n.K; ++k) {
+          const V3& r = n.r[k];
+          const double QWp = n.w[k];
           acc(x0[jj] * QWp * eam_der(OR_F_PD, rPd[i], md, eps) * eam_der(OR_RHO_H, vnorm(r), md, eps) / vnorm(r), r, k);
           acc(x0[jj] * QWp * eam_der(OR_F_H, rH[jj], md, eps) * eam_der(OR_RHO_PD, vnorm(r), md, eps) / vnorm(r), r, k);
           sum_V += 0.5 * x0[jj] * QWp * eam_val(OR_PHI_PDH, vnorm(r));
@@ -615,9 +654,10 @@ extern "C" int or_eval(const or_params* p, const or_nbr* nb, const double* X,
       for (int t = 0; t < L.n(2, i); ++t) { /* H-Pd pairs :234-256 */
         int jj = L.at(2, i, t);
         V3 qj = image_of(p, qi, S.q(jj));
-        double sigmaj = S.sig(jj);
-        for (int k = 0; k < QK; ++k) {
-          V3 r = vsub(vsub(vadd(qi, vscale(sq2 * sigmai, QUAD.xi1[k])), qj), vscale(sq2 * sigmaj, QUAD.xi2[k]));
+        pair_nodes(p->quad_rule, qi, sigmai, qj, S.sig(jj), n);
+        for (int k = 0; k < n.K; ++k) {
+          const V3& r = n.r[k];
+          const double QWp = n.w[k];
           acc(QWp * eam_der(OR_F_H, rH[i], md, eps) * eam_der(OR_RHO_PD, vnorm(r), md, eps) / vnorm(r), r, k);
           acc(QWp * eam_der(OR_F_PD, rPd[jj], md, eps) * eam_der(OR_RHO_H, vnorm(r), md, eps) / vnorm(r), r, k);
           sum_V += 0.5 * x0[i] * QWp * eam_val(OR_PHI_PDH, vnorm(r));
@@ -627,9 +667,10 @@ extern "C" int or_eval(const or_params* p, const or_nbr* nb, const double* X,
       for (int t = 0; t < L.n(3, i); ++t) { /* H-H pairs :259-281 */
         int jj = L.at(3, i, t);
         V3 qj = image_of(p, qi, S.q(npd + jj));
-        double sigmaj = S.sig(npd + jj);
-        for (int k = 0; k < QK; ++k) {
-          V3 r = vsub(vsub(vadd(qi, vscale(sq2 * sigmai, QUAD.xi1[k])), qj), vscale(sq2 * sigmaj, QUAD.xi2[k]));
+        pair_nodes(p->quad_rule, qi, sigmai, qj, S.sig(npd + jj), n);
+        for (int k = 0; k < n.K; ++k) {
+          const V3& r = n.r[k];
+          const double QWp = n.w[k];
           acc(x0[jj] * QWp * eam_der(OR_F_H, rH[i], md, eps) * eam_der(OR_RHO_H, vnorm(r), md, eps) / vnorm(r), r, k);
           acc(x0[jj] * QWp * eam_der(OR_F_H, rH[jj], md, eps) * eam_der(OR_RHO_H, vnorm(r), md, eps) / vnorm(r), r, k);
           sum_V += 0.5 * x0[i] * x0[jj] * QWp * eam_val(OR_PHI_H, vnorm(r));
@@ -661,7 +702,7 @@ extern "C" int or_chem_potential(const or_params* p, const or_nbr* nb, const dou
   g_potential = p->potential;
   const int npd = p->n_pd, nh = p->n_h, N = npd + nh;
   const int md = p->deriv_mode;
-  const double eps = p->eps, T = p->T, QWp = QUAD.w, sq2 = std::sqrt(2.0);
+  const double eps = p->eps, T = p->T;
   Sys S{p, X, npd, nh};
   Lists L = lists_of(nb);
   std::vector<double> rho(N), out(nh, 0.0);
@@ -675,6 +716,7 @@ extern "C" int or_chem_potential(const or_params* p, const or_nbr* nb, const dou
     const int s = npd + i;
     const V3 qi = S.q(s);
     const double sigmai = S.sig(s);
+    Nodes n;
     double sum_Vx = 0.0;
     sum_Vx += eam_val(OR_F_H, rH[i]);
     const double nHPd = (double)L.n(2, i);
@@ -687,9 +729,10 @@ extern "C" int or_chem_potential(const or_params* p, const or_nbr* nb, const dou
     for (int j = 0; j < L.n(2, i); ++j) { /* H-Pd :679-698 */
       int jj = L.at(2, i, j);
       V3 qj = image_of(p, qi, S.q(jj));
-      double sigmaj = S.sig(jj);
-      for (int k = 0; k < QK; ++k) {
-        V3 r = vsub(vsub(vadd(qi, vscale(sq2 * sigmai, QUAD.xi1[k])), qj), vscale(sq2 * sigmaj, QUAD.xi2[k]));
+      pair_nodes(p->quad_rule, qi, sigmai, qj, S.sig(jj), n);
+      for (int k = 0; k < n.K; ++k) {
+        const V3& r = n.r[k];
+        const double QWp = n.w[k];
         sum_Vx += QWp * eam_der(OR_F_PD, rPd[jj], md, eps) * eam_val(OR_RHO_H, vnorm(r));
         sum_Vx += QWp * eam_val(OR_PHI_PDH, vnorm(r));
       }
@@ -699,9 +742,10 @@ extern "C" int or_chem_potential(const or_params* p, const or_nbr* nb, const dou
     for (int j = 0; j < L.n(3, i); ++j) { /* H-H :701-719 */
       int jj = L.at(3, i, j);
       V3 qj = image_of(p, qi, S.q(npd + jj));
-      double sigmaj = S.sig(npd + jj);
-      for (int k = 0; k < QK; ++k) {
-        V3 r = vsub(vsub(vadd(qi, vscale(sq2 * sigmai, QUAD.xi1[k])), qj), vscale(sq2 * sigmaj, QUAD.xi2[k]));
+      pair_nodes(p->quad_rule, qi, sigmai, qj, S.sig(npd + jj), n);
+      for (int k = 0; k < n.K; ++k) {
+        const V3& r = n.r[k];
+        const double QWp = n.w[k];
         sum_Vx += x0[jj] * QWp * eam_der(OR_F_H, rH[jj], md, eps) * eam_val(OR_RHO_H, vnorm(r));
         sum_Vx += x0[jj] * QWp * eam_val(OR_PHI_H, vnorm(r));
       }

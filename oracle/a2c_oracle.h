@@ -24,6 +24,9 @@
  *                   analytic derivatives.
  *   or_quadrature   bit-exact vs the reference's gaussianQuadrature() built
  *                   from quadrature.cpp (oracle/_ref); Gaussian moments.
+ *   GH3 rule        degree-4 test potential against the closed-form Gaussian
+ *                   moment |d|^4 + 10 s^2 |d|^2 + 15 s^4 (which the 12-point
+ *                   rule misses); FD gradient; FD chemical potential.
  *   or_nbr_*        brute-force all-pairs; FCC shell counts; list symmetry.
  *   or_density,
  *   or_eval         sigma->0 limit equals static EAM energy; degree-2 test
@@ -46,8 +49,15 @@ enum { OR_F_PD = 0, OR_F_H = 1, OR_RHO_PD = 2, OR_RHO_H = 3,
        OR_PHI_PD = 4, OR_PHI_PDH = 5, OR_PHI_H = 6, OR_NFUNC = 7 };
 enum { OR_DERIV_ANALYTIC = 0, OR_DERIV_FD = 1 };
 /* OR_POT_TEST replaces every EAM function by g(r)=r^2 (pairs and density) and
- * F(rho)=rho, so the averaged energy has a closed-form Gaussian moment. */
-enum { OR_POT_EAM = 0, OR_POT_TEST = 1 };
+ * F(rho)=rho, so the averaged energy has a closed-form Gaussian moment;
+ * OR_POT_TEST4 uses g(r)=r^4 (exact for GH3, not for the 12-point rule). */
+enum { OR_POT_EAM = 0, OR_POT_TEST = 1, OR_POT_TEST4 = 2 };
+/* Phase-packet rules: the reference's 12-point rule (quadrature.cpp:43-75), or
+ * a 3-point-per-axis Gauss-Hermite product rule on the relative coordinate
+ * (27 nodes; builder addition, not in the reference): every pair function
+ * depends only on r = d + sqrt(2)(sigma_i xi_1 - sigma_j xi_2) ~ N(d, s^2 I)
+ * with s^2 = sigma_i^2 + sigma_j^2, so E g(|r|) = sum_k w_k g(|d + s z_k|). */
+enum { OR_QUAD_R12 = 0, OR_QUAD_GH3 = 1 };
 
 typedef struct or_params {
   int32_t n_pd, n_h;
@@ -60,6 +70,7 @@ typedef struct or_params {
   double eps;         /* FD step (reference: input.file.eps) */
   int32_t potential;  /* OR_POT_* */
   int32_t threads;    /* OpenMP threads for the per-site loops (<=0: 1) */
+  int32_t quad_rule;  /* OR_QUAD_* */
 } or_params;
 
 double or_eam(int fn, double x);
