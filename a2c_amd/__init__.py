@@ -36,6 +36,7 @@ A2C_ENOTCONV = -7
 
 DERIV = {"analytic": 0, "fd": 1}
 PREC = {"fp64": 0, "mixed": 1}
+QUAD = {"r12": 0, "gh3": 1}  # 12-point rule of quadrature.cpp:43-75; 27-node Gauss-Hermite
 LIST_IDS = {"PdPd": 0, "PdH": 1, "HPd": 2, "HH": 3, "HH1": 4}
 
 
@@ -252,10 +253,11 @@ class FreeEntropy:
     """One lattice on one GPU (a2c_fe_ctx)."""
 
     def __init__(self, n_pd, n_h, box=(0.0, 0.0, 0.0), periodic=(0, 0, 0), rc=5.35, a_pd=3.89,
-                 T=300.0, deriv="analytic", eps=1e-5, precision="fp64", device=0, max_neighbors=0):
+                 T=300.0, deriv="analytic", eps=1e-5, precision="fp64", device=0, max_neighbors=0, quad="r12"):
         d = FeDesc()
         lib.a2c_fe_desc_default(C.byref(d))
         d.n_pd, d.n_h = n_pd, n_h
+        d.quad_rule = QUAD[quad]
         for c in range(3):
             d.box[c] = float(box[c])
             d.periodic[c] = int(bool(periodic[c]))
@@ -493,10 +495,12 @@ def lmvm_opts(**kw):
     return o
 
 
-def fe_desc(n_pd, n_h, box=(0.0, 0.0, 0.0), periodic=(0, 0, 0), rc=5.35, a_pd=3.89, T=300.0, max_neighbors=0):
+def fe_desc(n_pd, n_h, box=(0.0, 0.0, 0.0), periodic=(0, 0, 0), rc=5.35, a_pd=3.89, T=300.0, max_neighbors=0,
+            quad="r12"):
     d = FeDesc()
     lib.a2c_fe_desc_default(C.byref(d))
     d.n_pd, d.n_h = n_pd, n_h
+    d.quad_rule = QUAD[quad]
     for c in range(3):
         d.box[c] = float(box[c])
         d.periodic[c] = int(bool(periodic[c]))
@@ -549,8 +553,8 @@ class Batch:
     def mem_used(self):
         return lib.a2c_batch_mem_used(self._b)
 
-    def add(self, X, x, n_pd, n_h, box=(0, 0, 0), periodic=(0, 0, 0), T=300.0, rc=5.35, a_pd=3.89):
-        d = fe_desc(n_pd, n_h, box, periodic, rc, a_pd, T)
+    def add(self, X, x, n_pd, n_h, box=(0, 0, 0), periodic=(0, 0, 0), T=300.0, rc=5.35, a_pd=3.89, quad="r12"):
+        d = fe_desc(n_pd, n_h, box, periodic, rc, a_pd, T, quad=quad)
         X = _f64(X, 4 * (n_pd + n_h))
         x = _f64(x, n_h)
         slot = C.c_int32()

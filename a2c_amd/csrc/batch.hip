@@ -450,10 +450,11 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   if (ids.empty()) return A2C_OK;
   std::vector<LatDev> lats(ids.size());
   std::vector<int32_t> map, omap;  // site block -> lattice; pair-owner workgroup -> lattice
-  int cum = 0, ocum = 0;
+  int cum = 0, ocum = 0, rules = 0;
   for (size_t k = 0; k < ids.size(); ++k) {
     Slot& s = b->slots[ids[k]];
     LatDev L = s.fe->sys();
+    rules |= 1 << L.rule;
     const double* X;
     double* G;
     state_of(s, &X, &G);
@@ -488,6 +489,7 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   L.ob2l = dom;
   L.onblk = ocum;
   L.nlat = (int)ids.size();
+  L.rules = rules;
   L.stream = b->stream;
   L.deriv_mode = b->desc.deriv_mode;
   L.fd_eps = b->desc.fd_eps;

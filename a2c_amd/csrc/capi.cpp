@@ -69,7 +69,8 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
   if (!desc || !out) return A2C_EINVAL;
   *out = nullptr;
   if (desc->n_pd < 0 || desc->n_h < 0 || (int64_t)desc->n_pd + desc->n_h > (1 << 30)) return A2C_EINVAL;
-  if (!(desc->rc > 0) || !(desc->temperature > 0) || desc->quad_rule != A2C_QUAD_R12) return A2C_EINVAL;
+  if (!(desc->rc > 0) || !(desc->temperature > 0)) return A2C_EINVAL;
+  if (desc->quad_rule != A2C_QUAD_R12 && desc->quad_rule != A2C_QUAD_GH3) return A2C_EINVAL;
   if (desc->deriv_mode != A2C_DERIV_ANALYTIC && desc->deriv_mode != A2C_DERIV_FD) return A2C_EINVAL;
   if (desc->deriv_mode == A2C_DERIV_FD && !(desc->fd_eps > 0)) return A2C_EINVAL;
   if (desc->precision != A2C_PREC_FP64 && desc->precision != A2C_PREC_MIXED) return A2C_EINVAL;

@@ -8,7 +8,9 @@
 //         the per-site objective share and the global reduction.
 //   FP32: the 12 quadrature points of a pair: |r|^2 = |d|^2 + a^2 +- 2 a d_c
 //         (each point displaces d along one axis by a = sqrt(6) sigma), 1/|r|
-//         by v_rsq_f32, and the EAM functions.
+//         by v_rsq_f32, and the EAM functions.  Under the Gauss-Hermite rule
+//         the 27 nodes r_k = d + s sqrt(3) a_k (a_k in {-1,0,1}^3) have
+//         |r_k|^2 = |d|^2 + 3 s^2 |a_k|^2 + 2 sqrt(3) s (a_k . d).
 //
 // The EAM functions are piecewise cubic Hermite polynomials on a uniform grid
 // in r (h = 0.01 A over [1, 6] A, switch radii 5.0, 5.05, 5.35 on grid
@@ -38,6 +40,9 @@ constexpr double H64 = 0.01;
 constexpr double QW = 1.0 / 12.0;     // 0.5/dim, quadrature.cpp:39
 constexpr double SQ6 = 2.449489742783178098;
 constexpr float SQ6F = 2.449489742783178098f;
+constexpr int GH3 = A2C_QUAD_GH3;
+constexpr float GHZF = 1.7320508075688772935f;  // Gauss-Hermite node sqrt(3)
+__device__ constexpr float ghwf(int a) { return a == 0 ? 2.0f / 3.0f : 1.0f / 6.0f; }
 enum { T_FPD = 0, T_FH = 1, T_PHPD = 2, T_PHPDH = 3, T_PHH = 4, NTAB = 5 };
 
 // Exponential tails beyond the switch radii: funRc * exp(k (r - rs)).
@@ -368,18 +373,53 @@ __device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb,
   if (wb == wa) sb = sa;
 }
 
+// Weighted Gauss-Hermite sums of two density tables at the 27 nodes of a pair
+// (s = sqrt(sigma_i^2 + sigma_j^2)): a rolled loop over the z offset a2, the
+// 9 nodes of each plane unrolled (a fully unrolled 27 spills at 64 VGPRs).
+__device__ __forceinline__ void pair_density2_gh(const float4* tab, int wa, int wb, const float d[3], float sd,
+                                                 float& sa, float& sb) {
+  const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float s3 = 3.0f * sd * sd, tt = 2.0f * GHZF * sd;
+  const float t0 = tt * d[0], t1 = tt * d[1], t2 = tt * d[2];
+  sa = 0.0f;
+  sb = 0.0f;
+#pragma unroll 1
+  for (int a2 = -1; a2 <= 1; ++a2) {
+    const float base = D2 + s3 * (float)(a2 * a2) + (float)a2 * t2;
+    float pa = 0.0f, pb = 0.0f;
+#pragma unroll
+    for (int a1 = -1; a1 <= 1; ++a1) {
+#pragma unroll
+      for (int a0 = -1; a0 <= 1; ++a0) {
+        const float w = ghwf(a0) * ghwf(a1);
+        const float x = fmaxf(base + s3 * (float)(a0 * a0 + a1 * a1) + (a0 * t0 + a1 * t1), 1e-12f);
+        const float r = x * rsq(x);
+        pa += w * tab_v(tab, wa, r);
+        if (wb != wa) pb += w * tab_v(tab, wb, r);
+      }
+    }
+    const float w2 = a2 ? 1.0f / 6.0f : 2.0f / 3.0f;
+    sa += w2 * pa;
+    sb += w2 * pb;
+  }
+  if (wb == wa) sb = sa;
+}
+
 // Partner list of each list: PdPd<->PdPd, PdH<->HPd, HH<->HH.
 __device__ __forceinline__ const Ell& partner_of(const LatDev& S, int list) {
   return list == 0 ? S.pdpd : (list == 1 ? S.hpd : (list == 2 ? S.pdh : S.hh));
 }
 
-// 8 waves per SIMD: the lookups are latency-bound, 64 VGPRs hold this pass without spills
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_dens_own(
+// 8 waves per SIMD: the lookups are latency-bound, 64 VGPRs hold the 12-point
+// pass without spills (the Gauss-Hermite planes take 69: 7 waves)
+template <int RULE>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(RULE == GH3 ? 6 : 8))) void k_dens_own(
     const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l, const float4* __restrict__ gtab) {
   __shared__ float4 tab[2 * NT];  // f_Pd, f_H
-  stage_tables(tab, gtab, 2 * NT);
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
+  if (S.rule != RULE) return;  // whole workgroup: the other rule's instance serves this lattice
+  stage_tables(tab, gtab, 2 * NT);
   const int K = S.kl32;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_32) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -407,7 +447,12 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8))) void k_
       float d[3];
       pair_vec32(S, a, b, d);
       float mine, theirs;  // neighbor-type density (for me), my-type density (for the neighbor)
-      pair_density2(tab, side == 0 ? T_FPD : T_FH, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
+      if (RULE == GH3) {
+        const float bs = (float)b.s, as = (float)a.s;
+        pair_density2_gh(tab, side == 0 ? T_FPD : T_FH, mytab, d, sqrtf(as * as + bs * bs), mine, theirs);
+      } else {
+        pair_density2(tab, side == 0 ? T_FPD : T_FH, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
+      }
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
       const int rv = L.orv[e];
       if (rv >= 0) reinterpret_cast<float*>(P.rbuf)[(size_t)rv * P.rows + jl] = theirs;
@@ -438,7 +483,7 @@ __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lat
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)reinterpret_cast<const float*>(L.rbuf)[e];
     }
   }
-  rho *= QW;
+  rho *= S.rule == GH3 ? 1.0 : QW;
   S.rho[s] = rho;
   const double xs = pd ? 1.0 : xo[row];
   const eam::VD F = pd ? eam::F_Pd_vd(rho) : eam::F_H_vd(rho);
@@ -488,12 +533,68 @@ __device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d
   return p;
 }
 
+// Gauss-Hermite pair sums.  With u_k = w_k (c/|r_k|) and the node offsets
+// a_k, n_k = |a_k|^2:  U = sum u, Va = sum u a, Wn = sum u n;
+//   q-gradient  sum u r = d U + sqrt(3) s Va,
+//   Sz = sum u (r . z) = sqrt(3) d . Va + 3 s Wn,  d/dsigma_k = Sz sigma_k / s.
+template <int KIND>
+__device__ __forceinline__ PairSums2 pair_grad2_gh(const float4* tab, const float d[3], float si, float sj, float Fi,
+                                                   float Fj) {
+  const float sd = sqrtf(si * si + sj * sj);
+  const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float s3 = 3.0f * sd * sd, tt = 2.0f * GHZF * sd;
+  const float t0 = tt * d[0], t1 = tt * d[1], t2 = tt * d[2];
+  float pe = 0.0f, U = 0.0f, V0 = 0.0f, V1 = 0.0f, V2 = 0.0f, Wn = 0.0f;
+#pragma unroll 1  // one z plane (9 nodes, unrolled) per trip, as in pair_density2_gh
+  for (int a2 = -1; a2 <= 1; ++a2) {
+    const float base = D2 + s3 * (float)(a2 * a2) + (float)a2 * t2;
+    float p9 = 0.0f, u9 = 0.0f, v0 = 0.0f, v1 = 0.0f, n9 = 0.0f;
+#pragma unroll
+    for (int a1 = -1; a1 <= 1; ++a1) {
+#pragma unroll
+      for (int a0 = -1; a0 <= 1; ++a0) {
+        const int n = a0 * a0 + a1 * a1;
+        const float w = ghwf(a0) * ghwf(a1);
+        const float x = fmaxf(base + s3 * (float)n + (a0 * t0 + a1 * t1), 1e-12f);
+        const float rinv = rsq(x);
+        float phi, cc;
+        point_coef<KIND>(tab, x * rinv, Fi, Fj, phi, cc);
+        p9 += w * phi;
+        const float u = w * cc * rinv;
+        u9 += u;
+        if (a0) v0 += a0 * u;
+        if (a1) v1 += a1 * u;
+        if (n) n9 += n * u;
+      }
+    }
+    const float w2 = a2 ? 1.0f / 6.0f : 2.0f / 3.0f;
+    pe += w2 * p9;
+    U += w2 * u9;
+    V0 += w2 * v0;
+    V1 += w2 * v1;
+    V2 += (float)a2 * w2 * u9;
+    Wn += w2 * fmaf((float)(a2 * a2), u9, n9);
+  }
+  PairSums2 p;
+  p.pe = pe;
+  const float sv = GHZF * sd;
+  p.g[0] = INVH * fmaf(d[0], U, sv * V0);
+  p.g[1] = INVH * fmaf(d[1], U, sv * V1);
+  p.g[2] = INVH * fmaf(d[2], U, sv * V2);
+  const float sz = INVH * fmaf(GHZF, fmaf(d[0], V0, fmaf(d[1], V1, d[2] * V2)), 3.0f * sd * Wn);
+  p.gsi = sz * (si / sd);
+  p.gsj = sz * (sj / sd);
+  return p;
+}
+
+template <int RULE>
 __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l,
                                                  const float4* __restrict__ gtab) {
   __shared__ float4 tab[NTAB * NT];
-  stage_grad_tables(tab, gtab);
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
+  if (S.rule != RULE) return;  // whole workgroup: the other rule's instance serves this lattice
+  stage_grad_tables(tab, gtab);
   const int K = S.kl32;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_32) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -524,10 +625,18 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
       pair_vec32(S, a, b, d);
       const float aj = SQ6F * (float)b.s;
       PairSums2 p;
-      if (list == 0) p = pair_grad2<0>(tab, d, ai, aj, Fi, Fj);
-      else if (list == 1) p = pair_grad2<1>(tab, d, ai, aj, Fi, Fj);
-      else if (list == 2) p = pair_grad2<2>(tab, d, ai, aj, Fi, Fj);
-      else p = pair_grad2<3>(tab, d, ai, aj, Fi, Fj);
+      if (RULE == GH3) {
+        const float as = (float)a.s, bs = (float)b.s;
+        if (list == 0) p = pair_grad2_gh<0>(tab, d, as, bs, Fi, Fj);
+        else if (list == 1) p = pair_grad2_gh<1>(tab, d, as, bs, Fi, Fj);
+        else if (list == 2) p = pair_grad2_gh<2>(tab, d, as, bs, Fi, Fj);
+        else p = pair_grad2_gh<3>(tab, d, as, bs, Fi, Fj);
+      } else {
+        if (list == 0) p = pair_grad2<0>(tab, d, ai, aj, Fi, Fj);
+        else if (list == 1) p = pair_grad2<1>(tab, d, ai, aj, Fi, Fj);
+        else if (list == 2) p = pair_grad2<2>(tab, d, ai, aj, Fi, Fj);
+        else p = pair_grad2<3>(tab, d, ai, aj, Fi, Fj);
+      }
       const double w = side == 0 ? 1.0 : xo[jl];
       e += w * p.pe;
       g0 += w * p.g[0]; g1 += w * p.g[1]; g2 += w * p.g[2];
@@ -582,12 +691,14 @@ __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lat
     const double xi = pd ? 1.0 : xo[row];
     const double sg = S.X[4 * (size_t)s + 3];
     const double T = S.T;
-    const double V = S.emb[s] + QW * xi * S.epair[s];  // owned pairs carry their full energy
-    const double sc = QW * xi / T;
+    const bool gh = S.rule == GH3;
+    const double qw = gh ? 1.0 : QW, sq = gh ? 1.0 : SQ6;
+    const double V = S.emb[s] + qw * xi * S.epair[s];  // owned pairs carry their full energy
+    const double sc = qw * xi / T;
     G[4 * (size_t)s + 0] = sc * g0;
     G[4 * (size_t)s + 1] = sc * g1;
     G[4 * (size_t)s + 2] = sc * g2;
-    G[4 * (size_t)s + 3] = SQ6 * sc * gs - 3.0 * KB * xi / sg;
+    G[4 * (size_t)s + 3] = sq * sc * gs - 3.0 * KB * xi / sg;
     fs = V / T - 3.0 * KB * xi * log(sg);
   }
   red[threadIdx.x] = fs;
@@ -639,7 +750,7 @@ int build_tables(a2c_fe_ctx* c) {
 }
 
 int run_eval32(const Launch& L, Timing* tm) {
-  if (L.mixed_full) {  // every pair from both ends (no reverse buffers)
+  if (L.mixed_full && L.rules == (1 << A2C_QUAD_R12)) {  // every pair from both ends (12-point rule only)
     if (tm) tm->mark(L.stream, "density32");
     if (L.nblk > 0) {
       hipLaunchKernelGGL(k_density32, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.tab);
@@ -652,13 +763,19 @@ int run_eval32(const Launch& L, Timing* tm) {
     }
   } else if (L.nblk > 0) {
     if (tm) tm->mark(L.stream, "dens_own");
-    hipLaunchKernelGGL(k_dens_own, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
+    if (L.rules & (1 << A2C_QUAD_R12))
+      hipLaunchKernelGGL(k_dens_own<A2C_QUAD_R12>, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
+    if (L.rules & (1 << GH3))
+      hipLaunchKernelGGL(k_dens_own<GH3>, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "dens_recv");
     hipLaunchKernelGGL(k_dens_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "grad_own");
-    hipLaunchKernelGGL(k_grad_own, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
+    if (L.rules & (1 << A2C_QUAD_R12))
+      hipLaunchKernelGGL(k_grad_own<A2C_QUAD_R12>, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
+    if (L.rules & (1 << GH3))
+      hipLaunchKernelGGL(k_grad_own<GH3>, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l, L.tab);
     if (hipGetLastError() != hipSuccess) return A2C_EHIP;
     if (tm) tm->mark(L.stream, "grad_recv");
     hipLaunchKernelGGL(k_grad_recv, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);

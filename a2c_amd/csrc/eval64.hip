@@ -20,7 +20,10 @@
 // reproducible).  Blocks map to lattices through b2l, so one launch serves a
 // batch of uneven lattices.  The 12-point rule of quadrature.cpp:43-75 is used
 // in its structured form: point (c, +-) of site i displaces r along axis c by
-// +-sqrt(6) sigma_i, point (c, +-) of site j by -+sqrt(6) sigma_j.
+// +-sqrt(6) sigma_i, point (c, +-) of site j by -+sqrt(6) sigma_j.  The
+// Gauss-Hermite rule (A2C_QUAD_GH3) has its own instances of the pair
+// kernels; each instance skips the lattices of the other rule, and the
+// per-site kernels apply the rule's factors (1/12 and sqrt(6), or 1).
 #include <cmath>
 #include <cstdlib>
 #include <string>
@@ -34,6 +37,12 @@ namespace {
 constexpr int TB = SITE_BLOCK;
 constexpr double QW = 1.0 / 12.0;             // 0.5/dim, quadrature.cpp:39
 constexpr double SQ6 = 2.449489742783178098;  // sqrt(2)*sqrt(dim/2)
+constexpr int GH3 = A2C_QUAD_GH3;
+// Factors the per-site kernels apply to the pair sums: the 12-point sums are
+// unweighted (weight 1/12) and their sigma terms carry r . e_c, a factor
+// sqrt(6) short of d/dsigma; the Gauss-Hermite sums are complete.
+template <int RULE> constexpr double rule_qw() { return RULE == GH3 ? 1.0 : QW; }
+template <int RULE> constexpr double rule_sq() { return RULE == GH3 ? 1.0 : SQ6; }
 
 struct Site {
   double q[3];
@@ -76,12 +85,39 @@ __device__ __forceinline__ void point_r(int p, const double d[3], double ai, dou
   r[c] += disp;
 }
 
+// Sum of g(|r|) over the nodes of one pair: unweighted over the 12 points, or
+// weighted over the 27 Gauss-Hermite nodes (rule_qw completes either).
+template <int RULE, class Fn>
+__device__ __forceinline__ double pair_quad(const double d[3], double si, double sj, Fn g) {
+  double acc = 0.0;
+  if (RULE == GH3) {
+    const double sd = sqrt(si * si + sj * sj);
+#pragma unroll 1
+    for (int k = 0; k < 27; ++k) {
+      double z[3], w;
+      gh_node(k, z, w);
+      const double r0 = d[0] + sd * z[0], r1 = d[1] + sd * z[1], r2 = d[2] + sd * z[2];
+      acc += w * g(sqrt(r0 * r0 + r1 * r1 + r2 * r2));
+    }
+  } else {
+    const double ai = SQ6 * si, aj = SQ6 * sj;
+#pragma unroll
+    for (int p = 0; p < 12; ++p) {
+      double r[3];
+      point_r(p, d, ai, aj, r);
+      acc += g(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
+    }
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------- density
-template <int DM>
+template <int DM, int RULE>
 __global__ __launch_bounds__(TB) void k_density64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                   double eps) {
   const int blk = xcd_block();
   const LatDev& S = lats[b2l[blk]];
+  if (S.rule != RULE) return;  // the other rule's instance serves this lattice
   const int s = (blk - S.blk0) * TB + threadIdx.x;
   const int N = S.npd + S.nh;
   if (s >= N) return;
@@ -92,7 +128,7 @@ __global__ __launch_bounds__(TB) void k_density64(const LatDev* __restrict__ lat
   const Ell LP = pd ? S.pdpd : S.hpd;
   const Ell LH = pd ? S.pdh : S.hh;
   const Site a = load_site(X, s);
-  const double ai = SQ6 * a.s;
+  constexpr double qw = rule_qw<RULE>();
   double rho = 0.0;
   const int np = LP.cnt[row];
   for (int t = 0; t < np; ++t) {
@@ -100,15 +136,7 @@ __global__ __launch_bounds__(TB) void k_density64(const LatDev* __restrict__ lat
     const Site b = load_site(X, j);
     double d[3];
     pair_vec(S, a, b, d);
-    const double aj = SQ6 * b.s;
-    double acc = 0.0;
-#pragma unroll
-    for (int p = 0; p < 12; ++p) {
-      double r[3];
-      point_r(p, d, ai, aj, r);
-      acc += eam::f_Pd(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
-    }
-    rho += acc * QW;
+    rho += pair_quad<RULE>(d, a.s, b.s, [](double r) { return eam::f_Pd(r); }) * qw;
   }
   const int nhn = LH.cnt[row];
   for (int t = 0; t < nhn; ++t) {
@@ -116,15 +144,7 @@ __global__ __launch_bounds__(TB) void k_density64(const LatDev* __restrict__ lat
     const Site b = load_site(X, S.npd + h);
     double d[3];
     pair_vec(S, a, b, d);
-    const double aj = SQ6 * b.s;
-    double acc = 0.0;
-#pragma unroll
-    for (int p = 0; p < 12; ++p) {
-      double r[3];
-      point_r(p, d, ai, aj, r);
-      acc += eam::f_H(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
-    }
-    rho += xo[h] * acc * QW;
+    rho += xo[h] * pair_quad<RULE>(d, a.s, b.s, [](double r) { return eam::f_H(r); }) * qw;
   }
   S.rho[s] = rho;
   const double F = pd ? eam::F_Pd(rho) : eam::F_H(rho);
@@ -175,11 +195,35 @@ __device__ __forceinline__ void pair_point(double r, double Fi, double Fj, doubl
 }
 
 // Accumulate one neighbor pair: energy (phi summed over points), q-gradient
-// and sigma-gradient (r . xi1 is nonzero only on the six points of site i).
-template <int DM, int KIND>
-__device__ __forceinline__ void pair_accum(const double d[3], double ai, double aj, double Fi, double Fj,
+// and sigma-gradient (12-point rule: r . xi1 is nonzero only on the six
+// points of site i; Gauss-Hermite rule: sum of w (c/|r|) (r . z) sigma_i/s).
+template <int DM, int KIND, int RULE>
+__device__ __forceinline__ void pair_accum(const double d[3], double si, double sj, double Fi, double Fj,
                                            double eps, double& e, double g[3], double& gs) {
   double pe = 0.0, px = 0.0, py = 0.0, pz = 0.0, ps = 0.0;
+  if (RULE == GH3) {
+    const double sd = sqrt(si * si + sj * sj);
+#pragma unroll 1
+    for (int k = 0; k < 27; ++k) {
+      double z[3], w;
+      gh_node(k, z, w);
+      const double r[3] = {d[0] + sd * z[0], d[1] + sd * z[1], d[2] + sd * z[2]};
+      const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      double phi, c;
+      pair_point<DM, KIND>(rn, Fi, Fj, eps, phi, c);
+      pe += w * phi;
+      const double u = w * c / rn;
+      px += u * r[0];
+      py += u * r[1];
+      pz += u * r[2];
+      ps += u * (r[0] * z[0] + r[1] * z[1] + r[2] * z[2]);
+    }
+    e += pe;
+    g[0] += px; g[1] += py; g[2] += pz;
+    gs += ps * (si / sd);
+    return;
+  }
+  const double ai = SQ6 * si, aj = SQ6 * sj;
 #pragma unroll 1  // rolled, as in pair_both
   for (int p = 0; p < 12; ++p) {
     double r[3];
@@ -202,12 +246,13 @@ __device__ __forceinline__ void pair_accum(const double d[3], double ai, double 
   gs += ps;
 }
 
-template <int DM>
+template <int DM, int RULE>
 __global__ __launch_bounds__(TB) void k_grad64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                double eps) {
   __shared__ double red[TB];
   const int blk = xcd_block();
   const LatDev& S = lats[b2l[blk]];
+  if (S.rule != RULE) return;  // whole block: the other rule's instance serves this lattice
   const int lb = blk - S.blk0;
   const int s = lb * TB + threadIdx.x;
   const int N = S.npd + S.nh;
@@ -221,7 +266,6 @@ __global__ __launch_bounds__(TB) void k_grad64(const LatDev* __restrict__ lats, 
     const Ell LP = pd ? S.pdpd : S.hpd;
     const Ell LH = pd ? S.pdh : S.hh;
     const Site a = load_site(X, s);
-    const double ai = SQ6 * a.s;
     const double Fi = Fp[s];
     const double xi = pd ? 1.0 : xo[row];
     double ePd = 0.0, eH = 0.0, g[3] = {0.0, 0.0, 0.0}, gs = 0.0;
@@ -231,8 +275,8 @@ __global__ __launch_bounds__(TB) void k_grad64(const LatDev* __restrict__ lats, 
       const Site b = load_site(X, j);
       double d[3];
       pair_vec(S, a, b, d);
-      if (pd) pair_accum<DM, 0>(d, ai, SQ6 * b.s, Fi, Fp[j], eps, ePd, g, gs);
-      else pair_accum<DM, 2>(d, ai, SQ6 * b.s, Fi, Fp[j], eps, ePd, g, gs);
+      if (pd) pair_accum<DM, 0, RULE>(d, a.s, b.s, Fi, Fp[j], eps, ePd, g, gs);
+      else pair_accum<DM, 2, RULE>(d, a.s, b.s, Fi, Fp[j], eps, ePd, g, gs);
     }
     const int nhn = LH.cnt[row];
     for (int t = 0; t < nhn; ++t) {
@@ -242,21 +286,22 @@ __global__ __launch_bounds__(TB) void k_grad64(const LatDev* __restrict__ lats, 
       pair_vec(S, a, b, d);
       const double xh = xo[h];
       double e1 = 0.0, g1[3] = {0.0, 0.0, 0.0}, gs1 = 0.0;
-      if (pd) pair_accum<DM, 1>(d, ai, SQ6 * b.s, Fi, Fp[S.npd + h], eps, e1, g1, gs1);
-      else pair_accum<DM, 3>(d, ai, SQ6 * b.s, Fi, Fp[S.npd + h], eps, e1, g1, gs1);
+      if (pd) pair_accum<DM, 1, RULE>(d, a.s, b.s, Fi, Fp[S.npd + h], eps, e1, g1, gs1);
+      else pair_accum<DM, 3, RULE>(d, a.s, b.s, Fi, Fp[S.npd + h], eps, e1, g1, gs1);
       eH += xh * e1;
       g[0] += xh * g1[0]; g[1] += xh * g1[1]; g[2] += xh * g1[2];
       gs += xh * gs1;
     }
     // V share of site s: embedding + half of every pair (minimizer.cpp:169-286)
-    const double V = S.emb[s] + 0.5 * QW * xi * (ePd + eH);
+    constexpr double qw = rule_qw<RULE>(), sq = rule_sq<RULE>();
+    const double V = S.emb[s] + 0.5 * qw * xi * (ePd + eH);
     const double T = S.T;
-    const double sc = QW * xi / T;
+    const double sc = qw * xi / T;
     double* __restrict__ G = S.G;
     G[4 * (size_t)s + 0] = sc * g[0];
     G[4 * (size_t)s + 1] = sc * g[1];
     G[4 * (size_t)s + 2] = sc * g[2];
-    G[4 * (size_t)s + 3] = SQ6 * sc * gs - 3.0 * KB * xi / a.s;
+    G[4 * (size_t)s + 3] = sq * sc * gs - 3.0 * KB * xi / a.s;
     fs = V / T - 3.0 * KB * xi * log(a.s);
   }
   red[threadIdx.x] = fs;
@@ -278,9 +323,11 @@ __device__ __forceinline__ const Ell& partner_list(const LatDev& S, int list) {
   return list == 0 ? S.pdpd : (list == 1 ? S.hpd : (list == 2 ? S.pdh : S.hh));
 }
 
+template <int RULE>
 __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
+  if (S.rule != RULE) return;
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -306,17 +353,32 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
       const Site b = load_site(X, side == 0 ? jl : S.npd + jl);
       double d[3];
       pair_vec(S, a, b, d);
-      const double aj = SQ6 * b.s;
       double mine = 0.0, theirs = 0.0;  // neighbor-species density at s, s-species density at the neighbor
+      if (RULE == GH3) {
+        const double sd = sqrt(a.s * a.s + b.s * b.s);
 #pragma unroll 1
-      for (int p = 0; p < 12; ++p) {
-        double r[3];
-        point_r(p, d, ai, aj, r);
-        const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
-        const double fh = (side == 1 || !pd) ? eam::f_H(rn) : 0.0;
-        mine += side == 0 ? fpd : fh;
-        theirs += pd ? fpd : fh;
+        for (int k = 0; k < 27; ++k) {
+          double z[3], wk;
+          gh_node(k, z, wk);
+          const double r0 = d[0] + sd * z[0], r1 = d[1] + sd * z[1], r2 = d[2] + sd * z[2];
+          const double rn = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+          const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
+          const double fh = (side == 1 || !pd) ? eam::f_H(rn) : 0.0;
+          mine += wk * (side == 0 ? fpd : fh);
+          theirs += wk * (pd ? fpd : fh);
+        }
+      } else {
+        const double aj = SQ6 * b.s;
+#pragma unroll 1
+        for (int p = 0; p < 12; ++p) {
+          double r[3];
+          point_r(p, d, ai, aj, r);
+          const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+          const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
+          const double fh = (side == 1 || !pd) ? eam::f_H(rn) : 0.0;
+          mine += side == 0 ? fpd : fh;
+          theirs += pd ? fpd : fh;
+        }
       }
       rho += (side == 0 ? 1.0 : xo[jl]) * mine;
       const int rv = L.orv[e];
@@ -347,7 +409,7 @@ __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ l
       rho += (side == 0 ? 1.0 : xo[L.rj[e]]) * share[e];
     }
   }
-  rho *= QW;
+  rho *= S.rule == GH3 ? 1.0 : QW;
   S.rho[s] = rho;
   const double xs = pd ? 1.0 : xo[row];
   const eam::VD F = pd ? eam::F_Pd_vd(rho) : eam::F_H_vd(rho);
@@ -359,12 +421,35 @@ __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ l
 // (the partner's is -g), and the sigma terms of both ends (site s: its own
 // six points; the partner: the six points that displace it, seen from its
 // side with r -> -r).
-template <int KIND>
-__device__ __forceinline__ void pair_both(const double d[3], double ai, double aj, double Fi, double Fj,
+template <int KIND, int RULE>
+__device__ __forceinline__ void pair_both(const double d[3], double si, double sj, double Fi, double Fj,
                                           double& pe, double g[3], double& gsi, double& gsj) {
   pe = 0.0;
   g[0] = g[1] = g[2] = 0.0;
   gsi = gsj = 0.0;
+  if (RULE == GH3) {  // both ends move every node: dr/dsigma_k = z sigma_k / s
+    const double sd = sqrt(si * si + sj * sj);
+    double sz = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 27; ++k) {
+      double z[3], w;
+      gh_node(k, z, w);
+      const double r[3] = {d[0] + sd * z[0], d[1] + sd * z[1], d[2] + sd * z[2]};
+      const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      double phi, c;
+      pair_point<A2C_DERIV_ANALYTIC, KIND>(rn, Fi, Fj, 0.0, phi, c);
+      pe += w * phi;
+      const double u = w * c / rn;
+      g[0] += u * r[0];
+      g[1] += u * r[1];
+      g[2] += u * r[2];
+      sz += u * (r[0] * z[0] + r[1] * z[1] + r[2] * z[2]);
+    }
+    gsi = sz * (si / sd);
+    gsj = sz * (sj / sd);
+    return;
+  }
+  const double ai = SQ6 * si, aj = SQ6 * sj;
 #pragma unroll 1  // rolled: the FP64 closed forms keep ~140 VGPRs live (3 waves per SIMD) instead of ~260 (1)
   for (int p = 0; p < 12; ++p) {
     double r[3];
@@ -383,9 +468,11 @@ __device__ __forceinline__ void pair_both(const double d[3], double ai, double a
   }
 }
 
+template <int RULE>
 __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
+  if (S.rule != RULE) return;
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -397,7 +484,6 @@ __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ la
   const bool pd = s < S.npd;
   const int row = pd ? s : s - S.npd;
   const Site a = load_site(X, s);
-  const double ai = SQ6 * a.s;
   const double Fi = Fp[s];
   double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, gs = 0.0;
 #pragma unroll
@@ -414,12 +500,12 @@ __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ la
       const Site b = load_site(X, j);
       double d[3];
       pair_vec(S, a, b, d);
-      const double aj = SQ6 * b.s, Fj = Fp[j];
+      const double Fj = Fp[j];
       double pe, pg[3], psi, psj;
-      if (list == 0) pair_both<0>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
-      else if (list == 1) pair_both<1>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
-      else if (list == 2) pair_both<2>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
-      else pair_both<3>(d, ai, aj, Fi, Fj, pe, pg, psi, psj);
+      if (list == 0) pair_both<0, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
+      else if (list == 1) pair_both<1, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
+      else if (list == 2) pair_both<2, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
+      else pair_both<3, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
       const double w = side == 0 ? 1.0 : xo[jl];
       e += w * pe;
       g0 += w * pg[0]; g1 += w * pg[1]; g2 += w * pg[2];
@@ -472,12 +558,14 @@ __global__ __launch_bounds__(TB) void k_grad_recv64(const LatDev* __restrict__ l
     const double xi = pd ? 1.0 : xo[row];
     const double sg = S.X[4 * (size_t)s + 3];
     const double T = S.T;
-    const double V = S.emb[s] + QW * xi * S.epair[s];  // owned pairs carry their full energy
-    const double sc = QW * xi / T;
+    const bool gh = S.rule == GH3;
+    const double qw = gh ? 1.0 : QW, sq = gh ? 1.0 : SQ6;
+    const double V = S.emb[s] + qw * xi * S.epair[s];  // owned pairs carry their full energy
+    const double sc = qw * xi / T;
     G[4 * (size_t)s + 0] = sc * g0;
     G[4 * (size_t)s + 1] = sc * g1;
     G[4 * (size_t)s + 2] = sc * g2;
-    G[4 * (size_t)s + 3] = SQ6 * sc * gs - 3.0 * KB * xi / sg;
+    G[4 * (size_t)s + 3] = sq * sc * gs - 3.0 * KB * xi / sg;
     fs = V / T - 3.0 * KB * xi * log(sg);
   }
   red[threadIdx.x] = fs;
@@ -508,28 +596,39 @@ __global__ __launch_bounds__(1024) void k_final_sum(const LatDev* __restrict__ l
   if (threadIdx.x == 0) *S.f = red[0];
 }
 
-int run_density64(const Launch& L) {
-  if (L.nblk == 0) return A2C_OK;
-  if (L.deriv_mode == A2C_DERIV_FD)
-    hipLaunchKernelGGL(k_density64<A2C_DERIV_FD>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.fd_eps);
-  else
-    hipLaunchKernelGGL(k_density64<A2C_DERIV_ANALYTIC>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l,
-                       L.fd_eps);
+namespace {
+// Launch the instance of each quadrature rule present in the launch.
+template <template <int> class K, class... A>
+int per_rule(const Launch& L, int nblk, A... args) {
+  if (nblk == 0) return A2C_OK;
+  if (L.rules & (1 << A2C_QUAD_R12))
+    hipLaunchKernelGGL(K<A2C_QUAD_R12>::fn, dim3(nblk), dim3(TB), 0, L.stream, args...);
+  if (L.rules & (1 << GH3)) hipLaunchKernelGGL(K<GH3>::fn, dim3(nblk), dim3(TB), 0, L.stream, args...);
   return hipGetLastError() == hipSuccess ? A2C_OK : A2C_EHIP;
+}
+template <int R> struct DensFd { static constexpr auto fn = k_density64<A2C_DERIV_FD, R>; };
+template <int R> struct DensAn { static constexpr auto fn = k_density64<A2C_DERIV_ANALYTIC, R>; };
+template <int R> struct GradFd { static constexpr auto fn = k_grad64<A2C_DERIV_FD, R>; };
+template <int R> struct GradAn { static constexpr auto fn = k_grad64<A2C_DERIV_ANALYTIC, R>; };
+template <int R> struct DensOwn { static constexpr auto fn = k_dens_own64<R>; };
+template <int R> struct GradOwn { static constexpr auto fn = k_grad_own64<R>; };
+}  // namespace
+
+int run_density64(const Launch& L) {
+  if (L.deriv_mode == A2C_DERIV_FD) return per_rule<DensFd>(L, L.nblk, L.lats, L.b2l, L.fd_eps);
+  return per_rule<DensAn>(L, L.nblk, L.lats, L.b2l, L.fd_eps);
 }
 
 int run_eval64(const Launch& L, Timing* tm) {
   if (L.deriv_mode == A2C_DERIV_ANALYTIC && !L.full_pairs64) {
     if (L.nblk > 0) {
       if (tm) tm->mark(L.stream, "dens_own64");
-      hipLaunchKernelGGL(k_dens_own64, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l);
-      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+      if (per_rule<DensOwn>(L, L.onblk, L.lats, L.ob2l)) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "dens_recv64");
       hipLaunchKernelGGL(k_dens_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
       if (hipGetLastError() != hipSuccess) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "grad_own64");
-      hipLaunchKernelGGL(k_grad_own64, dim3(L.onblk), dim3(TB), 0, L.stream, L.lats, L.ob2l);
-      if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+      if (per_rule<GradOwn>(L, L.onblk, L.lats, L.ob2l)) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "grad_recv64");
       hipLaunchKernelGGL(k_grad_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
       if (hipGetLastError() != hipSuccess) return A2C_EHIP;
@@ -542,13 +641,10 @@ int run_eval64(const Launch& L, Timing* tm) {
   if (tm) tm->mark(L.stream, "density64");
   if (int rc = run_density64(L)) return rc;
   if (tm) tm->mark(L.stream, "grad64");
-  if (L.nblk > 0) {
-    if (L.deriv_mode == A2C_DERIV_FD)
-      hipLaunchKernelGGL(k_grad64<A2C_DERIV_FD>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l, L.fd_eps);
-    else
-      hipLaunchKernelGGL(k_grad64<A2C_DERIV_ANALYTIC>, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l,
-                         L.fd_eps);
-    if (hipGetLastError() != hipSuccess) return A2C_EHIP;
+  if (L.deriv_mode == A2C_DERIV_FD) {
+    if (per_rule<GradFd>(L, L.nblk, L.lats, L.b2l, L.fd_eps)) return A2C_EHIP;
+  } else if (per_rule<GradAn>(L, L.nblk, L.lats, L.b2l, L.fd_eps)) {
+    return A2C_EHIP;
   }
   if (tm) tm->mark(L.stream, "final_sum");
   if (int rc = run_final_sum(L)) return rc;
@@ -587,6 +683,7 @@ static Launch single_launch(a2c_fe_ctx* c, bool fp64) {
   L.onblk = site_blocks(c->N) * lanes_per_site(c->N, fp64);
   L.nblk = site_blocks(c->N);
   L.nlat = 1;
+  L.rules = 1 << c->desc.quad_rule;
   L.stream = c->stream;
   L.deriv_mode = c->desc.deriv_mode;
   L.fd_eps = c->desc.fd_eps;

@@ -57,6 +57,7 @@ struct LatDev {
   double box[3] = {0, 0, 0};
   double ibox[3] = {0, 0, 0};  // 1/box: the image shift of a listed pair is L*rint(d/L) with d/L as d*(1/L)
   double T = 0;
+  int32_t rule = 0;          // A2C_QUAD_R12 or A2C_QUAD_GH3
   Ell pdpd, pdh, hpd, hh, hh1;
   const double* X = nullptr;  // 4N state (32-byte aligned)
   double* G = nullptr;        // 4N gradient
@@ -96,6 +97,22 @@ template <typename T>
 __device__ inline T site_sum(T v, int K) {
   for (int off = K >> 1; off > 0; off >>= 1) v += __shfl_down(v, off, K);
   return v;
+}
+
+// 3-point Gauss-Hermite product rule on the relative coordinate: per axis
+// nodes -sqrt(3), 0, sqrt(3) (standard normal), weights 1/6, 2/3, 1/6.
+// Node k in 0..26 has per-axis offsets a_c = (k / 3^c) % 3 - 1, so
+// z = sqrt(3) a and weight w = prod_c gh_w(a_c).  The nodes of one pair are
+// r_k = d + s z_k with s = sqrt(sigma_i^2 + sigma_j^2), and
+// dr_k/dsigma_i = z_k sigma_i / s.
+constexpr double GH_Z = 1.7320508075688772935;
+__host__ __device__ constexpr double gh_w(int a) { return a == 0 ? 2.0 / 3.0 : 1.0 / 6.0; }
+__device__ inline void gh_node(int k, double z[3], double& w) {
+  const int a0 = k % 3 - 1, a1 = (k / 3) % 3 - 1, a2 = k / 9 - 1;
+  z[0] = GH_Z * a0;
+  z[1] = GH_Z * a1;
+  z[2] = GH_Z * a2;
+  w = gh_w(a0) * gh_w(a1) * gh_w(a2);
 }
 
 constexpr int NUM_XCD = 8;
@@ -165,6 +182,7 @@ struct a2c_fe_ctx {
       s.periodic[c] = desc.periodic[c];
     }
     s.T = desc.temperature;
+    s.rule = desc.quad_rule;
     s.pdpd = ell[0]; s.pdh = ell[1]; s.hpd = ell[2]; s.hh = ell[3]; s.hh1 = ell[4];
     s.x = dx; s.rho = drho; s.Fp = dFp; s.aux = daux; s.emb = demb; s.part = dpart; s.epair = depair;
     return s;
@@ -201,6 +219,7 @@ struct Launch {
   int onblk = 0;                  // pair-owner workgroups
   bool mixed_full = false;    // mixed path: every pair from both ends (A/B reference)
   bool full_pairs64 = false;  // FP64 analytic path: every pair from both ends (A/B reference)
+  int rules = 1;              // bit r set: some lattice of the launch uses quadrature rule r
 };
 bool mixed_full_pairs();  // A2C_MIXED_PAIRS=full selects the both-ends kernels
 bool fp64_full_pairs();   // A2C_FP64_PAIRS=full likewise for the FP64 kernels

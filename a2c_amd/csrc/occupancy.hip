@@ -14,7 +14,8 @@
 // square X_a^2; P_j and Q_b are computed once per site (k_chem_pre) instead of
 // once per (site, neighbor).  gamma = ln(x/(1-x)) + S_a/(kB T) + 1.5 for
 // 0 < x < 1 (else 1.5), capped at gamma_bd with full_H = 1 when x > 0.9999
-// (:721-722, :793-800); full_H sites keep gamma.
+// (:721-722, :793-800); full_H sites keep gamma.  The pair sums sum_k w use
+// the context's quadrature rule (12-point, w = 1/12, or Gauss-Hermite).
 //
 // Diffusion: f_a = gamma_a - ln(x_a/(1-x_a)) once per site (k_diff_f), then
 // x_a += (rate x_deriv) dt with x_deriv = sum over HH1(a) of
@@ -51,11 +52,27 @@ __device__ __forceinline__ void pair_vec(const DevSys& S, const Site& a, const S
   }
 }
 
-// 12-point sums of f_H and of phi (phi_PdH or phi_H) for one pair.
-template <int HH>
-__device__ __forceinline__ void pair_fh_phi(const double d[3], double ai, double aj, double& sf, double& sp) {
+// Sums of f_H and of phi (phi_PdH or phi_H) over the nodes of one pair:
+// unweighted over the 12 points (the caller applies 1/12), weighted over the
+// 27 Gauss-Hermite nodes.
+template <int HH, int RULE>
+__device__ __forceinline__ void pair_fh_phi(const double d[3], double si, double sj, double& sf, double& sp) {
   sf = 0.0;
   sp = 0.0;
+  if (RULE == A2C_QUAD_GH3) {
+    const double sd = sqrt(si * si + sj * sj);
+#pragma unroll 1
+    for (int k = 0; k < 27; ++k) {
+      double z[3], w;
+      gh_node(k, z, w);
+      const double r0 = d[0] + sd * z[0], r1 = d[1] + sd * z[1], r2 = d[2] + sd * z[2];
+      const double rn = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+      sf += w * eam::f_H(rn);
+      sp += w * eam::phi_morse<HH>(rn);
+    }
+    return;
+  }
+  const double ai = SQ6 * si, aj = SQ6 * sj;
 #pragma unroll
   for (int p = 0; p < 12; ++p) {
     const int c = (p >> 1) % 3;
@@ -85,6 +102,7 @@ __global__ void k_chem_pre(DevSys S, const double* __restrict__ xo, double T, do
   corr[s] = pd ? T * cp0 * (cp2 * n + cp3 * X) : T * ch0 * (ch2 * n + ch3 * X);
 }
 
+template <int RULE>
 __global__ __launch_bounds__(TB) void k_chem(DevSys S, const double* __restrict__ X, const double* __restrict__ xo,
                                              const double* __restrict__ rho, const double* __restrict__ Fp,
                                              const double* __restrict__ corr, double ch0, double ch1, double ch2,
@@ -95,7 +113,7 @@ __global__ __launch_bounds__(TB) void k_chem(DevSys S, const double* __restrict_
   if (full[a] == 1) return;  // fully occupied reservoir: gamma untouched
   const int s = S.npd + a;
   const Site si = load_site(X, s);
-  const double ai = SQ6 * si.s;
+  constexpr double qw = RULE == A2C_QUAD_GH3 ? 1.0 : QW;
   const double T = S.T;
   double sum = eam::F_H(rho[s]);
   // correction of the site itself
@@ -111,8 +129,8 @@ __global__ __launch_bounds__(TB) void k_chem(DevSys S, const double* __restrict_
     double d[3];
     pair_vec(S, si, b, d);
     double sf, sp;
-    pair_fh_phi<0>(d, ai, SQ6 * b.s, sf, sp);
-    sum += QW * (Fp[j] * sf + sp) + corr[j];
+    pair_fh_phi<0, RULE>(d, si.s, b.s, sf, sp);
+    sum += qw * (Fp[j] * sf + sp) + corr[j];
   }
   for (int t = 0; t < nhh; ++t) {  // H-H pairs (:701-719)
     const int h = S.hh.idx[(size_t)t * S.hh.rows + a];
@@ -120,8 +138,8 @@ __global__ __launch_bounds__(TB) void k_chem(DevSys S, const double* __restrict_
     double d[3];
     pair_vec(S, si, b, d);
     double sf, sp;
-    pair_fh_phi<1>(d, ai, SQ6 * b.s, sf, sp);
-    sum += xo[h] * (QW * (Fp[S.npd + h] * sf + sp) + corr[S.npd + h]);
+    pair_fh_phi<1, RULE>(d, si.s, b.s, sf, sp);
+    sum += xo[h] * (qw * (Fp[S.npd + h] * sf + sp) + corr[S.npd + h]);
   }
   const double x = xo[a];
   const double raw = (x > 0.0 && x < 1.0) ? log(x / (1.0 - x)) + sum / (KB * T) : 0.0;
@@ -177,8 +195,9 @@ int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const dou
     A2C_HIP(c, hipGetLastError());
   }
   if (S.nh > 0) {
-    hipLaunchKernelGGL(k_chem, dim3((S.nh + TB - 1) / TB), dim3(TB), 0, c->stream, S, dX, c->dx, c->drho, c->dFp,
-                       dcorr, coe_h[0], coe_h[1], coe_h[2], coe_h[3], gamma_bd, dgam, dfull);
+    hipLaunchKernelGGL(S.rule == A2C_QUAD_GH3 ? k_chem<A2C_QUAD_GH3> : k_chem<A2C_QUAD_R12>,
+                       dim3((S.nh + TB - 1) / TB), dim3(TB), 0, c->stream, S, dX, c->dx, c->drho, c->dFp, dcorr,
+                       coe_h[0], coe_h[1], coe_h[2], coe_h[3], gamma_bd, dgam, dfull);
     A2C_HIP(c, hipGetLastError());
   }
   return A2C_OK;

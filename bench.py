@@ -4,7 +4,8 @@
 Workload (one step = one objective+gradient evaluation of the whole lattice,
 inputs resident in HBM): FCC Pd 40x40x40 cells = 256,000 Pd + 256,000 H
 sites, a = 3.89 A, periodic, H occupancy iid U(0, 0.6), displacements
-N(0, 0.03 A), sigma U(0.08, 0.15) A, T = 500 K, rc = 5.35 A, 12-point rule.
+N(0, 0.03 A), sigma U(0.08, 0.15) A, T = 500 K, rc = 5.35 A, 12-point rule
+(--quad gh3: the 27-node Gauss-Hermite rule).
 
 Prints one JSON line:  value = sites * steps / seconds of the timed region
 (HIP events on the context stream, bracketed by device synchronisation),
@@ -28,7 +29,7 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector FP64 / FP32 (pub
 # gradient pass (FMA = 2, sqrt/exp/rcp/log = 1): Pd-Pd, Pd-H, H-H.
 FLOP_DENS = (62, 65, 27)
 FLOP_GRAD = (180, 104, 72)
-K = 12
+POINTS = {"r12": 12, "gh3": 27}  # quadrature points per pair
 
 
 def parse():
@@ -38,6 +39,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--cells", type=int, default=40)
     p.add_argument("--precision", choices=["mixed", "fp64"], default=os.environ.get("A2C_BENCH_PREC", "mixed"))
+    p.add_argument("--quad", choices=["r12", "gh3"], default="r12",
+                   help="quadrature rule of the single-lattice workload")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--calib", action="store_true",
@@ -82,7 +85,7 @@ def cpu_baseline(args, mk):
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     X, x, box, npd, nh = mk(6)
-    orc = O.Oracle(npd, nh, box, (1, 1, 1), T=500.0, deriv="fd", threads=threads)
+    orc = O.Oracle(npd, nh, box, (1, 1, 1), T=500.0, deriv="fd", threads=threads, quad=args.quad)
     orc.build_neighbors(X)
     orc.eval(X, x)  # warm-up
     n, t0 = 0, time.perf_counter()
@@ -225,12 +228,13 @@ def main():
     X, x, box, npd, nh = mk(args.cells)
     N = npd + nh
     log("workload: %d cells^3, %d sites" % (args.cells, N))
-    fe = A.FreeEntropy(npd, nh, box, (1, 1, 1), T=500.0, precision="fp64")
+    fe = A.FreeEntropy(npd, nh, box, (1, 1, 1), T=500.0, precision="fp64", quad=args.quad)
     fe.set_occupancy(x)
     t0 = time.perf_counter()
     st = fe.findNeighbors(X)
     nbr_ms = (time.perf_counter() - t0) * 1e3
     U = (st["total"][0] / 2.0, float(st["total"][1]), st["total"][3] / 2.0)  # unordered PdPd, PdH, HH
+    K = POINTS[args.quad]
     flop_grad = K * sum(c * u for c, u in zip(FLOP_GRAD, U))
     flop_dens = K * sum(c * u for c, u in zip(FLOP_DENS, U))
     log("neighbors: %.1f ms, max %d, totals %s" % (nbr_ms, st["max_neib"], st["total"]))
@@ -264,7 +268,7 @@ def main():
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
-        if tr.get("kernel") == gkern:
+        if tr.get("kernel") == gkern and tr.get("quad", "r12") == args.quad:
             traffic = tr["bytes_per_dispatch"]
     obj_err = abs(f - f64) / abs(f64)
     grad_err = float(np.linalg.norm(G - G64) / np.linalg.norm(G64))
@@ -282,8 +286,9 @@ def main():
         "dtype": "f32-bulk/f64-reduce" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": "single large lattice: FCC Pd 40^3 cells + octahedral H, periodic, "
-                               "x~U(0,0.6), disp N(0,0.03), sigma U(0.08,0.15), T=500K, R12 rule",
-                   "cells": args.cells, "sites": N, "precision": args.precision},
+                               "x~U(0,0.6), disp N(0,0.03), sigma U(0.08,0.15), T=500K, %s rule"
+                               % {"r12": "R12", "gh3": "GH3"}[args.quad],
+                   "cells": args.cells, "sites": N, "precision": args.precision, "quad": args.quad},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[args.precision], "unit": "TFLOP/s",
                      "frac": achieved / PEAK[args.precision], "traffic": traffic, "kernel": gkern,
                      "kernel_ms": gms, "algorithmic_flop": flop_grad},

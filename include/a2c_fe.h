@@ -36,7 +36,13 @@ extern "C" {
 
 enum a2c_deriv_mode { A2C_DERIV_ANALYTIC = 0, A2C_DERIV_FD = 1 };
 enum a2c_precision { A2C_PREC_FP64 = 0, A2C_PREC_MIXED = 1 };
-enum a2c_quad_rule { A2C_QUAD_R12 = 0 };
+enum a2c_quad_rule {
+  A2C_QUAD_R12 = 0, /* the reference's 12-point rule (quadrature.cpp:43-75), weight 1/12 */
+  A2C_QUAD_GH3 = 1  /* 3-point-per-axis Gauss-Hermite rule on the relative coordinate
+                       r = d + s z, s^2 = sigma_i^2 + sigma_j^2 (27 nodes; not in the
+                       reference: exact to degree 5 per axis where the 12-point rule
+                       is exact to degree 3) */
+};
 
 typedef struct a2c_fe_desc {
   int32_t n_pd, n_h;
@@ -45,7 +51,7 @@ typedef struct a2c_fe_desc {
   double rc;              /* neighbor cutoff, Angstrom (input.file.rc; 5.35) */
   double a_pd;            /* Pd lattice constant for shell classification (input.file.a_Pd) */
   double temperature;     /* T (input.file.T0), K */
-  int32_t quad_rule;      /* A2C_QUAD_R12: quadrature.cpp:43-75, weight 1/12 */
+  int32_t quad_rule;      /* A2C_QUAD_R12 (default) or A2C_QUAD_GH3 */
   int32_t deriv_mode;     /* A2C_DERIV_ANALYTIC or A2C_DERIV_FD (reference) */
   double fd_eps;          /* FD step for A2C_DERIV_FD (input.file.eps; 1e-5) */
   int32_t precision;      /* A2C_PREC_FP64 or A2C_PREC_MIXED */

@@ -73,6 +73,9 @@ def test_descriptor_validation_happens_before_device_use():
     d.temperature = 300.0
     d.max_neighbors = 100000
     assert A.lib.a2c_fe_create(C.byref(d), C.byref(ctx)) == A.A2C_EINVAL
+    d.max_neighbors = 256
+    d.quad_rule = 2  # only A2C_QUAD_R12 (0) and A2C_QUAD_GH3 (1) exist
+    assert A.lib.a2c_fe_create(C.byref(d), C.byref(ctx)) == A.A2C_EINVAL
 
 
 def test_mixed_tables_reproduce_the_closed_forms():
