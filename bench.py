@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--precision", choices=["mixed", "fp64"], default=os.environ.get("A2C_BENCH_PREC", "mixed"))
     p.add_argument("--quad", choices=["r12", "gh3"], default="r12",
                    help="quadrature rule of the single-lattice workload")
+    p.add_argument("--no-gh3", action="store_true", help="skip the Gauss-Hermite line beside the 12-point one")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--calib", action="store_true",
@@ -226,60 +227,17 @@ def main():
         return A.lattice(n, a=3.89, seed=2, disp_std=0.03, sigma=(0.08, 0.15), x=(0.0, 0.6), x_clip=(0.0, 1.0))
 
     X, x, box, npd, nh = mk(args.cells)
+    log("workload: %d cells^3, %d sites" % (args.cells, npd + nh))
+    m = measure(A, args, X, x, box, npd, nh, args.quad, args.steps, args.warmup, calib=args.calib)
     N = npd + nh
-    log("workload: %d cells^3, %d sites" % (args.cells, N))
-    fe = A.FreeEntropy(npd, nh, box, (1, 1, 1), T=500.0, precision="fp64", quad=args.quad)
-    fe.set_occupancy(x)
-    t0 = time.perf_counter()
-    st = fe.findNeighbors(X)
-    nbr_ms = (time.perf_counter() - t0) * 1e3
-    U = (st["total"][0] / 2.0, float(st["total"][1]), st["total"][3] / 2.0)  # unordered PdPd, PdH, HH
-    K = POINTS[args.quad]
-    flop_grad = K * sum(c * u for c, u in zip(FLOP_GRAD, U))
-    flop_dens = K * sum(c * u for c, u in zip(FLOP_DENS, U))
-    log("neighbors: %.1f ms, max %d, totals %s" % (nbr_ms, st["max_neib"], st["total"]))
-    if args.calib:
-        nb = 256 << 20
-        a, b = fe.dev_alloc(nb), fe.dev_alloc(nb)
-        fe.dev_copy(b, a, nb)
-        fe.dev_free(a)
-        fe.dev_free(b)
-
-    # FP64 comparator (always measured; the reference answer for the error)
-    tm64, wall64, f64, G64 = run_gpu(fe, X, max(3, args.steps // 10), 2)
-    ms64 = tm64["total_ms"] / max(3, args.steps // 10)
-    log("fp64: %.3f ms/eval %s" % (ms64, tm64["kernels_ms"]))
-
-    result = {}
-    if args.precision == "mixed":
-        fe.set_precision("mixed")
-        tm, wall, f, G = run_gpu(fe, X, args.steps, args.warmup)
-        # pair-once default: the owner kernel carries every pair term; full-list mode: grad32
-        gkern = "grad_own" if "grad_own" in tm["kernels_ms"] else "grad32"
-    else:
-        tm, wall, f, G = tm64, wall64, f64, G64
-        args.steps = max(3, args.steps // 10)
-        gkern = "grad_own64" if "grad_own64" in tm["kernels_ms"] else "grad64"
-    ms = tm["total_ms"] / args.steps
-    kern = tm["kernels_ms"]
-    gms = kern.get(gkern, float("nan"))
-    achieved = flop_grad / (gms * 1e-3) / 1e12
-    traffic = None  # HBM bytes per dispatch of gkern from the committed PMC summary (FETCH/WRITE_SIZE, scaled)
-    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
-    if os.path.exists(tpath):
-        tr = json.load(open(tpath))
-        if tr.get("kernel") == gkern and tr.get("quad", "r12") == args.quad:
-            traffic = tr["bytes_per_dispatch"]
-    obj_err = abs(f - f64) / abs(f64)
-    grad_err = float(np.linalg.norm(G - G64) / np.linalg.norm(G64))
     result = {
         "metric": "free-entropy+grad atom-evals/s",
-        "value": N * args.steps / (tm["total_ms"] * 1e-3),
+        "value": m["value"],
         "unit": "atom-evals/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": m["steps"],
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -289,20 +247,87 @@ def main():
                                "x~U(0,0.6), disp N(0,0.03), sigma U(0.08,0.15), T=500K, %s rule"
                                % {"r12": "R12", "gh3": "GH3"}[args.quad],
                    "cells": args.cells, "sites": N, "precision": args.precision, "quad": args.quad},
+        "roofline": m["roofline"],
+        "kernels_ms": m["kernels_ms"],
+        "fp64_gpu": m["fp64_gpu"],
+        "error_vs_fp64": m["error_vs_fp64"],
+        "neighbor_build_ms": m["neighbor_build_ms"],
+        "density_pass_flop": m["density_pass_flop"],
+        "wall_s_timed": m["wall_s_timed"],
+    }
+    if args.quad == "r12" and not args.no_gh3:
+        # the other rule on the same lattice, reported beside the headline
+        g = measure(A, args, X, x, box, npd, nh, "gh3", max(10, args.steps // 2), args.warmup)
+        result["gh3"] = {k: g[k] for k in ("value", "ms_per_step", "steps", "kernels_ms", "fp64_gpu",
+                                           "error_vs_fp64")}
+        result["gh3"]["roofline"] = g["roofline"]
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(args, mk)
+    print(json.dumps(result), flush=True)
+
+
+def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False):
+    """One rule on one lattice: the FP64 kernel (comparator), then the
+    measured precision; returns throughput, per-kernel times, roofline of
+    the gradient owner kernel and the error against the FP64 kernel."""
+    N = npd + nh
+    fe = A.FreeEntropy(npd, nh, box, (1, 1, 1), T=500.0, precision="fp64", quad=quad)
+    fe.set_occupancy(x)
+    t0 = time.perf_counter()
+    st = fe.findNeighbors(X)
+    nbr_ms = (time.perf_counter() - t0) * 1e3
+    U = (st["total"][0] / 2.0, float(st["total"][1]), st["total"][3] / 2.0)  # unordered PdPd, PdH, HH
+    K = POINTS[quad]
+    flop_grad = K * sum(c * u for c, u in zip(FLOP_GRAD, U))
+    flop_dens = K * sum(c * u for c, u in zip(FLOP_DENS, U))
+    log("%s neighbors: %.1f ms, max %d, totals %s" % (quad, nbr_ms, st["max_neib"], st["total"]))
+    if calib:
+        nb = 256 << 20
+        a, b = fe.dev_alloc(nb), fe.dev_alloc(nb)
+        fe.dev_copy(b, a, nb)
+        fe.dev_free(a)
+        fe.dev_free(b)
+
+    # FP64 comparator (always measured; the reference answer for the error)
+    s64 = max(3, steps // 10)
+    tm64, wall64, f64, G64 = run_gpu(fe, X, s64, 2)
+    ms64 = tm64["total_ms"] / s64
+    log("%s fp64: %.3f ms/eval %s" % (quad, ms64, tm64["kernels_ms"]))
+    if args.precision == "mixed":
+        fe.set_precision("mixed")
+        tm, wall, f, G = run_gpu(fe, X, steps, warmup)
+        # pair-once default: the owner kernel carries every pair term; full-list mode: grad32
+        gkern = "grad_own" if "grad_own" in tm["kernels_ms"] else "grad32"
+    else:
+        tm, wall, f, G = tm64, wall64, f64, G64
+        steps = s64
+        gkern = "grad_own64" if "grad_own64" in tm["kernels_ms"] else "grad64"
+    fe.close()
+    kern = tm["kernels_ms"]
+    gms = kern.get(gkern, float("nan"))
+    achieved = flop_grad / (gms * 1e-3) / 1e12
+    traffic = None  # HBM bytes per dispatch of gkern from the committed PMC summary (FETCH/WRITE_SIZE, scaled)
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("kernel") == gkern and tr.get("quad", "r12") == quad:
+            traffic = tr["bytes_per_dispatch"]
+    log("%s %s: %.3f ms/eval %s" % (quad, args.precision, tm["total_ms"] / steps, kern))
+    return {
+        "value": N * steps / (tm["total_ms"] * 1e-3),
+        "steps": steps,
+        "ms_per_step": tm["total_ms"] / steps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[args.precision], "unit": "TFLOP/s",
                      "frac": achieved / PEAK[args.precision], "traffic": traffic, "kernel": gkern,
                      "kernel_ms": gms, "algorithmic_flop": flop_grad},
         "kernels_ms": kern,
         "fp64_gpu": {"value": N / (ms64 * 1e-3), "ms_per_step": ms64, "kernels_ms": tm64["kernels_ms"]},
-        "error_vs_fp64": {"objective_rel": obj_err, "gradient_rel_l2": grad_err},
+        "error_vs_fp64": {"objective_rel": abs(f - f64) / abs(f64),
+                          "gradient_rel_l2": float(np.linalg.norm(G - G64) / np.linalg.norm(G64))},
         "neighbor_build_ms": nbr_ms,
         "density_pass_flop": flop_dens,
         "wall_s_timed": wall,
     }
-    fe.close()
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(args, mk)
-    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

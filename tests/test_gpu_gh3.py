@@ -136,7 +136,11 @@ def test_batch_with_both_rules_matches_single_lattices(precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_relaxation_gh3_is_stationary_for_the_oracle(precision):
-    X, x, box, per, npd, nh = specimen(3, seed=1, disp=0.02, sigma=(0.10, 0.10), x=(0.10, 0.10))
+    """The plain 6^3-cell lattice (1,728 sites, x = 0.10, sigma = 0.10 A,
+    displacements N(0, 0.02 A), T = 300 K) under the Gauss-Hermite rule: one
+    evaluation, then a relaxation whose end state the oracle sees as
+    stationary."""
+    X, x, box, per, npd, nh = specimen(6, seed=1, disp=0.02, sigma=(0.10, 0.10), x=(0.10, 0.10))
     fe = A.FreeEntropy(npd, nh, box, per, T=300.0, precision=precision, quad="gh3")
     fe.set_occupancy(x)
     fe.findNeighbors(X)
