@@ -87,6 +87,7 @@ class BatchDesc(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("precision", C.c_int32), ("deriv_mode", C.c_int32), ("fd_eps", C.c_double),
         ("max_lattices", C.c_int32), ("hbm_budget_bytes", C.c_size_t), ("max_neighbors", C.c_int32),
+        ("graphs", C.c_int32),
     ]
 
 
@@ -166,6 +167,7 @@ def _load():
         "a2c_batch_last_error": (C.c_char_p, [VP]),
         "a2c_batch_footprint": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
         "a2c_batch_mem_used": (C.c_size_t, [VP]),
+        "a2c_batch_graph_stats": (I, [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "a2c_batch_add": (I, [VP, P(FeDesc), dp, dp, P(C.c_int32)]),
         "a2c_batch_remove": (I, [VP, C.c_int32]),
         "a2c_batch_set_state": (I, [VP, C.c_int32, dp]),
@@ -513,10 +515,12 @@ class Batch:
     lockstep LMVM with admission/retirement between iterations."""
 
     def __init__(self, precision="mixed", max_lattices=64, hbm_budget_bytes=200 << 30, device=0,
-                 deriv="analytic", eps=1e-5):
+                 deriv="analytic", eps=1e-5, graphs=None):
         d = BatchDesc()
         lib.a2c_batch_desc_default(C.byref(d))
         d.device, d.precision, d.deriv_mode, d.fd_eps = device, PREC[precision], DERIV[deriv], eps
+        if graphs is not None:
+            d.graphs = int(bool(graphs))
         d.max_lattices, d.hbm_budget_bytes = max_lattices, hbm_budget_bytes
         self._b = C.c_void_p()
         rc = lib.a2c_batch_create(C.byref(d), C.byref(self._b))
@@ -552,6 +556,12 @@ class Batch:
 
     def mem_used(self):
         return lib.a2c_batch_mem_used(self._b)
+
+    def graph_stats(self):
+        """Launch sequences so far: {"replays", "captures", "direct"}."""
+        v = [C.c_int64() for _ in range(3)]
+        self._check(lib.a2c_batch_graph_stats(self._b, *[C.byref(x) for x in v]))
+        return dict(zip(("replays", "captures", "direct"), (x.value for x in v)))
 
     def add(self, X, x, n_pd, n_h, box=(0, 0, 0), periodic=(0, 0, 0), T=300.0, rc=5.35, a_pd=3.89, quad="r12"):
         d = fe_desc(n_pd, n_h, box, periodic, rc, a_pd, T, quad=quad)

@@ -17,14 +17,29 @@
 // that need it (segmented kernels), and each scalar decision is taken per slot
 // on the host after one synchronisation per round.  Slots can be added and
 // started between iterations (continuous batching).
+//
+// Launch sequences.  Every operation between two host synchronisations is
+// queued with a key made of its kernel and all its argument values; the
+// descriptors those kernels read are staged in one pinned arena, handed out
+// in the same order every time, so a steady-state optimizer iteration repeats
+// its keys exactly.  With a2c_batch_desc.graphs set, a sequence seen for the
+// second time is captured once into a hipGraph and from then on replayed: the
+// descriptors are copied, then the whole sequence (kernels and result copies)
+// is one graph launch.  Graph and direct issue run the same kernels in the
+// same order with the same arguments, so results are bitwise identical either
+// way.  Replay is off by default: on this runtime a replayed iteration of a
+// small lattice takes as long as direct issue (the per-kernel cost is on the
+// device side), and capturing new sequences costs time in a changing batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
-#include <functional>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "a2c_batch.h"
@@ -220,15 +235,16 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
 // ---------------------------------------------------------------- host side
 // Pinned staging for per-launch descriptor arrays.  Space is handed out
 // bump-style and reclaimed only after the stream has been synchronised.
-// Launches that read staged descriptors are queued (q) and issued by flush()
-// after one copy of everything staged since the last flush: an optimizer
-// iteration is ~50 small launches, and one host-to-device copy per launch
-// would cost more than the kernels of a small lattice.
+// Operations are queued (q, with their key) and issued at the next
+// synchronisation after one copy of everything staged: an optimizer iteration
+// is ~50 small launches, and one host-to-device copy per launch would cost
+// more than the kernels of a small lattice.
 struct Arena {
   char* h = nullptr;
   char* d = nullptr;
-  size_t cap = 0, used = 0, copied = 0;
+  size_t cap = 0, used = 0;
   std::vector<std::function<void(hipStream_t)>> q;
+  std::vector<uint64_t> key;  // kind and argument values of every queued operation, in order
   template <class T>
   T* push(const T* src, size_t n) {
     const size_t bytes = ((n * sizeof(T)) + 255) & ~(size_t)255;
@@ -266,12 +282,30 @@ struct Slot {
 
 }  // namespace a2c
 
+namespace a2c {
+// Instantiated launch sequences, by key.
+struct GraphCache {
+  struct Entry {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last = 0;  // use stamp (least recently used is evicted)
+  };
+  static constexpr size_t CAP = 64;
+  std::unordered_multimap<uint64_t, Entry> map;
+  std::unordered_map<uint64_t, int> seen;  // keys issued directly so far (hash -> count)
+  uint64_t clock = 0;
+  int64_t replays = 0, captures = 0, direct = 0;
+};
+}  // namespace a2c
+
 struct a2c_batch {
   a2c_batch_desc desc{};
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<a2c::Slot> slots;
   a2c::Arena arena;
+  a2c::GraphCache graphs;
+  int qrc = 0;  // first error of a queued operation
   double* dF = nullptr;    // objective per slot (device)
   double* hF = nullptr;    // pinned mirror
   double* dres = nullptr;  // 2 doubles per slot (device dot results read by the host)
@@ -296,23 +330,98 @@ int berr(a2c_batch* b, int code, const std::string& m) {
     if (e_ != hipSuccess) return berr((b), A2C_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// copy the staged descriptors, then issue the queued launches in order
-int flush(a2c_batch* b) {
+inline uint64_t u64(const void* p) { return (uint64_t)(uintptr_t)p; }
+inline uint64_t u64(double v) {
+  uint64_t r;
+  std::memcpy(&r, &v, sizeof r);
+  return r;
+}
+inline uint64_t u64(long long v) { return (uint64_t)v; }
+
+// Queue one operation; key holds its kind and every argument value it passes.
+void enqueue(a2c_batch* b, std::initializer_list<uint64_t> key, std::function<void(hipStream_t)> fn) {
+  b->arena.key.insert(b->arena.key.end(), key);
+  b->arena.q.push_back(std::move(fn));
+}
+void enqueue_d2h(a2c_batch* b, void* dst, const void* src, size_t bytes) {
+  enqueue(b, {7, u64(dst), u64(src), (uint64_t)bytes}, [=](hipStream_t st) {
+    (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+  });
+}
+
+uint64_t key_hash(const std::vector<uint64_t>& k) {  // FNV-1a over the words
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t w : k) h = (h ^ w) * 1099511628211ull;
+  return h;
+}
+
+// Issue the queued sequence: replay its graph, capture it (second sighting
+// of its key), or issue it directly.
+int issue(a2c_batch* b) {
   Arena& a = b->arena;
-  if (a.used > a.copied) {
-    B_HIP(b, hipMemcpyAsync(a.d + a.copied, a.h + a.copied, a.used - a.copied, hipMemcpyHostToDevice, b->stream));
-    a.copied = a.used;
+  if (a.q.empty()) return A2C_OK;
+  if (a.used) B_HIP(b, hipMemcpyAsync(a.d, a.h, a.used, hipMemcpyHostToDevice, b->stream));
+  GraphCache& gc = b->graphs;
+  hipGraphExec_t exec = nullptr;
+  if (b->desc.graphs) {
+    a.key.push_back(a.used);
+    const uint64_t h = key_hash(a.key);
+    auto range = gc.map.equal_range(h);
+    for (auto it = range.first; it != range.second; ++it)
+      if (it->second.key == a.key) {
+        exec = it->second.exec;
+        it->second.last = ++gc.clock;
+        gc.replays++;
+        break;
+      }
+    if (!exec && ++gc.seen[h] >= 2) {
+      if (gc.map.size() >= GraphCache::CAP) {  // evict the least recently used
+        auto old = gc.map.begin();
+        for (auto it = gc.map.begin(); it != gc.map.end(); ++it)
+          if (it->second.last < old->second.last) old = it;
+        (void)hipGraphExecDestroy(old->second.exec);
+        gc.map.erase(old);
+      }
+      hipGraph_t g = nullptr;
+      B_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      b->qrc = 0;
+      for (auto& f : a.q) f(b->stream);
+      const hipError_t le = hipGetLastError();
+      const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+      if (le != hipSuccess || ce != hipSuccess || b->qrc) {
+        if (g) (void)hipGraphDestroy(g);
+        return berr(b, b->qrc ? b->qrc : A2C_EHIP, std::string("launch sequence capture failed: ") +
+                                                       hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      B_HIP(b, ie);
+      GraphCache::Entry e;
+      e.key = a.key;
+      e.exec = exec;
+      e.last = ++gc.clock;
+      gc.map.emplace(h, std::move(e));
+      gc.captures++;
+    }
   }
-  for (auto& f : a.q) f(b->stream);
+  if (exec) {
+    B_HIP(b, hipGraphLaunch(exec, b->stream));
+  } else {
+    b->qrc = 0;
+    for (auto& f : a.q) f(b->stream);
+    B_HIP(b, hipGetLastError());
+    if (b->qrc) return berr(b, b->qrc, "fused eval launch failed");
+    gc.direct++;
+  }
   a.q.clear();
-  B_HIP(b, hipGetLastError());
+  a.key.clear();
   return A2C_OK;
 }
 
 int bsync(a2c_batch* b) {
-  if (int rc = flush(b)) return rc;
+  if (int rc = issue(b)) return rc;
   B_HIP(b, hipStreamSynchronize(b->stream));
-  b->arena.used = b->arena.copied = 0;
+  b->arena.used = 0;
   return A2C_OK;
 }
 
@@ -368,8 +477,21 @@ int seg_axpby(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   int nblk;
   if (int rc = seg_launch_prep<SegAx>(b, ids, fill0, &ops, &map, &nblk)) return rc;
   if (nblk)
-    b->arena.q.emplace_back([=](hipStream_t st) { hipLaunchKernelGGL(k_seg_axpby, dim3(nblk), dim3(TB), 0, st, ops, map); });
+    enqueue(b, {1, u64(ops), u64(map), (uint64_t)nblk},
+            [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_axpby, dim3(nblk), dim3(TB), 0, st, ops, map); });
   return A2C_OK;
+}
+
+// block partials, then one fixed-order sum per op
+void enqueue_dot(a2c_batch* b, const SegDot* ops, const int32_t* map, int nblk, unsigned nops) {
+  double* part = b->dpart;
+  if (nblk)
+    enqueue(b, {2, u64(ops), u64(map), u64(part), (uint64_t)nblk}, [=](hipStream_t st) {
+      hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk);
+    });
+  enqueue(b, {3, u64(ops), u64(part), (uint64_t)nblk, nops}, [=](hipStream_t st) {
+    hipLaunchKernelGGL(k_seg_dot_final, dim3(nops), dim3(1024), 0, st, ops, part, nblk);
+  });
 }
 
 // dot products per slot; results (2 per slot) returned in out[k*2 + 0/1]
@@ -388,15 +510,8 @@ int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<do
   int32_t* map;
   int nblk;
   if (int rc = seg_launch_prep<SegDot>(b, ids, fill2, &ops, &map, &nblk)) return rc;
-  double* part = b->dpart;
-  const unsigned nops = (unsigned)ids.size();
-  if (nblk)
-    b->arena.q.emplace_back(
-        [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk); });
-  b->arena.q.emplace_back(
-      [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot_final, dim3(nops), dim3(1024), 0, st, ops, part, nblk); });
-  if (int rc = flush(b)) return rc;
-  B_HIP(b, hipMemcpyAsync(b->hres, b->dres, sizeof(double) * 2 * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
+  enqueue_dot(b, ops, map, nblk, (unsigned)ids.size());
+  enqueue_d2h(b, b->hres, b->dres, sizeof(double) * 2 * b->slots.size());
   if (int rc = bsync(b)) return rc;
   for (size_t k = 0; k < ids.size(); ++k) {
     out[2 * k] = b->hres[2 * ids[k]];
@@ -419,13 +534,7 @@ int seg_dot_dev(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   int32_t* map;
   int nblk;
   if (int rc = seg_launch_prep<SegDot>(b, ids, fill2, &ops, &map, &nblk)) return rc;
-  double* part = b->dpart;
-  const unsigned nops = (unsigned)ids.size();
-  if (nblk)
-    b->arena.q.emplace_back(
-        [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk); });
-  b->arena.q.emplace_back(
-      [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_dot_final, dim3(nops), dim3(1024), 0, st, ops, part, nblk); });
+  enqueue_dot(b, ops, map, nblk, (unsigned)ids.size());
   return A2C_OK;
 }
 
@@ -437,8 +546,8 @@ int seg_bound(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   int nblk;
   if (int rc = seg_launch_prep<SegBound>(b, ids, fill, &ops, &map, &nblk)) return rc;
   if (nblk)
-    b->arena.q.emplace_back(
-        [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_bound<MODE>, dim3(nblk), dim3(TB), 0, st, ops, map); });
+    enqueue(b, {4, (uint64_t)MODE, u64(ops), u64(map), (uint64_t)nblk},
+            [=](hipStream_t st) { hipLaunchKernelGGL(k_seg_bound<MODE>, dim3(nblk), dim3(TB), 0, st, ops, map); });
   return A2C_OK;
 }
 
@@ -481,7 +590,6 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   int32_t* dm = map.empty() ? nullptr : b->arena.push(map.data(), map.size());
   int32_t* dom = omap.empty() ? nullptr : b->arena.push(omap.data(), omap.size());
   if (!dl || (!map.empty() && (!dm || !dom))) return berr(b, A2C_ENOMEM, "staging arena too small");
-  if (int rc = flush(b)) return rc;  // the evaluation kernels are launched directly
   Launch L;
   L.lats = dl;
   L.b2l = dm;
@@ -496,10 +604,17 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   L.tab = b->dtab;
   L.mixed_full = mixed_full_pairs();
   L.full_pairs64 = fp64_full_pairs();
-  int rc = fp64 ? run_eval64(L, nullptr) : run_eval32(L, nullptr);
-  if (rc) return berr(b, rc, "fused eval launch failed");
+  enqueue(b, {5, (uint64_t)fp64, u64(L.lats), u64(L.b2l), u64(L.ob2l), (uint64_t)L.nblk, (uint64_t)L.onblk,
+              (uint64_t)L.nlat, (uint64_t)L.rules, (uint64_t)L.deriv_mode, u64(L.fd_eps), u64(L.tab),
+              (uint64_t)L.mixed_full, (uint64_t)L.full_pairs64},
+          [=](hipStream_t st) {
+            Launch l = L;
+            l.stream = st;
+            const int rc = fp64 ? run_eval64(l, nullptr) : run_eval32(l, nullptr);
+            if (rc && !b->qrc) b->qrc = rc;
+          });
   if (!sync) return A2C_OK;
-  B_HIP(b, hipMemcpyAsync(b->hF, b->dF, sizeof(double) * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
+  enqueue_d2h(b, b->hF, b->dF, sizeof(double) * b->slots.size());
   return bsync(b);
 }
 
@@ -554,7 +669,7 @@ int eval_slots(a2c_batch* b, const std::vector<int>& ids, bool sync = true) {
   }
   if (int rc = fused_eval(b, m32, false, lmvm_state, false)) return rc;
   if (int rc = fused_eval(b, m64, true, lmvm_state, false)) return rc;
-  B_HIP(b, hipMemcpyAsync(b->hF, b->dF, sizeof(double) * b->slots.size(), hipMemcpyDeviceToHost, b->stream));
+  enqueue_d2h(b, b->hF, b->dF, sizeof(double) * b->slots.size());
   if (!sync) return A2C_OK;
   if (int rc = bsync(b)) return rc;
   for (int i : ids) b->slots[i].f = b->hF[i];
@@ -888,11 +1003,14 @@ void a2c_batch_desc_default(a2c_batch_desc* d) {
   d->max_lattices = 64;
   d->hbm_budget_bytes = (size_t)200 << 30;
   d->max_neighbors = NBR_CAP;
+  const char* g = std::getenv("A2C_GRAPHS");
+  d->graphs = (g && std::string(g) == "1") ? 1 : 0;
 }
 
 int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
   if (!desc || !out || desc->max_lattices <= 0 || desc->max_lattices > (1 << 20)) return A2C_EINVAL;
   if (desc->precision != A2C_PREC_FP64 && desc->precision != A2C_PREC_MIXED) return A2C_EINVAL;
+  if (desc->graphs != 0 && desc->graphs != 1) return A2C_EINVAL;
   *out = nullptr;
   a2c_batch* b = new (std::nothrow) a2c_batch;
   if (!b) return A2C_ENOMEM;
@@ -919,6 +1037,7 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
 void a2c_batch_destroy(a2c_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& e : b->graphs.map) (void)hipGraphExecDestroy(e.second.exec);
   for (auto& s : b->slots) {
     lmvm_free(s);
     if (s.owned) a2c_fe_destroy(s.fe);
@@ -949,6 +1068,14 @@ size_t a2c_batch_footprint(int32_t n_pd, int32_t n_h, int32_t lmvm_m) {
 }
 
 size_t a2c_batch_mem_used(const a2c_batch* b) { return b ? b->mem_used : 0; }
+
+int a2c_batch_graph_stats(const a2c_batch* b, int64_t* replays, int64_t* captures, int64_t* direct) {
+  if (!b) return A2C_EINVAL;
+  if (replays) *replays = b->graphs.replays;
+  if (captures) *captures = b->graphs.captures;
+  if (direct) *direct = b->graphs.direct;
+  return A2C_OK;
+}
 
 int a2c_batch_add(a2c_batch* b, const a2c_fe_desc* lat, const double* X, const double* x, int32_t* slot) {
   if (!b) return A2C_EINVAL;

@@ -29,6 +29,10 @@ typedef struct a2c_batch_desc {
   int32_t max_lattices;     /* slot capacity */
   size_t hbm_budget_bytes;  /* admission limit (sum of a2c_batch_footprint) */
   int32_t max_neighbors;
+  int32_t graphs;           /* 1: a launch sequence that recurs between host synchronisations is
+                               captured into a hipGraph and replayed; 0 (default): every launch is
+                               issued directly.  Results are bitwise identical either way.  The
+                               environment variable A2C_GRAPHS=1 makes 1 the default. */
 } a2c_batch_desc;
 
 void a2c_batch_desc_default(a2c_batch_desc* d);
@@ -40,6 +44,10 @@ const char* a2c_batch_last_error(const a2c_batch* b);
  * with m pairs); admission compares the sum against hbm_budget_bytes. */
 size_t a2c_batch_footprint(int32_t n_pd, int32_t n_h, int32_t lmvm_m);
 size_t a2c_batch_mem_used(const a2c_batch* b);
+
+/* Launch sequences issued so far: replayed from a captured hipGraph,
+ * captured (and run), issued directly.  Any pointer may be null. */
+int a2c_batch_graph_stats(const a2c_batch* b, int64_t* replays, int64_t* captures, int64_t* direct);
 
 /* Admit one lattice: lat describes it like a single context (its precision,
  * derivative mode and device are the batch's); X (4N) and x (n_h) are its

@@ -107,3 +107,15 @@ def test_serving_request_stream_is_seeded():
     assert all(6 <= r["n"] <= 48 for r in a) and all(300 <= r["T"] <= 700 for r in a)
     t = [r["arrival"] for r in a]
     assert t == sorted(t) and 1.0 < 20 / t[-1] < 20.0
+
+
+def test_batch_descriptor_graph_flag(monkeypatch):
+    d = A.BatchDesc()
+    A.lib.a2c_batch_desc_default(C.byref(d))
+    assert d.graphs == 0
+    monkeypatch.setenv("A2C_GRAPHS", "1")
+    A.lib.a2c_batch_desc_default(C.byref(d))
+    assert d.graphs == 1
+    d.graphs = 2  # only 0 and 1; rejected before any device use
+    b = C.c_void_p()
+    assert A.lib.a2c_batch_create(C.byref(d), C.byref(b)) == A.A2C_EINVAL

@@ -93,3 +93,60 @@ def test_admission_by_hbm_budget():
         B.remove(a)
         assert B.mem_used() == 0
         B.add(**s)
+
+
+# ---------------------------------------------------------------- hipGraph replay
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_matches_direct_issue_bitwise(precision):
+    """Recurring launch sequences replayed from captured hipGraphs give the
+    same relaxations, bit for bit, as direct issue of every launch."""
+    specs = [make(s) for s in LATS[:3]]
+    opts = dict(gatol=1e-9, grtol=0.0, max_it=3000)
+    out = {}
+    for graphs in (False, True):
+        with A.Batch(precision=precision, graphs=graphs) as B:
+            slots = [B.add(**s) for s in specs]
+            for sl in slots:
+                B.start(sl, **opts)
+            B.run()
+            out[graphs] = ([B.report(sl) for sl in slots], [B.get_state(sl)[0] for sl in slots], B.graph_stats())
+    (r0, X0, st0), (r1, X1, st1) = out[False], out[True]
+    assert st0["replays"] == 0 and st0["captures"] == 0
+    assert st1["replays"] > 10 * st1["captures"] > 0, st1  # steady state runs from graphs
+    for a, b, xa, xb in zip(r0, r1, X0, X1):
+        assert a == b and a["reason"] > 0
+        assert np.array_equal(xa, xb)
+
+
+def test_graph_replays_of_one_evaluation_with_direct_work_between():
+    """The same fused evaluation replayed several times, with other launches
+    and synchronisations between the replays: every replay equals the
+    directly issued result."""
+    specs = [make(s) for s in LATS]
+    with A.Batch(precision="mixed", graphs=False) as B0:
+        s0 = [B0.add(**s) for s in specs]
+        f_ref = B0.eval(s0)
+        G_ref = [B0.get_state(sl)[1] for sl in s0]
+    with A.Batch(precision="mixed", graphs=True) as B:
+        slots = [B.add(**s) for s in specs]
+        for rep in range(5):
+            f = B.eval(slots)
+            assert np.array_equal(f, f_ref), rep
+            for sl, G in zip(slots, G_ref):
+                assert np.array_equal(B.get_state(sl)[1], G), rep
+            B.eval(slots[:2], precision="fp64")  # a different sequence between the replays
+        st = B.graph_stats()
+        assert st["captures"] >= 1 and st["replays"] >= 3, st
+
+
+def test_single_lattice_minimizer_same_with_and_without_graphs(monkeypatch):
+    s = make(LATS[1])
+    res = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("A2C_GRAPHS", env)
+        with A.FreeEntropy(s["n_pd"], s["n_h"], s["box"], s["periodic"], T=s["T"], precision="mixed") as fe:
+            fe.set_occupancy(s["x"])
+            fe.findNeighbors(s["X"])
+            res.append(fe.minimizeFreeEntropy(s["X"], gatol=1e-9, grtol=0.0))
+    (Xa, ra), (Xb, rb) = res
+    assert ra == rb and np.array_equal(Xa, Xb)
