@@ -91,6 +91,24 @@ class BatchDesc(C.Structure):
     ]
 
 
+TUNE_MAX = 8
+
+
+class TuneTable(C.Structure):
+    _fields_ = [("n_classes", C.c_int32), ("max_sites", C.c_int32 * TUNE_MAX),
+                ("lanes_mixed", C.c_int32 * TUNE_MAX), ("lanes_fp64", C.c_int32 * TUNE_MAX)]
+
+
+class AutotuneOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("n_sizes", C.c_int32), ("cells", C.c_int32 * TUNE_MAX),
+                ("target_sites", C.c_int64), ("reps", C.c_int32), ("apply", C.c_int32)]
+
+
+class AutotuneReport(C.Structure):
+    _fields_ = [("sites", C.c_int32 * TUNE_MAX), ("copies", C.c_int32 * TUNE_MAX),
+                ("ns_per_site", C.c_double * 4 * 2 * TUNE_MAX)]
+
+
 class DmdParams(C.Structure):
     _fields_ = [
         ("dt", C.c_double), ("nu", C.c_double), ("Qm", C.c_double), ("xLb", C.c_double), ("xUb", C.c_double),
@@ -176,6 +194,10 @@ def _load():
         "a2c_batch_lmvm_start": (I, [VP, C.c_int32, P(LmvmOpts)]),
         "a2c_batch_lmvm_step": (I, [VP, C.c_int32, P(C.c_int32)]),
         "a2c_batch_lmvm_report": (I, [VP, C.c_int32, P(LmvmReport)]),
+        "a2c_tune_get": (None, [P(TuneTable)]),
+        "a2c_tune_set": (I, [P(TuneTable)]),
+        "a2c_autotune_opts_default": (None, [P(AutotuneOpts)]),
+        "a2c_autotune": (I, [P(AutotuneOpts), P(TuneTable), P(AutotuneReport)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -622,12 +644,73 @@ class Batch:
                 return
 
 
+# --------------------------------------------------------------------------
+# launch-shape table (include/a2c_tune.h)
+# --------------------------------------------------------------------------
+def _table_dict(t):
+    n = t.n_classes
+    return {"max_sites": list(t.max_sites[:n]), "lanes_mixed": list(t.lanes_mixed[:n]),
+            "lanes_fp64": list(t.lanes_fp64[:n])}
+
+
+def tune_get():
+    """The lanes-per-site table in effect: {"max_sites", "lanes_mixed", "lanes_fp64"}."""
+    t = TuneTable()
+    lib.a2c_tune_get(C.byref(t))
+    return _table_dict(t)
+
+
+def tune_set(table=None):
+    """Install a table (dict as returned by tune_get), or the default when None."""
+    if table is None:
+        rc = lib.a2c_tune_set(None)
+    else:
+        t = TuneTable()
+        n = len(table["max_sites"])
+        if not 1 <= n <= TUNE_MAX or len(table["lanes_mixed"]) != n or len(table["lanes_fp64"]) != n:
+            raise A2CError(A2C_EINVAL, "malformed tune table")
+        t.n_classes = n
+        for c in range(n):
+            t.max_sites[c] = min(int(table["max_sites"][c]), 2**31 - 1)
+            t.lanes_mixed[c] = int(table["lanes_mixed"][c])
+            t.lanes_fp64[c] = int(table["lanes_fp64"][c])
+        rc = lib.a2c_tune_set(C.byref(t))
+    if rc != 0:
+        raise A2CError(rc, "a2c_tune_set rejected the table")
+
+
+def autotune(cells=None, target_sites=None, reps=None, apply=False, device=0):
+    """Measure lanes per site for each lattice size (include/a2c_tune.h).
+    Returns (table, report) where report lists, per size, sites, copies and
+    ns per site for K = 1, 2, 4, 8 in mixed and FP64 precision."""
+    o = AutotuneOpts()
+    lib.a2c_autotune_opts_default(C.byref(o))
+    o.device, o.apply = device, int(bool(apply))
+    if cells is not None:
+        o.n_sizes = len(cells)
+        for i, c in enumerate(cells):
+            o.cells[i] = int(c)
+    if target_sites is not None:
+        o.target_sites = int(target_sites)
+    if reps is not None:
+        o.reps = int(reps)
+    t, r = TuneTable(), AutotuneReport()
+    rc = lib.a2c_autotune(C.byref(o), C.byref(t), C.byref(r))
+    if rc != 0:
+        raise A2CError(rc, "a2c_autotune failed")
+    rows = []
+    for i in range(o.n_sizes):
+        rows.append({"cells": o.cells[i], "sites": r.sites[i], "copies": r.copies[i],
+                     "ns_per_site": {"mixed": list(r.ns_per_site[i][0]), "fp64": list(r.ns_per_site[i][1])}})
+    return _table_dict(t), rows
+
+
 def exported_symbols():
     """Symbols include/*.h declares (checked by the CPU test suite)."""
     import re
     root = os.path.dirname(_HERE)
     names = []
-    for h in ("a2c_fe.h", "a2c_lattice.h", "a2c_batch.h", "a2c_dmd.h"):
+    for h in ("a2c_fe.h", "a2c_lattice.h", "a2c_batch.h", "a2c_dmd.h", "a2c_tune.h"):
         txt = open(os.path.join(root, "include", h)).read()
         names += re.findall(r"\b(a2c_\w+)\s*\(", txt)
     return sorted(set(names))

@@ -615,12 +615,26 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const Ell& P = partner_of(S, list);
     const int n = L.ocnt[row];
+    int jl_n = 0;  // the next pair's neighbor, state and F' (loaded one pair ahead)
+    Site b_n{};
+    float Fj_n = 0.0f;
+    if (ph < n) {
+      jl_n = L.oj[(size_t)ph * L.rows + row];
+      const int jn = side == 0 ? jl_n : S.npd + jl_n;
+      b_n = load_site(X, jn);
+      Fj_n = aux[jn].x;
+    }
     for (int t = ph; t < n; t += K) {
       const size_t ei = (size_t)t * L.rows + row;
-      const int jl = L.oj[ei];
-      const int j = side == 0 ? jl : S.npd + jl;
-      const Site b = load_site(X, j);
-      const float Fj = aux[j].x;
+      const int jl = jl_n;
+      const Site b = b_n;
+      const float Fj = Fj_n;
+      if (t + K < n) {  // the next pair's gathers overlap this pair's points
+        jl_n = L.oj[(size_t)(t + K) * L.rows + row];
+        const int jn = side == 0 ? jl_n : S.npd + jl_n;
+        b_n = load_site(X, jn);
+        Fj_n = aux[jn].x;
+      }
       float d[3];
       pair_vec32(S, a, b, d);
       const float aj = SQ6F * (float)b.s;

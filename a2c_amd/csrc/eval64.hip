@@ -714,12 +714,13 @@ int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool 
 // gradient or objective buffers (or the neighbor lists) changed.
 int sync_lattice(a2c_fe_ctx* c, const double* dX, double* dG, double* dF) {
   LatDev& h = *c->hlat;
-  if (!c->lat_dirty && h.X == dX && h.G == dG && h.f == dF) return A2C_OK;
+  const int kl32 = lanes_per_site(c->N, false), kl64 = lanes_per_site(c->N, true);
+  if (!c->lat_dirty && h.X == dX && h.G == dG && h.f == dF && h.kl32 == kl32 && h.kl64 == kl64) return A2C_OK;
   A2C_HIP(c, hipStreamSynchronize(c->stream));  // the pinned copy may still be in flight
   h = c->sys();
   h.blk0 = 0;
-  h.kl32 = lanes_per_site(c->N, false);
-  h.kl64 = lanes_per_site(c->N, true);
+  h.kl32 = kl32;  // the launch grid (single_launch) takes K from the same table
+  h.kl64 = kl64;
   h.owg0_32 = h.owg0_64 = 0;
   h.X = dX;
   h.G = dG;

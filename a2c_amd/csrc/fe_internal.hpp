@@ -74,21 +74,15 @@ using DevSys = LatDev;
 constexpr int SITE_BLOCK = 256;  // sites per block of every site-parallel kernel
 __host__ __device__ inline int site_blocks(int n) { return (n + SITE_BLOCK - 1) / SITE_BLOCK; }
 
-// Logical site block of a workgroup.  Workgroups are dealt to the 8 XCDs
-// round-robin by index; this bijection gives XCD k one contiguous range of
-// site blocks, so the neighbor data its waves gather stays within its own L2
-// instead of every XCD touching the whole lattice.
-// Lanes per site in the pair-owner passes.  A small lattice has too few
-// sites to fill the GPU with one lane per site, and each lane would walk its
-// ~40 owned pairs x 12 points serially; splitting a site's pairs over 8 lanes
-// (combined in a fixed shuffle order) cuts that latency 8x.  The choice
-// depends only on the lattice (and the precision), so its results are the
-// same in any batch.  Only lattices under 16,384 sites are split: larger
-// ones already fill the GPU in a batch, where one lane per site keeps the
-// neighbor-list loads coalesced (splitting up to 65,536 sites measured 13%
-// fewer batched relaxations/s), and the mixed passes stage 40 KB of tables
-// per workgroup.
-inline int lanes_per_site(int n_sites, bool fp64) { return (void)fp64, n_sites < 16384 ? 8 : 1; }
+// Lanes per site of the pair-owner passes.  A small lattice gives too few
+// waves to fill the GPU with one lane per site, and each lane would walk its
+// ~40 owned pairs x 12 points serially; splitting a site's pairs over K lanes
+// (combined in a fixed shuffle order) cuts that latency K-fold.  Large
+// lattices already fill the GPU in a batch, where one lane per site keeps the
+// neighbor-list loads coalesced.  K comes from the process-wide table of
+// a2c_tune.h (lattice size class and precision), so it depends only on the
+// lattice: its results are the same in any batch.
+int lanes_per_site(int n_sites, bool fp64);
 constexpr int MAX_LANES_PER_SITE = 8;
 
 // Fixed-order sum over the K lanes of a site (K a power of two <= 64, lanes
@@ -115,6 +109,10 @@ __device__ inline void gh_node(int k, double z[3], double& w) {
   w = gh_w(a0) * gh_w(a1) * gh_w(a2);
 }
 
+// Logical site block of a workgroup.  Workgroups are dealt to the 8 XCDs
+// round-robin by index; this bijection gives XCD k one contiguous range of
+// site blocks, so the neighbor data its waves gather stays within its own L2
+// instead of every XCD touching the whole lattice.
 constexpr int NUM_XCD = 8;
 __device__ inline int xcd_block() {
   const int b = blockIdx.x, n = gridDim.x;

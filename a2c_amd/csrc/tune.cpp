@@ -1,0 +1,185 @@
+// Launch-shape table of the pair-owner passes (lanes per site by lattice
+// size and precision) and the autotuner that measures it.
+//
+// The built-in table is the outcome of a2c_autotune on one MI355X with the
+// default options (fused batched evaluations of 6^3..40^3-cell lattices
+// filling about one million sites per launch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "a2c_batch.h"
+#include "a2c_lattice.h"
+#include "a2c_tune.h"
+#include "fe_internal.hpp"
+
+namespace a2c {
+namespace {
+
+a2c_tune_table default_table() {
+  // a2c_autotune with the default options on one MI355X (ns per site of a
+  // fused evaluation, K = 1/2/4/8: 40^3 cells mixed 1.81/1.66/1.65/1.83, FP64
+  // 6.38/6.27/6.20/6.68; 6^3 cells x 256 mixed 2.06/1.76/1.69/1.88)
+  static const int32_t max_sites[] = {2660, 6604, 15288, 32003, 71831, 170267, 366357, INT_MAX};
+  static const int32_t lanes_mixed[] = {4, 4, 4, 4, 4, 4, 4, 4};
+  static const int32_t lanes_fp64[] = {4, 4, 4, 2, 4, 4, 4, 4};
+  a2c_tune_table t;
+  std::memset(&t, 0, sizeof t);
+  t.n_classes = 8;
+  for (int c = 0; c < t.n_classes; ++c) {
+    t.max_sites[c] = max_sites[c];
+    t.lanes_mixed[c] = lanes_mixed[c];
+    t.lanes_fp64[c] = lanes_fp64[c];
+  }
+  return t;
+}
+
+a2c_tune_table g_table = default_table();
+
+bool valid_lanes(int k) { return k == 1 || k == 2 || k == 4 || k == 8; }
+static_assert(MAX_LANES_PER_SITE == 8, "the table allows up to 8 lanes per site");
+
+bool valid(const a2c_tune_table& t) {
+  if (t.n_classes < 1 || t.n_classes > A2C_TUNE_MAX_CLASSES) return false;
+  for (int c = 0; c < t.n_classes; ++c) {
+    if (!valid_lanes(t.lanes_mixed[c]) || !valid_lanes(t.lanes_fp64[c])) return false;
+    if (t.max_sites[c] < 1 || (c > 0 && t.max_sites[c] <= t.max_sites[c - 1])) return false;
+  }
+  return true;
+}
+
+// a table with one class: K lanes for every lattice in both precisions
+a2c_tune_table uniform(int k) {
+  a2c_tune_table t;
+  std::memset(&t, 0, sizeof t);
+  t.n_classes = 1;
+  t.max_sites[0] = INT_MAX;
+  t.lanes_mixed[0] = t.lanes_fp64[0] = k;
+  return t;
+}
+
+}  // namespace
+
+int lanes_per_site(int n_sites, bool fp64) {
+  const a2c_tune_table& t = g_table;
+  int c = 0;
+  while (c + 1 < t.n_classes && n_sites > t.max_sites[c]) ++c;
+  return fp64 ? t.lanes_fp64[c] : t.lanes_mixed[c];
+}
+
+}  // namespace a2c
+
+extern "C" {
+
+void a2c_tune_get(a2c_tune_table* out) {
+  if (out) *out = a2c::g_table;
+}
+
+int a2c_tune_set(const a2c_tune_table* t) {
+  if (!t) {
+    a2c::g_table = a2c::default_table();
+    return A2C_OK;
+  }
+  if (!a2c::valid(*t)) return A2C_EINVAL;
+  a2c::g_table = *t;
+  return A2C_OK;
+}
+
+void a2c_autotune_opts_default(a2c_autotune_opts* o) {
+  std::memset(o, 0, sizeof(*o));
+  const int cells[] = {6, 8, 11, 14, 18, 24, 32, 40};
+  o->n_sizes = 8;
+  for (int i = 0; i < 8; ++i) o->cells[i] = cells[i];
+  o->target_sites = 1000000;
+  o->reps = 10;
+  o->apply = 0;
+}
+
+int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_report* report) {
+  if (!o || !out || o->n_sizes < 1 || o->n_sizes > A2C_TUNE_MAX_CLASSES || o->reps < 1 || o->target_sites < 1)
+    return A2C_EINVAL;
+  for (int i = 0; i < o->n_sizes; ++i)
+    if (o->cells[i] < 3 || o->cells[i] > 128 || (i > 0 && o->cells[i] <= o->cells[i - 1])) return A2C_EINVAL;
+  a2c_autotune_report rep;
+  std::memset(&rep, 0, sizeof rep);
+  a2c_tune_table res;
+  std::memset(&res, 0, sizeof res);
+  res.n_classes = o->n_sizes;
+  const a2c_tune_table saved = a2c::g_table;
+  const int lanes[4] = {1, 2, 4, 8};
+  int rc = A2C_OK;
+  for (int i = 0; i < o->n_sizes && rc == A2C_OK; ++i) {
+    const int n = o->cells[i];
+    const int npd = 4 * n * n * n, N = 2 * npd;
+    const int copies = (int)std::max<int64_t>(1, std::min<int64_t>(256, o->target_sites / N));
+    rep.sites[i] = N;
+    rep.copies[i] = copies;
+    a2c_batch_desc bd;
+    a2c_batch_desc_default(&bd);
+    bd.device = o->device;
+    bd.precision = A2C_PREC_MIXED;
+    bd.max_lattices = copies;
+    bd.graphs = 0;
+    a2c_batch* b = nullptr;
+    if ((rc = a2c_batch_create(&bd, &b))) break;
+    std::vector<int32_t> slots(copies);
+    std::vector<double> X(8 * (size_t)N), x(npd), box(3), f(copies);
+    for (int k = 0; k < copies && rc == A2C_OK; ++k) {
+      // the batched workload's recipe: displacement 0.02 A, sigma 0.10 A, mean H fraction in [0.01, 0.6]
+      a2c_lattice_desc L;
+      a2c_lattice_desc_default(&L);
+      L.nx = L.ny = L.nz = n;
+      L.seed = 7000 + 97 * (uint64_t)i + k;
+      L.disp_std = 0.02;
+      L.sigma_lo = L.sigma_hi = 0.10;
+      L.x_lo = 0.01 + 0.59 * (k % 8) / 7.0;
+      L.x_hi = L.x_lo;
+      if ((rc = a2c_lattice_fcc_pdh(&L, X.data(), x.data(), box.data()))) break;
+      a2c_fe_desc d;
+      a2c_fe_desc_default(&d);
+      d.n_pd = d.n_h = npd;
+      for (int c = 0; c < 3; ++c) {
+        d.box[c] = box[c];
+        d.periodic[c] = 1;
+      }
+      d.temperature = 500.0;
+      rc = a2c_batch_add(b, &d, X.data(), x.data(), &slots[k]);
+    }
+    for (int p = 0; p < 2 && rc == A2C_OK; ++p) {
+      const int prec = p == 0 ? A2C_PREC_MIXED : A2C_PREC_FP64;
+      double best = HUGE_VAL;
+      int bestk = 1;
+      for (int li = 0; li < 4 && rc == A2C_OK; ++li) {
+        a2c::g_table = a2c::uniform(lanes[li]);
+        for (int w = 0; w < 2 && rc == A2C_OK; ++w) rc = a2c_batch_eval(b, copies, slots.data(), prec, f.data());
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < o->reps && rc == A2C_OK; ++r) rc = a2c_batch_eval(b, copies, slots.data(), prec, f.data());
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double ns = s * 1e9 / ((double)o->reps * copies * N);
+        rep.ns_per_site[i][p][li] = ns;
+        if (ns < best) {
+          best = ns;
+          bestk = lanes[li];
+        }
+      }
+      (p == 0 ? res.lanes_mixed : res.lanes_fp64)[i] = bestk;
+    }
+    a2c_batch_destroy(b);
+  }
+  a2c::g_table = saved;
+  if (rc) return rc;
+  for (int i = 0; i + 1 < o->n_sizes; ++i)  // boundary at the geometric mean of neighboring sizes
+    res.max_sites[i] = (int32_t)std::sqrt((double)rep.sites[i] * rep.sites[i + 1]);
+  res.max_sites[o->n_sizes - 1] = INT_MAX;
+  *out = res;
+  if (report) *report = rep;
+  if (o->apply) a2c::g_table = res;
+  return A2C_OK;
+}
+
+}  // extern "C"

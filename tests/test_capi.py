@@ -119,3 +119,23 @@ def test_batch_descriptor_graph_flag(monkeypatch):
     d.graphs = 2  # only 0 and 1; rejected before any device use
     b = C.c_void_p()
     assert A.lib.a2c_batch_create(C.byref(d), C.byref(b)) == A.A2C_EINVAL
+
+
+def test_tune_table_get_set_and_validation():
+    """The lanes-per-site table is host state: no GPU needed to read, set or
+    reject it."""
+    t0 = A.tune_get()
+    assert t0["max_sites"][-1] == 2**31 - 1 and all(k in (1, 2, 4, 8) for k in t0["lanes_mixed"] + t0["lanes_fp64"])
+    try:
+        A.tune_set({"max_sites": [1000, 2**31 - 1], "lanes_mixed": [4, 2], "lanes_fp64": [8, 1]})
+        assert A.tune_get() == {"max_sites": [1000, 2**31 - 1], "lanes_mixed": [4, 2], "lanes_fp64": [8, 1]}
+        for bad in ({"max_sites": [10], "lanes_mixed": [3], "lanes_fp64": [1]},             # K not a power of 2
+                    {"max_sites": [10], "lanes_mixed": [16], "lanes_fp64": [1]},            # K above 8
+                    {"max_sites": [100, 50], "lanes_mixed": [1, 1], "lanes_fp64": [1, 1]},  # not increasing
+                    {"max_sites": [], "lanes_mixed": [], "lanes_fp64": []}):
+            with pytest.raises(A.A2CError):
+                A.tune_set(bad)
+        assert A.tune_get()["lanes_mixed"] == [4, 2]  # a rejected table changes nothing
+    finally:
+        A.tune_set(None)
+    assert A.tune_get() == t0

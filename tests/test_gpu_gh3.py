@@ -84,7 +84,7 @@ def test_objective_gradient_mixed_gh3(case):
 
 
 def test_mixed_matches_fp64_kernel_gh3_at_12_cubed():
-    """13,824 sites (one lane per site): mixed vs the FP64 kernel."""
+    """13,824 sites: mixed vs the FP64 kernel."""
     X, x, box, per3, npd, nh = specimen(12, seed=2, disp=0.03, sigma=(0.08, 0.15), x=(0.0, 0.6))
     fe = A.FreeEntropy(npd, nh, box, per3, T=500.0, quad="gh3")
     fe.set_occupancy(x)
