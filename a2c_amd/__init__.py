@@ -193,6 +193,7 @@ def _load():
         "a2c_batch_eval": (I, [VP, C.c_int32, ip, C.c_int32, dp]),
         "a2c_batch_lmvm_start": (I, [VP, C.c_int32, P(LmvmOpts)]),
         "a2c_batch_lmvm_step": (I, [VP, C.c_int32, P(C.c_int32)]),
+        "a2c_batch_lmvm_step_slots": (I, [VP, C.c_int32, ip, C.c_int32, P(C.c_int32)]),
         "a2c_batch_lmvm_report": (I, [VP, C.c_int32, P(LmvmReport)]),
         "a2c_tune_get": (None, [P(TuneTable)]),
         "a2c_tune_set": (I, [P(TuneTable)]),
@@ -621,6 +622,13 @@ class Batch:
     def step(self, max_iters=1):
         n = C.c_int32()
         self._check(lib.a2c_batch_lmvm_step(self._b, max_iters, C.byref(n)))
+        return n.value
+
+    def step_slots(self, slots, max_iters=1):
+        """Advance only the listed slots; returns the running count of the batch."""
+        ids = np.ascontiguousarray(slots, dtype=np.int32)
+        n = C.c_int32()
+        self._check(lib.a2c_batch_lmvm_step_slots(self._b, len(ids), _iptr(ids), max_iters, C.byref(n)))
         return n.value
 
     def report(self, slot):

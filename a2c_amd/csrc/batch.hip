@@ -954,17 +954,19 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
 
 }  // namespace
 
-// Advance every running slot by up to max_iters iterations.
-int batch_step(a2c_batch* b, int max_iters, int* n_running) {
+// Advance every running slot (or, with a filter, the running slots it
+// selects) by up to max_iters iterations.
+int batch_step(a2c_batch* b, int max_iters, int* n_running, const std::vector<char>* only = nullptr) {
+  auto chosen = [&](size_t i) { return !only || (*only)[i]; };
   std::vector<int> init;
   for (size_t i = 0; i < b->slots.size(); ++i)
-    if (b->slots[i].used && b->slots[i].running && !b->slots[i].started) init.push_back((int)i);
+    if (chosen(i) && b->slots[i].used && b->slots[i].running && !b->slots[i].started) init.push_back((int)i);
   if (int rc = lmvm_init(b, init)) return rc;
   for (int it = 0; it < max_iters; ++it) {
     std::vector<int> act;
     for (size_t i = 0; i < b->slots.size(); ++i) {
       Slot& s = b->slots[i];
-      if (s.used && s.running && s.started) {
+      if (chosen(i) && s.used && s.running && s.started) {
         if (s.rep.iterations >= s.opts.max_it) { s.rep.reason = -2; s.running = false; continue; }
         act.push_back((int)i);
       }
@@ -1178,6 +1180,19 @@ int a2c_batch_lmvm_step(a2c_batch* b, int32_t max_iters, int32_t* n_running) {
   if (!b || max_iters < 0) return A2C_EINVAL;
   b->err.clear();
   return batch_step(b, max_iters, n_running);
+}
+
+int a2c_batch_lmvm_step_slots(a2c_batch* b, int32_t n, const int32_t* slots, int32_t max_iters,
+                              int32_t* n_running) {
+  if (!b || n < 0 || (n > 0 && !slots) || max_iters < 0) return A2C_EINVAL;
+  b->err.clear();
+  std::vector<char> only(b->slots.size(), 0);
+  for (int k = 0; k < n; ++k) {
+    if (slots[k] < 0 || slots[k] >= (int)b->slots.size() || !b->slots[slots[k]].used)
+      return berr(b, A2C_EINVAL, "bad slot");
+    only[slots[k]] = 1;
+  }
+  return batch_step(b, max_iters, n_running, &only);
 }
 
 int a2c_batch_lmvm_report(a2c_batch* b, int32_t slot, a2c_lmvm_report* rep) {

@@ -1,11 +1,19 @@
 """Continuous-batching relaxation server on one GPU.
 
 Requests (a lattice to relax) arrive as a Poisson stream.  The loop keeps
-every admitted lattice resident in HBM in one a2c_batch and advances all
-running relaxations together, one LMVM iteration per tick (one fused launch
-per evaluation round for all of them).  Between ticks it admits the requests
-that have arrived, as long as their device footprint fits the HBM budget, and
-retires the converged ones.  Latency = finish - arrival.
+every admitted lattice resident in HBM in one a2c_batch and advances running
+relaxations together, one LMVM iteration per tick (one fused launch per
+evaluation round for all lattices stepped).  Which lattices a tick steps is
+decided by stride scheduling: each lattice carries a virtual time that
+advances by the cost of its iteration (ITER_FIXED_S + ITER_SITE_S x sites,
+measured on one MI355X) whenever it is stepped, and a tick steps every
+lattice within one smallest-iteration cost of the least virtual time.  Each
+request thus receives about equal GPU time, so a small request does not wait
+for the iterations of much larger ones and no request starves (a lattice's
+trajectory is the same whichever lattices share its steps).  Between ticks
+the loop admits the requests that have arrived, as long as their device
+footprint fits the HBM budget, and retires the converged ones.  Latency =
+finish - arrival.
 
 Workload (defaults): edge n log-uniform in [6, 48] cells (n^3 cells, 8 sites
 per cell), T ~ U(300, 700) K, mean H fraction ~ U(0, 0.6) with iid noise
@@ -51,8 +59,21 @@ def materialize(req):
     return X, x, box, npd, nh
 
 
-def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None):
+# Cost of one LMVM iteration of a lattice (mixed precision, in a batch): a
+# fixed part (~50 launches and 3 host round trips) and a per-site part.
+ITER_FIXED_S = 4e-4
+ITER_SITE_S = 9e-9
+
+
+def iter_cost(sites):
+    return ITER_FIXED_S + ITER_SITE_S * sites
+
+
+def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None, fair=True):
+    """fair: stride scheduling by iteration cost; else every running lattice
+    is stepped on every tick (lockstep)."""
     opts = opts or {"gatol": 1e-8, "grtol": 0.0}
+    vt = {}  # slot -> virtual time (seconds of GPU time received)
     pending = list(reqs)
     running = {}  # slot -> request
     done = []
@@ -69,13 +90,22 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
                 X, x, box, npd, nh = materialize(req)
                 slot = B.add(X, x, npd, nh, box, (1, 1, 1), T=req["T"])
                 B.start(slot, **opts)
+                vt[slot] = min(vt.values()) if vt else 0.0
                 req["admitted"] = time.perf_counter() - t0
                 running[slot] = req
                 pending.pop(0)
             if not running:
                 time.sleep(max(0.0, pending[0]["arrival"] - (time.perf_counter() - t0)))
                 continue
-            B.step(1)
+            if fair:
+                cost = {sl: iter_cost(8 * r["n"] ** 3) for sl, r in running.items()}
+                horizon = min(vt[sl] for sl in running) + min(cost.values())
+                due = [sl for sl in running if vt[sl] <= horizon]
+                B.step_slots(due, 1)
+                for sl in due:
+                    vt[sl] += cost[sl]
+            else:
+                B.step(1)
             for slot in list(running):
                 rep = B.report(slot)
                 if rep["reason"] != 0:
@@ -85,6 +115,7 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
                     req["report"] = rep
                     done.append(req)
                     B.remove(slot)
+                    vt.pop(slot, None)
                     if log:
                         log(req)
     wall = time.perf_counter() - t0
@@ -127,6 +158,7 @@ def main():
     p.add_argument("--requests", type=int, default=24)
     p.add_argument("--nmax", type=int, default=48)
     p.add_argument("--no-baseline", action="store_true")
+    p.add_argument("--lockstep", action="store_true", help="also serve with every lattice stepped every tick")
     a = p.parse_args()
     out = []
     for rate in [float(r) for r in a.rates.split(",")]:
@@ -136,6 +168,9 @@ def main():
                                file=sys.stderr, flush=True)
         done, wall = serve([dict(r) for r in reqs], log=note)
         row = {"rate": rate, "batched_mixed": summarize(done, wall)}
+        if a.lockstep:
+            d2, w2 = serve([dict(r) for r in reqs], fair=False)
+            row["batched_mixed_lockstep"] = summarize(d2, w2)
         if not a.no_baseline:
             d1, w1 = serve_one_at_a_time([dict(r) for r in reqs])
             row["one_at_a_time_fp64"] = summarize(d1, w1)

@@ -66,6 +66,12 @@ int a2c_batch_eval(a2c_batch* b, int32_t n, const int32_t* slots, int32_t precis
  * every running slot by up to max_iters iterations per call. */
 int a2c_batch_lmvm_start(a2c_batch* b, int32_t slot, const a2c_lmvm_opts* opts);
 int a2c_batch_lmvm_step(a2c_batch* b, int32_t max_iters, int32_t* n_running);
+/* The same for the listed slots only (those not running are skipped); a
+ * scheduler can step small lattices more often than large ones.  A slot's
+ * trajectory does not depend on which other slots share its steps.
+ * n_running counts every running slot of the batch. */
+int a2c_batch_lmvm_step_slots(a2c_batch* b, int32_t n, const int32_t* slots, int32_t max_iters,
+                              int32_t* n_running);
 /* reason 0 while the slot is still running */
 int a2c_batch_lmvm_report(a2c_batch* b, int32_t slot, a2c_lmvm_report* rep);
 

@@ -150,3 +150,33 @@ def test_single_lattice_minimizer_same_with_and_without_graphs(monkeypatch):
             res.append(fe.minimizeFreeEntropy(s["X"], gatol=1e-9, grtol=0.0))
     (Xa, ra), (Xb, rb) = res
     assert ra == rb and np.array_equal(Xa, Xb)
+
+
+def test_stepping_subsets_gives_the_alone_trajectories():
+    """Slots stepped in uneven subsets (as a size-tiered scheduler does) end
+    with the same reports and states, bit for bit, as relaxed alone."""
+    specs = [make(s) for s in LATS]
+    opts = dict(gatol=1e-9, grtol=0.0, max_it=8000)
+    alone = []
+    for s in specs:
+        with A.FreeEntropy(s["n_pd"], s["n_h"], s["box"], s["periodic"], T=s["T"], precision="mixed") as fe:
+            fe.set_occupancy(s["x"])
+            fe.findNeighbors(s["X"])
+            alone.append(fe.minimizeFreeEntropy(s["X"], **opts))
+    with A.Batch(precision="mixed") as B:
+        slots = [B.add(**s) for s in specs]
+        for sl in slots:
+            B.start(sl, **opts)
+        tick = 0
+        while True:
+            due = [sl for k, sl in enumerate(slots) if tick % (k + 1) == 0]  # slot k every (k+1)-th tick
+            running = B.step_slots(due, 1 + tick % 3)
+            tick += 1
+            if running == 0:
+                break
+        for sl, (Xa, ra) in zip(slots, alone):
+            assert B.report(sl) == ra
+            assert np.array_equal(B.get_state(sl)[0], Xa)
+    with A.Batch(precision="mixed") as B:
+        with pytest.raises(A.A2CError):
+            B.step_slots([5], 1)  # not a used slot

@@ -10,10 +10,11 @@ from a2c_amd import serve
 pytestmark = pytest.mark.gpu
 
 
-def test_served_results_equal_standalone():
+@pytest.mark.parametrize("fair", [True, False])
+def test_served_results_equal_standalone(fair):
     reqs = serve.make_requests(6, rate=50.0, nmin=3, nmax=6)
     opts = {"gatol": 1e-8, "grtol": 0.0}
-    done, wall = serve.serve([dict(r) for r in reqs], precision="mixed", opts=opts)
+    done, wall = serve.serve([dict(r) for r in reqs], precision="mixed", opts=opts, fair=fair)
     assert len(done) == len(reqs)
     overlap = sum(1 for d in done if any(o["admitted"] < d["admitted"] < o["finish"] for o in done if o is not d))
     assert overlap > 0  # some request was admitted while another was running
