@@ -97,6 +97,7 @@ struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null
 constexpr int SCAL_PER_SLOT = 72;  // >= m + 5 for m <= 64
 constexpr double SIGMA_STEP = 4.0;  // largest factor by which one line-search trial may change a width
 constexpr double Q_STEP = 0.25;     // largest displacement of a site in one line-search trial (A)
+constexpr int MIXED_MAX_TRIALS = 8;  // line-search trials of a mixed-precision iteration before FP64 takes over
 
 struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero the active part of d,
                    // 3 project a trial point and form the step direction it realises
@@ -230,6 +231,103 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
   }
 }
 
+// Two-loop recursion of one small slot in one workgroup.  The segmented form
+// costs ~4 launches per stored pair; a slot of a few thousand sites is
+// launch-bound, so one workgroup per slot runs the whole recursion (every dot
+// a fixed-order block sum), then d = -H g with the active part of the sigma
+// bound zeroed, and writes g . d.  Pass t applies the update of step t and
+// forms the dot the next step needs, so the recursion takes 2 np + 1 passes.
+constexpr int TL_THREADS = 1024;
+constexpr int TL_MAX_PAIRS = 64;
+constexpr int TL_MAX_SITES = 8192;  // larger slots take the segmented kernels
+struct TwoLoop {
+  const double* g;
+  double* q;
+  double* d;
+  const double* x;
+  double* out;                       // out[0] = g . d
+  const double* S[TL_MAX_PAIRS];     // step t = 0 is the newest pair
+  const double* Y[TL_MAX_PAIRS];
+  double rho[TL_MAX_PAIRS];
+  Metric met;
+  double lb, ub;
+  int32_t np, nsite;
+};
+
+// block sum in a fixed order: wave trees, then the 16 wave partials in order
+__device__ __forceinline__ double tl_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  __syncthreads();  // the previous sum's readers are done with red
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < TL_THREADS / 64; ++w) s += red[w];
+  return s;
+}
+
+__global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restrict__ ops) {
+  __shared__ double red[TL_THREADS / 64];
+  __shared__ double al[TL_MAX_PAIRS];  // A_t = s_t . q of the first loop
+  const TwoLoop& o = ops[blockIdx.x];
+  const int n = 4 * o.nsite, np = o.np;
+  // q = g; A_0 = s_0 . q; the scale gamma = s_0 . y_0 / y_0 . D y_0 of H0 = gamma D
+  double a = 0.0, sy = 0.0, yy = 0.0;
+  for (int i = threadIdx.x; i < n; i += TL_THREADS) {
+    const double gi = o.g[i], si = o.S[0][i], yi = o.Y[0][i];
+    o.q[i] = gi;
+    a += si * gi;
+    sy += si * yi;
+    yy += yi * yi * metric_inv(o.met, i >> 2);
+  }
+  double A = tl_sum(a, red);
+  const double gamma = tl_sum(sy, red) / tl_sum(yy, red);
+  if (threadIdx.x == 0) al[0] = A;
+  // first loop, newest to oldest: q -= rho_t A_t y_t; then q = gamma D q
+  for (int t = 0; t < np; ++t) {
+    const double c = -o.rho[t] * A;
+    const double* yt = o.Y[t];
+    const bool last = t + 1 == np;
+    const double* nx = last ? o.Y[np - 1] : o.S[t + 1];  // next dot: A_{t+1}, or B_{np-1} = y . q
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += TL_THREADS) {
+      double qi = o.q[i] + c * yt[i];
+      if (last) qi = gamma * metric_inv(o.met, i >> 2) * qi;
+      o.q[i] = qi;
+      acc += nx[i] * qi;
+    }
+    A = tl_sum(acc, red);
+    if (!last && threadIdx.x == 0) al[t + 1] = A;
+  }
+  // second loop, oldest to newest: q += rho_t (A_t - B_t) s_t, B_t = y_t . q
+  double B = A;
+  for (int t = np - 1; t >= 0; --t) {
+    const double c = o.rho[t] * (al[t] - B);
+    const double* st = o.S[t];
+    double acc = 0.0;
+    if (t > 0) {
+      const double* yn = o.Y[t - 1];
+      for (int i = threadIdx.x; i < n; i += TL_THREADS) {
+        const double qi = o.q[i] + c * st[i];
+        o.q[i] = qi;
+        acc += yn[i] * qi;
+      }
+    } else {  // d = -q, sigma components at an active bound zeroed; g . d
+      for (int i = threadIdx.x; i < n; i += TL_THREADS) {
+        double di = -(o.q[i] + c * st[i]);
+        if ((i & 3) == 3) {
+          const double xs = o.x[i];
+          if ((xs <= o.lb && di < 0.0) || (xs >= o.ub && di > 0.0)) di = 0.0;
+        }
+        o.d[i] = di;
+        acc += o.g[i] * di;
+      }
+    }
+    B = tl_sum(acc, red);
+  }
+  if (threadIdx.x == 0) o.out[0] = B;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -306,6 +404,7 @@ struct a2c_batch {
   a2c::Arena arena;
   a2c::GraphCache graphs;
   int qrc = 0;  // first error of a queued operation
+  bool fused_two_loop = true;  // small slots run the two-loop recursion in one workgroup
   double* dF = nullptr;    // objective per slot (device)
   double* hF = nullptr;    // pinned mirror
   double* dres = nullptr;  // 2 doubles per slot (device dot results read by the host)
@@ -732,17 +831,46 @@ int lmvm_init(a2c_batch* b, const std::vector<int>& ids) {
 // d = -H g for the listed slots (two-loop recursion, or scaled steepest
 // descent for slots with no pairs or marked steepest)
 int directions(a2c_batch* b, const std::vector<int>& ids) {
-  std::vector<int> tl, sd;
+  std::vector<int> tl, sd, small, rest;  // segmented two-loop, steepest descent, fused two-loop; tl + sd
   int maxp = 0;
   for (int i : ids) {
     Slot& s = b->slots[i];
     if (s.steepest || s.npairs == 0) {
       s.npairs = 0;
       sd.push_back(i);
+      rest.push_back(i);
+    } else if (b->fused_two_loop && s.N() <= TL_MAX_SITES) {
+      small.push_back(i);
     } else {
       tl.push_back(i);
+      rest.push_back(i);
       maxp = std::max(maxp, s.npairs);
     }
+  }
+  if (!small.empty()) {
+    std::vector<TwoLoop> ops(small.size());
+    for (size_t k = 0; k < small.size(); ++k) {
+      Slot& s = b->slots[small[k]];
+      TwoLoop& o = ops[k];
+      o.g = s.g; o.q = s.q; o.d = s.d; o.x = s.x;
+      o.out = b->dres + 2 * small[k];
+      for (int t = 0; t < s.npairs; ++t) {
+        const int p = (s.head - 1 - t + s.m) % s.m;
+        o.S[t] = s.S[p]; o.Y[t] = s.Y[p]; o.rho[t] = s.rho[p];
+      }
+      o.met = s.metric();
+      o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+      o.np = s.npairs; o.nsite = s.N();
+    }
+    const size_t need = (ops.size() * sizeof(TwoLoop) + 255) & ~(size_t)255;
+    if (b->arena.used + need > b->arena.cap)
+      if (int rc = bsync(b)) return rc;
+    const TwoLoop* dops = b->arena.push(ops.data(), ops.size());
+    if (!dops) return berr(b, A2C_ENOMEM, "staging arena too small");
+    const unsigned nops = (unsigned)ops.size();
+    enqueue(b, {6, u64(dops), nops}, [=](hipStream_t st) {
+      hipLaunchKernelGGL(k_two_loop, dim3(nops), dim3(TL_THREADS), 0, st, dops);
+    });
   }
   std::vector<double> v;
   auto scal = [b](const Slot& s) { return b->dscal + SCAL_PER_SLOT * (&s - b->slots.data()); };
@@ -811,12 +939,18 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
         }))
       return rc;
   }
-  if (int rc = seg_bound<2>(b, ids, [](SegBound& o, Slot& s) {
+  if (int rc = seg_bound<2>(b, rest, [](SegBound& o, Slot& s) {
         o.z = s.d; o.x = s.x; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
       }))
     return rc;
-  if (int rc = seg_dot(b, ids, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
-  for (size_t k = 0; k < ids.size(); ++k) b->slots[ids[k]].gd = v[2 * k];
+  if (!rest.empty()) {  // its copy of the results also brings the fused slots' g . d
+    if (int rc = seg_dot(b, rest, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
+    for (size_t k = 0; k < rest.size(); ++k) b->slots[rest[k]].gd = v[2 * k];
+  } else {
+    enqueue_d2h(b, b->hres, b->dres, sizeof(double) * 2 * b->slots.size());
+    if (int rc = bsync(b)) return rc;
+  }
+  for (int i : small) b->slots[i].gd = b->hres[2 * i];
   return A2C_OK;
 }
 
@@ -876,6 +1010,10 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
       s.nfev++;
       if (!std::isfinite(s.f)) { s.lsstat = -2; continue; }
       s.lsstat = s.ls.iterate(s.f, v[2 * k], s.stp);  // slope along the realised (projected) direction
+      // A mixed search that needs many trials has met the noise of the FP32
+      // objective: give up (the slot then continues in FP64 from the
+      // iteration's start) instead of spending up to 30 evaluations.
+      if (s.lsstat == 0 && s.mixed && s.opts.mixed_switch && s.nfev >= MIXED_MAX_TRIALS) s.lsstat = -3;
       if (s.lsstat == 0 && s.nfev >= 30) s.lsstat = 2;
       if (s.lsstat == 0) next.push_back(run[k]);
     }
@@ -895,7 +1033,10 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
       Slot& s = b->slots[i];
       s.f = s.fold;
       if (s.lsstat == -1) { s.rep.reason = -3; s.running = false; }
-      else if (s.mixed && s.opts.mixed_switch) { sw.push_back(i); s.npairs = 0; }  // incl. non-finite mixed trials
+      else if (s.mixed && s.opts.mixed_switch) {  // incl. non-finite mixed trials
+        sw.push_back(i);
+        if (s.lsstat != -3) s.npairs = 0;  // a search stopped by the trial cap keeps its curvature pairs
+      }
       else if (s.lsstat == -2) { s.rep.reason = -4; s.running = false; }
       else if (s.npairs == 0) { s.rep.reason = -6; s.running = false; }
       else s.npairs = 0;
@@ -1017,6 +1158,8 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
   a2c_batch* b = new (std::nothrow) a2c_batch;
   if (!b) return A2C_ENOMEM;
   b->desc = *desc;
+  const char* tl = std::getenv("A2C_TWO_LOOP");  // "segmented": every slot takes the segmented kernels
+  b->fused_two_loop = !(tl && std::string(tl) == "segmented");
   auto fail = [&](int rc) { a2c_batch_destroy(b); return rc; };
   if (hipSetDevice(desc->device) != hipSuccess) return fail(A2C_EHIP);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(A2C_EHIP);

@@ -98,13 +98,16 @@ def test_sigma_bound_respected():
 
 def test_sigma_upper_bound_caps_and_is_kkt():
     """sigma_ub: widths stay in [lb, ub]; at the cap the sigma gradient points
-    outward (df/dsigma < 0) and the projected gradient meets gatol."""
+    outward (df/dsigma < 0) and the projected gradient meets gatol.  With half
+    the widths at the cap, FP64 line searches stop resolving progress near
+    ||Pg|| ~ 3e-7 on this lattice (where the run goes below that depends on
+    rounding), so the tolerance is 1e-6."""
     X, x, box, per, npd, nh = specimen(3, seed=6, sigma=(0.10, 0.10))[:6]
     fe = A.FreeEntropy(npd, nh, box, per, T=300.0)
     fe.set_occupancy(x)
     fe.findNeighbors(X)
     ub = 0.09  # below the relaxed H widths (~0.15), above the Pd widths (~0.08)
-    Xr, rep = fe.minimizeFreeEntropy(X, sigma_ub=ub, gatol=1e-7, grtol=0.0)
+    Xr, rep = fe.minimizeFreeEntropy(X, sigma_ub=ub, gatol=1e-6, grtol=0.0)
     assert rep["reason"] > 0, rep
     s = Xr[3::4]
     assert np.all(s <= ub) and np.all(s >= 1e-6)
@@ -133,3 +136,43 @@ def test_preconditioned_and_plain_metrics_reach_the_same_state():
     w = np.concatenate([np.ones(npd), x])
     assert np.max(np.abs(d) * w[:, None]) <= 1e-6, np.max(np.abs(d) * w[:, None])
     print("iterations: precond %d, plain %d" % (r1["iterations"], r0["iterations"]))
+
+
+def test_fused_two_loop_matches_the_segmented_recursion(monkeypatch):
+    """Slots of up to 8,192 sites run the two-loop recursion in one workgroup;
+    A2C_TWO_LOOP=segmented sends them through the segmented kernels that
+    larger slots use.  Same iterations; states agree to rounding after a few
+    iterations and to 1e-6 A at convergence."""
+    X, x, box, per, npd, nh = specimen(4, seed=7, disp=0.03, sigma=(0.08, 0.15), x=(0.05, 0.5))
+    out = {}
+    for mode in ("fused", "segmented"):
+        if mode == "segmented":
+            monkeypatch.setenv("A2C_TWO_LOOP", "segmented")
+        fe = A.FreeEntropy(npd, nh, box, per, T=400.0, precision="fp64")
+        fe.set_occupancy(x)
+        fe.findNeighbors(X)
+        out[mode] = (fe.minimizeFreeEntropy(X, max_it=6), fe.minimizeFreeEntropy(X, gatol=1e-9, grtol=0.0))
+        fe.close()
+    (Xa, ra), (Xc, rc) = out["fused"][0], out["segmented"][0]
+    assert ra["iterations"] == rc["iterations"] == 6
+    assert np.max(np.abs(Xa - Xc)) <= 1e-9
+    (Xa, ra), (Xc, rc) = out["fused"][1], out["segmented"][1]
+    assert ra["reason"] > 0 and rc["reason"] > 0
+    dq, ds = compare(Xa, Xc)
+    assert dq <= 1e-6 and ds <= 1e-6
+
+
+def test_relaxation_of_a_lattice_above_the_fused_size():
+    """10,648 sites: the segmented two-loop kernels; the end state is
+    stationary for the oracle."""
+    X, x, box, per, npd, nh = specimen(11, seed=8, disp=0.02, sigma=(0.10, 0.10), x=(0.10, 0.10))
+    fe = A.FreeEntropy(npd, nh, box, per, T=300.0, precision="mixed")
+    fe.set_occupancy(x)
+    fe.findNeighbors(X)
+    Xr, rep = fe.minimizeFreeEntropy(X, gatol=1e-9, grtol=0.0)
+    assert rep["reason"] > 0, rep
+    orc = O.Oracle(npd, nh, box, per, T=300.0, threads=8)
+    orc.build_neighbors(X)
+    fo, Go = orc.eval(Xr, x)
+    assert abs(rep["f"] - fo) <= 1e-10 * abs(fo)
+    assert np.linalg.norm(Go) <= 1e-8
