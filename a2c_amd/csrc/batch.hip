@@ -413,6 +413,7 @@ struct a2c_batch {
   double* dpart = nullptr;  // 2 * max blocks
   size_t part_cap = 0;      // blocks
   float4* dtab = nullptr;
+  double* dtab64 = nullptr;
   size_t mem_used = 0;
 };
 
@@ -701,10 +702,11 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
   L.deriv_mode = b->desc.deriv_mode;
   L.fd_eps = b->desc.fd_eps;
   L.tab = b->dtab;
+  L.tab64 = b->dtab64;
   L.mixed_full = mixed_full_pairs();
   L.full_pairs64 = fp64_full_pairs();
   enqueue(b, {5, (uint64_t)fp64, u64(L.lats), u64(L.b2l), u64(L.ob2l), (uint64_t)L.nblk, (uint64_t)L.onblk,
-              (uint64_t)L.nlat, (uint64_t)L.rules, (uint64_t)L.deriv_mode, u64(L.fd_eps), u64(L.tab),
+              (uint64_t)L.nlat, (uint64_t)L.rules, (uint64_t)L.deriv_mode, u64(L.fd_eps), u64(L.tab), u64(L.tab64),
               (uint64_t)L.mixed_full, (uint64_t)L.full_pairs64},
           [=](hipStream_t st) {
             Launch l = L;
@@ -1175,6 +1177,11 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
   host_tables(t);
   if (hipMalloc(&b->dtab, sizeof(float4) * t.size()) != hipSuccess) return fail(A2C_ENOMEM);
   if (hipMemcpy(b->dtab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice) != hipSuccess) return fail(A2C_EHIP);
+  std::vector<double> t64;
+  host_tables64(t64);
+  if (hipMalloc(&b->dtab64, sizeof(double) * t64.size()) != hipSuccess) return fail(A2C_ENOMEM);
+  if (hipMemcpy(b->dtab64, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(A2C_EHIP);
   *out = b;
   return A2C_OK;
 }
@@ -1196,6 +1203,7 @@ void a2c_batch_destroy(a2c_batch* b) {
   if (b->hres) (void)hipHostFree(b->hres);
   (void)hipFree(b->dpart);
   (void)hipFree(b->dtab);
+  (void)hipFree(b->dtab64);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

@@ -93,8 +93,15 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
             hipMalloc(&c->dpart, 8 * (size_t)c->nblk_red) == hipSuccess && hipMalloc(&c->dF, 64) == hipSuccess &&
             hipHostMalloc(&c->hX, 32 * N) == hipSuccess && hipHostMalloc(&c->hG, 32 * N) == hipSuccess &&
             hipHostMalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && hipMalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
-            hipMalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess;
+            hipMalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess &&
+            hipMalloc(&c->dtab64, sizeof(double) * T64_NF * T64_N * T64_P) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
+  {
+    std::vector<double> t64;
+    host_tables64(t64);
+    if (hipMemcpy(c->dtab64, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(A2C_EHIP);
+  }
   new (c->hlat) LatDev();
   // all-zero block -> lattice map, long enough for the pair-owner launches too
   if (hipMemset(c->db2l, 0, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) != hipSuccess) return fail(A2C_EHIP);
@@ -109,6 +116,7 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   hipFree(c->dX); hipFree(c->dG); hipFree(c->dx); hipFree(c->drho); hipFree(c->dFp);
   hipFree(c->demb); hipFree(c->dpart); hipFree(c->dF); hipFree(c->dpdpdnum); hipFree(c->dtab); hipFree(c->daux);
+  hipFree(c->dtab64);
   for (int w = 0; w < 5; ++w) {
     hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt);
     a2c::free_pair_buffers(c->ell[w]);

@@ -3,7 +3,10 @@
 // reference's central-FD derivatives are formed by the callers.  Templated on
 // the floating type: double on the device (the FP64 kernels), long double on
 // the host (building the FP32 tables of the mixed path).  Constants are those
-// of potential.cpp, in its order, as the doubles the reference parses.
+// of potential.cpp, in its order, as the doubles the reference parses.  The
+// radial polynomials' scale t = r / 5 is applied as r * 0.2 (within an ulp
+// of the quotient): an FP64 divide is a ~10-instruction sequence, and the
+// Pd-Pd terms of one quadrature point would take four of them.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -76,7 +79,7 @@ A2C_HD VDT<T> F_H_vd(T rho) {
 // ---- density f_Pd(r), potential.cpp:64-98 -------------------------------------
 template <typename T = double>
 A2C_HD T f_Pd(T r) {
-  T t = r / 5.0, rr = t * 5.0;
+  T t = r * 0.2, rr = t * 5.0;
   if (rr <= 0.0) return 0.0;
   if (rr > 5.0) return 0.001510569378297 * std::exp(-0.077976566126292 / 0.001510569378297 * (rr - 5.0));
   T ra = t * (t * (t - 2.7267629107325706) + 1.8716766113599643) *
@@ -91,7 +94,7 @@ A2C_HD T f_Pd(T r) {
 }
 template <typename T = double>
 A2C_HD VDT<T> f_Pd_vd(T r) {
-  T t = r / 5.0, rr = t * 5.0;
+  T t = r * 0.2, rr = t * 5.0;
   if (rr <= 0.0) return {0.0, 0.0};
   if (rr > 5.0) {
     const T k = -0.077976566126292 / 0.001510569378297;
@@ -110,7 +113,7 @@ A2C_HD VDT<T> f_Pd_vd(T r) {
   h = {-255.8965835571289 + t * h.v, h.v + t * h.d};
   h = {0.6676807403564453 + t * h.v, h.v + t * h.d};
   h = {-0.02972698211669922 + t * h.v, h.v + t * h.d};
-  return {h.v, h.d / 5.0};
+  return {h.v, h.d * 0.2};
 }
 
 // ---- density f_H(r), potential.cpp:105-119 ------------------------------------
@@ -135,7 +138,7 @@ A2C_HD VDT<T> f_H_vd(T r) {
 // ---- pair phi_Pd(r), potential.cpp:126-155 ------------------------------------
 template <typename T = double>
 A2C_HD T phi_Pd(T r) {
-  T t = r / 5.0, rr = t * 5.0;
+  T t = r * 0.2, rr = t * 5.0;
   if (rr <= 0.0) return 0.0;
   if (rr > 5.05) return -0.016958780164980 * std::exp(0.089019679316328 / -0.016958780164980 * (rr - 5.05));
   T y = -79415.24035137112 * (t - 1.0699996145674568) * (t - 1.06015072612581) *
@@ -150,7 +153,7 @@ A2C_HD T phi_Pd(T r) {
 // density/embedding terms of the same point).
 template <typename T = double>
 A2C_HD VDT<T> phi_Pd_vd(T r, VDT<T> fpd) {
-  T t = r / 5.0, rr = t * 5.0;
+  T t = r * 0.2, rr = t * 5.0;
   if (rr <= 0.0) return {0.0, 0.0};
   if (rr > 5.05) {
     const T k = 0.089019679316328 / -0.016958780164980;
@@ -165,7 +168,7 @@ A2C_HD VDT<T> phi_Pd_vd(T r, VDT<T> fpd) {
   p = vmul(p, quadf(t, 1.6696359816422877, 0.7337878627470482));
   p = vmul(p, quadf(t, 1.1690370066230809, 0.3909805777737639));
   p = vmul(p, quadf(t, 0.2635598721249787, 0.033551116514910245));
-  return {p.v + 2 * 0.251546 * fpd.v, p.d / 5.0 + 2 * 0.251546 * fpd.d};
+  return {p.v + 2 * 0.251546 * fpd.v, p.d * 0.2 + 2 * 0.251546 * fpd.d};
 }
 
 // ---- pair phi_PdH / phi_H, potential.cpp:162-200 ------------------------------

@@ -10,7 +10,13 @@
 // Analytic mode evaluates each unordered pair once (k_dens_own64 ->
 // k_dens_recv64 -> k_grad_own64 -> k_grad_recv64): the pair's owner computes
 // both ends' shares and writes the partner's share into the partner's
-// received slot, as the mixed path does.  FD mode keeps the per-site
+// received slot, as the mixed path does.  These two owner kernels take the
+// exponential functions (f_H, phi_PdH, phi_H) from FP64 polynomial tables
+// in LDS (degree 8 on 0.05 A intervals, built in long double from the closed
+// forms: relative error below 1e-13 for values and slopes, under the FP64
+// rounding of the reference's own product polynomials) instead of three FP64
+// exp per quadrature point, and 1/|r| from the hardware estimate refined by
+// two Newton steps; the Pd polynomials stay closed forms.  FD mode keeps the per-site
 // full-list loop nest of the reference (every pair from both ends), since
 // its derivatives are the reference's central differences per site.
 //
@@ -27,6 +33,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "eam64.hpp"
 #include "fe_internal.hpp"
@@ -65,6 +72,55 @@ __device__ __forceinline__ void pair_vec(const LatDev& S, const Site& a, const S
     if (S.periodic[c]) qj = qj - S.box[c] * rint((qj - a.q[c]) * S.ibox[c]);
     d[c] = a.q[c] - qj;
   }
+}
+
+// ---------------------------------------------------------------- FP64 tables
+// Position of r in the tables: interval k, local coordinate u = (r - 1)/0.05 - k;
+// `in` is false outside [1, 6) A, where the closed forms are evaluated.
+struct Tpos {
+  int k;
+  double u;
+  bool in;
+};
+__device__ __forceinline__ Tpos tpos64(double r) {
+  const double x = fma(r, 20.0, -20.0);
+  Tpos p;
+  p.in = x >= 0.0 && x < (double)T64_N;
+  p.k = p.in ? (int)x : 0;
+  p.u = x - (double)p.k;
+  return p;
+}
+__device__ __forceinline__ double tab64_v(const double* c, double u) {
+  double v = c[T64_D];
+#pragma unroll
+  for (int j = T64_D - 1; j >= 0; --j) v = fma(v, u, c[j]);
+  return v;
+}
+__device__ __forceinline__ eam::VD tab64_vd(const double* c, double u) {
+  double v = c[T64_D], d = 0.0;
+#pragma unroll
+  for (int j = T64_D - 1; j >= 0; --j) {
+    d = fma(d, u, v);
+    v = fma(v, u, c[j]);
+  }
+  return {v, 20.0 * d};  // d/dr = 20 d/du
+}
+__device__ __forceinline__ const double* trow(const double* t, int f, int k) {
+  return t + ((size_t)f * T64_N + k) * T64_P;
+}
+__device__ __forceinline__ void stage64(double* lds, const double* __restrict__ g, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+  __syncthreads();
+}
+// 1/sqrt(x) for x in the normal range: the hardware estimate, then two steps
+// of Newton's iteration y <- y (3 - x y^2) / 2 (quadratic convergence; the
+// result is within a few ulp).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
 }
 
 template <int DM>
@@ -192,6 +248,34 @@ __device__ __forceinline__ void pair_point(double r, double Fi, double Fj, doubl
       c = (Fi + Fj) * h.d + ph.d;
     }
   }
+}
+
+// pair_point (analytic) with the exponential functions from the tables t64;
+// `out` is set when r lies outside them (the caller then redoes the pair with
+// the closed forms; the values returned here are discarded)
+template <int KIND>
+__device__ __forceinline__ void pair_point_tab(double r, double Fi, double Fj, const double* t64, double& phi,
+                                               double& c, bool& out) {
+  if (KIND == 0) {
+    pair_point<A2C_DERIV_ANALYTIC, 0>(r, Fi, Fj, 0.0, phi, c);
+    return;
+  }
+  const Tpos p = tpos64(r);
+  out |= !p.in;
+  const eam::VD h = tab64_vd(trow(t64, T64_FH, p.k), p.u);
+  const eam::VD ph = tab64_vd(trow(t64, KIND == 3 ? T64_PHH : T64_PHPDH, p.k), p.u);
+  phi = ph.v;
+  if (KIND == 3) {
+    c = (Fi + Fj) * h.d + ph.d;
+  } else {
+    const eam::VD f = eam::f_Pd_vd(r);
+    c = (KIND == 1) ? Fi * h.d + Fj * f.d + ph.d : Fi * f.d + Fj * h.d + ph.d;
+  }
+}
+// f_H from the tables (value)
+__device__ __forceinline__ double f_H_tab(double r, const double* t64) {
+  const Tpos p = tpos64(r);
+  return p.in ? tab64_v(trow(t64, T64_FH, p.k), p.u) : eam::f_H(r);
 }
 
 // Accumulate one neighbor pair: energy (phi summed over points), q-gradient
@@ -324,10 +408,13 @@ __device__ __forceinline__ const Ell& partner_list(const LatDev& S, int list) {
 }
 
 template <int RULE>
-__global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
+__global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l,
+                                                   const double* __restrict__ gtab64) {
+  __shared__ __align__(16) double t64[T64_N * T64_P];  // the f_H rows
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
   if (S.rule != RULE) return;
+  stage64(t64, gtab64 + (size_t)T64_FH * T64_N * T64_P, T64_N * T64_P);
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -361,9 +448,10 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
           double z[3], wk;
           gh_node(k, z, wk);
           const double r0 = d[0] + sd * z[0], r1 = d[1] + sd * z[1], r2 = d[2] + sd * z[2];
-          const double rn = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+          const double q2 = r0 * r0 + r1 * r1 + r2 * r2;
+          const double rn = q2 * rsqrt_nr(q2);
           const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
-          const double fh = (side == 1 || !pd) ? eam::f_H(rn) : 0.0;
+          const double fh = (side == 1 || !pd) ? f_H_tab(rn, t64) : 0.0;
           mine += wk * (side == 0 ? fpd : fh);
           theirs += wk * (pd ? fpd : fh);
         }
@@ -373,9 +461,10 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
         for (int p = 0; p < 12; ++p) {
           double r[3];
           point_r(p, d, ai, aj, r);
-          const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+          const double q2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+          const double rn = q2 * rsqrt_nr(q2);
           const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
-          const double fh = (side == 1 || !pd) ? eam::f_H(rn) : 0.0;
+          const double fh = (side == 1 || !pd) ? f_H_tab(rn, t64) : 0.0;
           mine += side == 0 ? fpd : fh;
           theirs += pd ? fpd : fh;
         }
@@ -422,7 +511,7 @@ __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ l
 // six points; the partner: the six points that displace it, seen from its
 // side with r -> -r).
 template <int KIND, int RULE>
-__device__ __forceinline__ void pair_both(const double d[3], double si, double sj, double Fi, double Fj,
+__device__ __forceinline__ void pair_both_closed(const double d[3], double si, double sj, double Fi, double Fj,
                                           double& pe, double g[3], double& gsi, double& gsj) {
   pe = 0.0;
   g[0] = g[1] = g[2] = 0.0;
@@ -435,11 +524,12 @@ __device__ __forceinline__ void pair_both(const double d[3], double si, double s
       double z[3], w;
       gh_node(k, z, w);
       const double r[3] = {d[0] + sd * z[0], d[1] + sd * z[1], d[2] + sd * z[2]};
-      const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      const double q2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double ri = rsqrt_nr(q2);
       double phi, c;
-      pair_point<A2C_DERIV_ANALYTIC, KIND>(rn, Fi, Fj, 0.0, phi, c);
+      pair_point<A2C_DERIV_ANALYTIC, KIND>(q2 * ri, Fi, Fj, 0.0, phi, c);
       pe += w * phi;
-      const double u = w * c / rn;
+      const double u = w * c * ri;
       g[0] += u * r[0];
       g[1] += u * r[1];
       g[2] += u * r[2];
@@ -454,11 +544,12 @@ __device__ __forceinline__ void pair_both(const double d[3], double si, double s
   for (int p = 0; p < 12; ++p) {
     double r[3];
     point_r(p, d, ai, aj, r);
-    const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const double q2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const double ri = rsqrt_nr(q2);
     double phi, c;
-    pair_point<A2C_DERIV_ANALYTIC, KIND>(rn, Fi, Fj, 0.0, phi, c);
+    pair_point<A2C_DERIV_ANALYTIC, KIND>(q2 * ri, Fi, Fj, 0.0, phi, c);
     pe += phi;
-    const double u = c / rn;
+    const double u = c * ri;
     g[0] += u * r[0];
     g[1] += u * r[1];
     g[2] += u * r[2];
@@ -468,11 +559,68 @@ __device__ __forceinline__ void pair_both(const double d[3], double si, double s
   }
 }
 
+template <int KIND, int RULE>
+__device__ __forceinline__ void pair_both(const double d[3], double si, double sj, double Fi, double Fj,
+                                          const double* t64, double& pe, double g[3], double& gsi, double& gsj) {
+  pe = 0.0;
+  g[0] = g[1] = g[2] = 0.0;
+  gsi = gsj = 0.0;
+  bool out = false;  // some point outside the tables' [1, 6) A
+  if (RULE == GH3) {  // both ends move every node: dr/dsigma_k = z sigma_k / s
+    const double sd = sqrt(si * si + sj * sj);
+    double sz = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 27; ++k) {
+      double z[3], w;
+      gh_node(k, z, w);
+      const double r[3] = {d[0] + sd * z[0], d[1] + sd * z[1], d[2] + sd * z[2]};
+      const double q2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double ri = rsqrt_nr(q2);
+      double phi, c;
+      pair_point_tab<KIND>(q2 * ri, Fi, Fj, t64, phi, c, out);
+      pe += w * phi;
+      const double u = w * c * ri;
+      g[0] += u * r[0];
+      g[1] += u * r[1];
+      g[2] += u * r[2];
+      sz += u * (r[0] * z[0] + r[1] * z[1] + r[2] * z[2]);
+    }
+    gsi = sz * (si / sd);
+    gsj = sz * (sj / sd);
+    if (out) pair_both_closed<KIND, RULE>(d, si, sj, Fi, Fj, pe, g, gsi, gsj);
+    return;
+  }
+  const double ai = SQ6 * si, aj = SQ6 * sj;
+#pragma unroll 1  // rolled: the FP64 closed forms keep ~140 VGPRs live (3 waves per SIMD) instead of ~260 (1)
+  for (int p = 0; p < 12; ++p) {
+    double r[3];
+    point_r(p, d, ai, aj, r);
+    const double q2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const double ri = rsqrt_nr(q2);
+    double phi, c;
+    pair_point_tab<KIND>(q2 * ri, Fi, Fj, t64, phi, c, out);
+    pe += phi;
+    const double u = c * ri;
+    g[0] += u * r[0];
+    g[1] += u * r[1];
+    g[2] += u * r[2];
+    const int ax = (p >> 1) % 3;
+    if (p < 6) gsi += ((p & 1) ? -u : u) * r[ax];
+    else gsj += ((p & 1) ? u : -u) * r[ax];
+  }
+  if (out) pair_both_closed<KIND, RULE>(d, si, sj, Fi, Fj, pe, g, gsi, gsj);
+}
+
+
+// 3 waves per SIMD (at most 168 VGPRs per lane)
 template <int RULE>
-__global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l) {
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_grad_own64(
+    const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l, const double* __restrict__ gtab64) {
+  __shared__ __align__(16) double t64[T64_NF * T64_N * T64_P];
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
   if (S.rule != RULE) return;
+  stage64(t64, gtab64, T64_NF * T64_N * T64_P);
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -502,10 +650,10 @@ __global__ __launch_bounds__(TB) void k_grad_own64(const LatDev* __restrict__ la
       pair_vec(S, a, b, d);
       const double Fj = Fp[j];
       double pe, pg[3], psi, psj;
-      if (list == 0) pair_both<0, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
-      else if (list == 1) pair_both<1, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
-      else if (list == 2) pair_both<2, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
-      else pair_both<3, RULE>(d, a.s, b.s, Fi, Fj, pe, pg, psi, psj);
+      if (list == 0) pair_both<0, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
+      else if (list == 1) pair_both<1, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
+      else if (list == 2) pair_both<2, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
+      else pair_both<3, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
       const double w = side == 0 ? 1.0 : xo[jl];
       e += w * pe;
       g0 += w * pg[0]; g1 += w * pg[1]; g2 += w * pg[2];
@@ -623,12 +771,12 @@ int run_eval64(const Launch& L, Timing* tm) {
   if (L.deriv_mode == A2C_DERIV_ANALYTIC && !L.full_pairs64) {
     if (L.nblk > 0) {
       if (tm) tm->mark(L.stream, "dens_own64");
-      if (per_rule<DensOwn>(L, L.onblk, L.lats, L.ob2l)) return A2C_EHIP;
+      if (per_rule<DensOwn>(L, L.onblk, L.lats, L.ob2l, L.tab64)) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "dens_recv64");
       hipLaunchKernelGGL(k_dens_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
       if (hipGetLastError() != hipSuccess) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "grad_own64");
-      if (per_rule<GradOwn>(L, L.onblk, L.lats, L.ob2l)) return A2C_EHIP;
+      if (per_rule<GradOwn>(L, L.onblk, L.lats, L.ob2l, L.tab64)) return A2C_EHIP;
       if (tm) tm->mark(L.stream, "grad_recv64");
       hipLaunchKernelGGL(k_grad_recv64, dim3(L.nblk), dim3(TB), 0, L.stream, L.lats, L.b2l);
       if (hipGetLastError() != hipSuccess) return A2C_EHIP;
@@ -656,6 +804,46 @@ int run_final_sum(const Launch& L) {
   if (L.nlat == 0) return A2C_OK;
   hipLaunchKernelGGL(k_final_sum, dim3(L.nlat), dim3(1024), 0, L.stream, L.lats);
   return hipGetLastError() == hipSuccess ? A2C_OK : A2C_EHIP;
+}
+
+// FP64 tables of the exponential functions: on each 0.05 A interval the
+// degree-8 polynomial interpolating the long-double closed form at the 9
+// Chebyshev points, in Newton form, expanded to monomials in u (all in long
+// double); the switch radius 5.35 A is an interval boundary.
+void host_tables64(std::vector<double>& t) {
+  static const std::vector<double> built = [] {
+    using LD = long double;
+    std::vector<double> out((size_t)T64_NF * T64_N * T64_P, 0.0);
+    auto fill = [&](int f, auto g) {
+      constexpr int D = T64_D;
+      const LD H = 0.05L, PI = 3.14159265358979323846264338327950288L;
+      for (int k = 0; k < T64_N; ++k) {
+        const LD a = 1.0L + H * k;
+        LD u[D + 1], c[D + 1], m[D + 1] = {};
+        for (int i = 0; i <= D; ++i) {
+          u[i] = 0.5L - 0.5L * cosl((2 * i + 1) * PI / (2.0L * (D + 1)));
+          c[i] = g(a + u[i] * H);
+        }
+        for (int j = 1; j <= D; ++j)  // divided differences
+          for (int i = D; i >= j; --i) c[i] = (c[i] - c[i - 1]) / (u[i] - u[i - j]);
+        for (int i = D; i >= 0; --i) {  // m <- m (u - u_i) + c_i
+          LD n[D + 1] = {};
+          for (int q = 0; q < D; ++q) {
+            n[q + 1] += m[q];
+            n[q] -= m[q] * u[i];
+          }
+          n[0] += c[i];
+          for (int q = 0; q <= D; ++q) m[q] = n[q];
+        }
+        for (int q = 0; q <= D; ++q) out[((size_t)f * T64_N + k) * T64_P + q] = (double)m[q];
+      }
+    };
+    fill(T64_FH, [](LD r) { return eam::f_H<LD>(r); });
+    fill(T64_PHPDH, [](LD r) { return eam::phi_morse<0, LD>(r); });
+    fill(T64_PHH, [](LD r) { return eam::phi_morse<1, LD>(r); });
+    return out;
+  }();
+  t = built;
 }
 
 bool mixed_full_pairs() {
@@ -688,6 +876,7 @@ static Launch single_launch(a2c_fe_ctx* c, bool fp64) {
   L.deriv_mode = c->desc.deriv_mode;
   L.fd_eps = c->desc.fd_eps;
   L.tab = c->dtab;
+  L.tab64 = c->dtab64;
   return L;
 }
 

@@ -158,6 +158,7 @@ struct a2c_fe_ctx {
 
   // mixed-precision tables + FP32 mirrors
   float4* dtab = nullptr;    // cubic coefficient tables (5 x 500 intervals)
+  double* dtab64 = nullptr;  // FP64 polynomial tables of the exponential functions
   float2* daux = nullptr;    // per site {F'(rho), x} in FP32
 
   // launch descriptors: this context is a batch of one lattice
@@ -213,6 +214,7 @@ struct Launch {
   int deriv_mode;
   double fd_eps;
   const float4* tab;  // mixed path
+  const double* tab64 = nullptr;  // FP64 pair-owner passes
   const int32_t* ob2l = nullptr;  // pair-owner workgroup -> lattice
   int onblk = 0;                  // pair-owner workgroups
   bool mixed_full = false;    // mixed path: every pair from both ends (A/B reference)
@@ -232,6 +234,13 @@ int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool 
 int build_tables(a2c_fe_ctx* c);                        // eval32.hip
 int tables_download(a2c_fe_ctx* c, float* out);         // 5*500 float4
 void host_tables(std::vector<float4>& t);              // FP32 cubic tables (host)
+// FP64 tables of the exponential EAM functions (f_H, phi_PdH, phi_H): per
+// function T64_N intervals of 0.05 A over [1, 6) A, each a polynomial of
+// degree T64_D in the local coordinate u in [0, 1), T64_P doubles per row
+// (coefficients c0..c8, one pad).
+constexpr int T64_N = 100, T64_D = 8, T64_P = 10, T64_NF = 3;
+enum { T64_FH = 0, T64_PHPDH = 1, T64_PHH = 2 };
+void host_tables64(std::vector<double>& t);
 int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, double* Xout, a2c_lmvm_report* rep);
 int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h,
                 double gamma_bd, double* dgam, int32_t* dfull, double* dcorr);  // occupancy.hip

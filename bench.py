@@ -53,6 +53,7 @@ def parse():
     p.add_argument("--dmd-steps", type=int, default=100)
     p.add_argument("--dmd-cells", type=int, default=64)
     p.add_argument("--fp64-sample", type=int, default=8, help="batched: lattices relaxed one at a time in FP64")
+    p.add_argument("--lmvm-m", type=int, default=5, help="batched: stored pairs of the LMVM (TAO's default 5)")
     return p.parse_args()
 
 
@@ -118,7 +119,7 @@ def batched(args):
                                        x=(max(xb - 0.05, 1e-3), xb + 0.05))
         specs.append(dict(X=X, x=x, box=box, periodic=(1, 1, 1), n_pd=npd, n_h=nh, T=T))
     sites = sum(s["n_pd"] + s["n_h"] for s in specs)
-    opts = dict(gatol=1e-8, grtol=0.0)
+    opts = dict(gatol=1e-8, grtol=0.0, m=args.lmvm_m)
     log("batched: %d lattices, %d sites" % (len(specs), sites))
     with A.Batch(precision="mixed", max_lattices=len(specs)) as B:
         t0 = time.perf_counter()
@@ -148,7 +149,7 @@ def batched(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32-bulk/f64-reduce", "data": "synthetic",
         "config": {"workload": "64 lattices, edge U{8..24} cells, T U(300,700) K, x U(0.01,0.6)+-0.05, "
                                "disp N(0,0.02), sigma 0.10, periodic; lockstep BLMVM to ||Pg||<=1e-8",
-                   "lattices": len(specs), "sites": sites},
+                   "lattices": len(specs), "sites": sites, "lmvm_m": args.lmvm_m},
         "converged": conv, "admission_s": t_adm, "relax_s": t_relax,
         "iterations": [r["iterations"] for r in reps], "evals": sum(r["evals"] for r in reps),
         "evals_fp64": sum(r["evals_fp64"] for r in reps),
