@@ -172,6 +172,7 @@ def _load():
         "a2c_lattice_fcc_pdh": (I, [P(LatticeDesc), dp, dp, dp]),
         "a2c_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "a2c_eam_tables": (I, [P(C.c_float), C.c_int32]),
+        "a2c_eam_tables64": (I, [dp, C.c_int32]),
         "a2c_enforce_boundary_conditions": (I, [C.c_int32, ip, D, D, dp, dp, ip]),
         "a2c_macro_results": (I, [C.c_int32, dp, C.c_int32, ip, dp, dp]),
         "a2c_read_results": (I, [C.c_char_p, C.c_int32, C.c_int32, D, D, dp, dp, dp, ip]),
@@ -251,6 +252,16 @@ def lattice(n, a=3.89, seed=1, lattice_id=0, disp_std=0.02, sigma=(0.10, 0.10),
     if rc != 0:
         raise A2CError(rc, "bad lattice descriptor")
     return X, xo, box, nper, nper
+
+
+def eam_tables64():
+    """FP64 polynomial tables of the FP64 pair-owner passes: array (3, 100, 10);
+    functions f_H, phi_PdH, phi_H; interval k is [1 + 0.05 k, 1 + 0.05 (k+1)),
+    value sum_j c_j u^j (j <= 8), u = (r - 1)/0.05 - k."""
+    n = lib.a2c_eam_tables64(None, 0)
+    out = np.zeros(n)
+    lib.a2c_eam_tables64(_dptr(out), n)
+    return out.reshape(3, n // 30, 10)
 
 
 def eam_tables():

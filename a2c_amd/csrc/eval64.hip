@@ -32,6 +32,7 @@
 // per-site kernels apply the rule's factors (1/12 and sqrt(6), or 1).
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -920,3 +921,12 @@ int sync_lattice(a2c_fe_ctx* c, const double* dX, double* dG, double* dF) {
 }
 
 }  // namespace a2c
+
+// Host-only: the FP64 tables of the FP64 pair-owner passes (layout in a2c_fe.h).
+extern "C" int a2c_eam_tables64(double* out, int32_t cap) {
+  std::vector<double> t;
+  a2c::host_tables64(t);
+  const int n = (int)t.size();
+  if (out && cap >= n) std::memcpy(out, t.data(), sizeof(double) * n);
+  return n;
+}

@@ -99,6 +99,32 @@ def test_mixed_tables_reproduce_the_closed_forms():
         assert np.max(np.abs(dv - ed)) <= 5e-6 * np.abs(ed).max(), t
 
 
+def test_fp64_tables_reproduce_the_closed_forms():
+    """The FP64 polynomial tables of the FP64 pair-owner passes against the
+    oracle's FP64 closed forms and analytic derivatives over the whole table
+    range, including both sides of the 5.35 A switch radius: values and
+    slopes to 1e-13 of the function's largest magnitude (the FP64 rounding
+    of the closed forms is ~1e-16 for these exponentials)."""
+    import oracle as O
+    tab = A.eam_tables64()
+    assert tab.shape == (3, 100, 10) and np.all(tab[:, :, 9] == 0.0)
+    fns = [O.RHO_H, O.PHI_PDH, O.PHI_H]
+    r = np.linspace(1.0, 6.0, 20001)[:-1]
+    k = np.minimum(np.floor((r - 1.0) / 0.05).astype(int), 99)
+    u = (r - 1.0) / 0.05 - k
+    for t, fn in enumerate(fns):
+        c = tab[t, k]
+        v = np.zeros_like(r)
+        d = np.zeros_like(r)
+        for j in range(8, -1, -1):
+            d = d * u + v
+            v = v * u + c[:, j]
+        d *= 20.0
+        ev, ed = O.eam(fn, r), O.eam_d(fn, r)
+        assert np.max(np.abs(v - ev)) <= 1e-13 * np.abs(ev).max(), t
+        assert np.max(np.abs(d - ed)) <= 1e-13 * np.abs(ed).max(), t
+
+
 def test_serving_request_stream_is_seeded():
     from a2c_amd import serve
     a = serve.make_requests(20, rate=5.0)
