@@ -255,13 +255,14 @@ def lattice(n, a=3.89, seed=1, lattice_id=0, disp_std=0.02, sigma=(0.10, 0.10),
 
 
 def eam_tables64():
-    """FP64 polynomial tables of the FP64 pair-owner passes: array (3, 100, 10);
-    functions f_H, phi_PdH, phi_H; interval k is [1 + 0.05 k, 1 + 0.05 (k+1)),
-    value sum_j c_j u^j (j <= 8), u = (r - 1)/0.05 - k."""
+    """FP64 polynomial tables of the FP64 pair-owner passes: (exp, pd) with
+    exp an array (3, 100, 10) for f_H, phi_PdH, phi_H (degree 8, one pad) and
+    pd an array (2, 100, 12) for f_Pd, phi_Pd (degree 11); interval k is
+    [1 + 0.05 k, 1 + 0.05 (k+1)), value sum_j c_j u^j, u = (r - 1)/0.05 - k."""
     n = lib.a2c_eam_tables64(None, 0)
     out = np.zeros(n)
     lib.a2c_eam_tables64(_dptr(out), n)
-    return out.reshape(3, n // 30, 10)
+    return out[:3000].reshape(3, 100, 10), out[3000:].reshape(2, 100, 12)
 
 
 def eam_tables():

@@ -94,7 +94,7 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
             hipHostMalloc(&c->hX, 32 * N) == hipSuccess && hipHostMalloc(&c->hG, 32 * N) == hipSuccess &&
             hipHostMalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && hipMalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
             hipMalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess &&
-            hipMalloc(&c->dtab64, sizeof(double) * T64_NF * T64_N * T64_P) == hipSuccess;
+            hipMalloc(&c->dtab64, sizeof(double) * T64_SIZE) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
   {
     std::vector<double> t64;

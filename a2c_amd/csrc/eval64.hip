@@ -10,13 +10,15 @@
 // Analytic mode evaluates each unordered pair once (k_dens_own64 ->
 // k_dens_recv64 -> k_grad_own64 -> k_grad_recv64): the pair's owner computes
 // both ends' shares and writes the partner's share into the partner's
-// received slot, as the mixed path does.  These two owner kernels take the
-// exponential functions (f_H, phi_PdH, phi_H) from FP64 polynomial tables
-// in LDS (degree 8 on 0.05 A intervals, built in long double from the closed
-// forms: relative error below 1e-13 for values and slopes, under the FP64
-// rounding of the reference's own product polynomials) instead of three FP64
-// exp per quadrature point, and 1/|r| from the hardware estimate refined by
-// two Newton steps; the Pd polynomials stay closed forms.  FD mode keeps the per-site
+// received slot, as the mixed path does.  These two owner kernels take EAM
+// functions from FP64 polynomial tables in LDS (on 0.05 A intervals,
+// interpolating the long-double closed forms): the exponential functions
+// (f_H, phi_PdH, phi_H) at degree 8, within 1e-13 of the closed forms in
+// value and slope, instead of three FP64 exp per quadrature point; the Pd
+// product polynomials f_Pd (density pass) and phi_Pd (gradient pass) at
+// degree 11, within the ~2e-11 rounding noise of the FP64 evaluation of
+// those polynomials themselves.  f_Pd' keeps its closed form.  1/|r| comes
+// from the hardware estimate refined by two Newton steps.  FD mode keeps the per-site
 // full-list loop nest of the reference (every pair from both ends), since
 // its derivatives are the reference's central differences per site.
 //
@@ -34,6 +36,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "eam64.hpp"
@@ -108,6 +111,27 @@ __device__ __forceinline__ eam::VD tab64_vd(const double* c, double u) {
 }
 __device__ __forceinline__ const double* trow(const double* t, int f, int k) {
   return t + ((size_t)f * T64_N + k) * T64_P;
+}
+// rows of the Pd polynomials (second part of the tables)
+__device__ __forceinline__ const double* qrow(const double* t, int f, int k) {
+  return t + T64_PART1 + ((size_t)f * T64_N + k) * T64Q_P;
+}
+template <int D>
+__device__ __forceinline__ double tabq_v(const double* c, double u) {
+  double v = c[D];
+#pragma unroll
+  for (int j = D - 1; j >= 0; --j) v = fma(v, u, c[j]);
+  return v;
+}
+template <int D>
+__device__ __forceinline__ eam::VD tabq_vd(const double* c, double u) {
+  double v = c[D], d = 0.0;
+#pragma unroll
+  for (int j = D - 1; j >= 0; --j) {
+    d = fma(d, u, v);
+    v = fma(v, u, c[j]);
+  }
+  return {v, 20.0 * d};
 }
 __device__ __forceinline__ void stage64(double* lds, const double* __restrict__ g, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
@@ -257,12 +281,20 @@ __device__ __forceinline__ void pair_point(double r, double Fi, double Fj, doubl
 template <int KIND>
 __device__ __forceinline__ void pair_point_tab(double r, double Fi, double Fj, const double* t64, double& phi,
                                                double& c, bool& out) {
-  if (KIND == 0) {
-    pair_point<A2C_DERIV_ANALYTIC, 0>(r, Fi, Fj, 0.0, phi, c);
-    return;
-  }
   const Tpos p = tpos64(r);
   out |= !p.in;
+  // f_Pd' stays a closed form here: a second 96-byte row per point makes the
+  // pass LDS-bound (measured: 1.83 ms instead of 1.45 ms at 512,000 sites),
+  // while phi_Pd from the table keeps the pair energy free of the FP64
+  // rounding noise of the product polynomial (the relaxation endgame then
+  // needs ~20% fewer FP64 evaluations).
+  if (KIND == 0) {
+    const eam::VD f = eam::f_Pd_vd(r);
+    const eam::VD ph = tabq_vd<T64Q_D>(qrow(t64, T64Q_PHPD, p.k), p.u);
+    phi = ph.v;
+    c = (Fi + Fj) * f.d + ph.d;
+    return;
+  }
   const eam::VD h = tab64_vd(trow(t64, T64_FH, p.k), p.u);
   const eam::VD ph = tab64_vd(trow(t64, KIND == 3 ? T64_PHH : T64_PHPDH, p.k), p.u);
   phi = ph.v;
@@ -273,10 +305,14 @@ __device__ __forceinline__ void pair_point_tab(double r, double Fi, double Fj, c
     c = (KIND == 1) ? Fi * h.d + Fj * f.d + ph.d : Fi * f.d + Fj * h.d + ph.d;
   }
 }
-// f_H from the tables (value)
+// f_H and f_Pd from the tables (values); closed forms outside [1, 6) A
 __device__ __forceinline__ double f_H_tab(double r, const double* t64) {
   const Tpos p = tpos64(r);
   return p.in ? tab64_v(trow(t64, T64_FH, p.k), p.u) : eam::f_H(r);
+}
+__device__ __forceinline__ double f_Pd_tab(double r, const double* t64) {
+  const Tpos p = tpos64(r);
+  return p.in ? tabq_v<T64Q_D>(qrow(t64, T64Q_FPD, p.k), p.u) : eam::f_Pd(r);
 }
 
 // Accumulate one neighbor pair: energy (phi summed over points), q-gradient
@@ -411,11 +447,14 @@ __device__ __forceinline__ const Ell& partner_list(const LatDev& S, int list) {
 template <int RULE>
 __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l,
                                                    const double* __restrict__ gtab64) {
-  __shared__ __align__(16) double t64[T64_N * T64_P];  // the f_H rows
+  // the table layout with only the rows this pass reads staged: f_H (part
+  // one, first function) and f_Pd (part two, first function)
+  __shared__ __align__(16) double t64[T64_PART1 + T64_N * T64Q_P];
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
   if (S.rule != RULE) return;
-  stage64(t64, gtab64 + (size_t)T64_FH * T64_N * T64_P, T64_N * T64_P);
+  for (int i = threadIdx.x; i < T64_N * T64_P; i += blockDim.x) t64[i] = gtab64[(size_t)T64_FH * T64_N * T64_P + i];
+  stage64(t64 + T64_PART1, gtab64 + T64_PART1 + (size_t)T64Q_FPD * T64_N * T64Q_P, T64_N * T64Q_P);
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -451,7 +490,7 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
           const double r0 = d[0] + sd * z[0], r1 = d[1] + sd * z[1], r2 = d[2] + sd * z[2];
           const double q2 = r0 * r0 + r1 * r1 + r2 * r2;
           const double rn = q2 * rsqrt_nr(q2);
-          const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
+          const double fpd = (side == 0 || pd) ? f_Pd_tab(rn, t64) : 0.0;
           const double fh = (side == 1 || !pd) ? f_H_tab(rn, t64) : 0.0;
           mine += wk * (side == 0 ? fpd : fh);
           theirs += wk * (pd ? fpd : fh);
@@ -464,7 +503,7 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
           point_r(p, d, ai, aj, r);
           const double q2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
           const double rn = q2 * rsqrt_nr(q2);
-          const double fpd = (side == 0 || pd) ? eam::f_Pd(rn) : 0.0;
+          const double fpd = (side == 0 || pd) ? f_Pd_tab(rn, t64) : 0.0;
           const double fh = (side == 1 || !pd) ? f_H_tab(rn, t64) : 0.0;
           mine += side == 0 ? fpd : fh;
           theirs += pd ? fpd : fh;
@@ -617,11 +656,11 @@ __device__ __forceinline__ void pair_both(const double d[3], double si, double s
 template <int RULE>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_grad_own64(
     const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l, const double* __restrict__ gtab64) {
-  __shared__ __align__(16) double t64[T64_NF * T64_N * T64_P];
+  __shared__ __align__(16) double t64[T64_SIZE];
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
   if (S.rule != RULE) return;
-  stage64(t64, gtab64, T64_NF * T64_N * T64_P);
+  stage64(t64, gtab64, T64_SIZE);
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
@@ -807,16 +846,17 @@ int run_final_sum(const Launch& L) {
   return hipGetLastError() == hipSuccess ? A2C_OK : A2C_EHIP;
 }
 
-// FP64 tables of the exponential functions: on each 0.05 A interval the
-// degree-8 polynomial interpolating the long-double closed form at the 9
-// Chebyshev points, in Newton form, expanded to monomials in u (all in long
-// double); the switch radius 5.35 A is an interval boundary.
+// FP64 tables: on each 0.05 A interval the polynomial (degree 8 for the
+// exponential functions, 11 for the Pd polynomials) interpolating the
+// long-double closed form at the Chebyshev points, in Newton form, expanded
+// to monomials in u (all in long double); the switch radii 5.0, 5.05 and
+// 5.35 A are interval boundaries.
 void host_tables64(std::vector<double>& t) {
   static const std::vector<double> built = [] {
     using LD = long double;
-    std::vector<double> out((size_t)T64_NF * T64_N * T64_P, 0.0);
-    auto fill = [&](int f, auto g) {
-      constexpr int D = T64_D;
+    std::vector<double> out((size_t)T64_SIZE, 0.0);
+    auto fill = [&](auto deg, size_t base, int pitch, auto g) {
+      constexpr int D = decltype(deg)::value;
       const LD H = 0.05L, PI = 3.14159265358979323846264338327950288L;
       for (int k = 0; k < T64_N; ++k) {
         const LD a = 1.0L + H * k;
@@ -836,12 +876,16 @@ void host_tables64(std::vector<double>& t) {
           n[0] += c[i];
           for (int q = 0; q <= D; ++q) m[q] = n[q];
         }
-        for (int q = 0; q <= D; ++q) out[((size_t)f * T64_N + k) * T64_P + q] = (double)m[q];
+        for (int q = 0; q <= D; ++q) out[base + (size_t)k * pitch + q] = (double)m[q];
       }
     };
-    fill(T64_FH, [](LD r) { return eam::f_H<LD>(r); });
-    fill(T64_PHPDH, [](LD r) { return eam::phi_morse<0, LD>(r); });
-    fill(T64_PHH, [](LD r) { return eam::phi_morse<1, LD>(r); });
+    using D8 = std::integral_constant<int, T64_D>;
+    using D11 = std::integral_constant<int, T64Q_D>;
+    fill(D8{}, (size_t)T64_FH * T64_N * T64_P, T64_P, [](LD r) { return eam::f_H<LD>(r); });
+    fill(D8{}, (size_t)T64_PHPDH * T64_N * T64_P, T64_P, [](LD r) { return eam::phi_morse<0, LD>(r); });
+    fill(D8{}, (size_t)T64_PHH * T64_N * T64_P, T64_P, [](LD r) { return eam::phi_morse<1, LD>(r); });
+    fill(D11{}, T64_PART1 + (size_t)T64Q_FPD * T64_N * T64Q_P, T64Q_P, [](LD r) { return eam::f_Pd<LD>(r); });
+    fill(D11{}, T64_PART1 + (size_t)T64Q_PHPD * T64_N * T64Q_P, T64Q_P, [](LD r) { return eam::phi_Pd<LD>(r); });
     return out;
   }();
   t = built;

@@ -234,12 +234,17 @@ int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool 
 int build_tables(a2c_fe_ctx* c);                        // eval32.hip
 int tables_download(a2c_fe_ctx* c, float* out);         // 5*500 float4
 void host_tables(std::vector<float4>& t);              // FP32 cubic tables (host)
-// FP64 tables of the exponential EAM functions (f_H, phi_PdH, phi_H): per
-// function T64_N intervals of 0.05 A over [1, 6) A, each a polynomial of
-// degree T64_D in the local coordinate u in [0, 1), T64_P doubles per row
-// (coefficients c0..c8, one pad).
+// FP64 tables of the EAM functions: per function T64_N intervals of 0.05 A
+// over [1, 6) A, each a polynomial in the local coordinate u in [0, 1).  Part
+// one: the exponential functions (f_H, phi_PdH, phi_H), degree T64_D, T64_P
+// doubles per row (c0..c8, one pad); part two, after it: the Pd polynomials
+// (f_Pd, phi_Pd), degree T64Q_D, T64Q_P doubles per row (c0..c11).
 constexpr int T64_N = 100, T64_D = 8, T64_P = 10, T64_NF = 3;
 enum { T64_FH = 0, T64_PHPDH = 1, T64_PHH = 2 };
+constexpr int T64Q_D = 11, T64Q_P = 12, T64Q_NF = 2;
+enum { T64Q_FPD = 0, T64Q_PHPD = 1 };
+constexpr int T64_PART1 = T64_NF * T64_N * T64_P;                // doubles
+constexpr int T64_SIZE = T64_PART1 + T64Q_NF * T64_N * T64Q_P;  // doubles
 void host_tables64(std::vector<double>& t);
 int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, double* Xout, a2c_lmvm_report* rep);
 int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h,

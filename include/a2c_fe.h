@@ -199,13 +199,14 @@ int a2c_fe_time_eval(a2c_fe_ctx* ctx, const double* dX, double* dG, double* df,
  * (10000); fills out when cap is large enough. */
 int a2c_eam_tables(float* out, int32_t cap);
 
-/* Host-only (no GPU): the FP64 tables of the FP64 pair-owner passes, 3
- * functions (f_H, phi_PdH, phi_H) x 100 intervals x 10 doubles (polynomial
- * coefficients c0..c8 and one pad); interval k covers r in [1 + 0.05k,
- * 1 + 0.05(k+1)) A, value sum_j c_j u^j with u = (r - 1)/0.05 - k.  Returns
- * the double count (3000); fills out when cap is large enough.  No
- * counterpart in the reference (it evaluates the closed forms of
- * potential.cpp:105-200). */
+/* Host-only (no GPU): the FP64 tables of the FP64 pair-owner passes.  Part
+ * one: 3 functions (f_H, phi_PdH, phi_H) x 100 intervals x 10 doubles
+ * (polynomial coefficients c0..c8 and one pad); part two, after it: 2
+ * functions (f_Pd, phi_Pd) x 100 intervals x 12 doubles (c0..c11).  Interval
+ * k covers r in [1 + 0.05k, 1 + 0.05(k+1)) A, value sum_j c_j u^j with
+ * u = (r - 1)/0.05 - k.  Returns the double count (5400); fills out when cap
+ * is large enough.  No counterpart in the reference (it evaluates the closed
+ * forms of potential.cpp:64-200). */
 int a2c_eam_tables64(double* out, int32_t cap);
 
 #ifdef __cplusplus

@@ -101,28 +101,40 @@ def test_mixed_tables_reproduce_the_closed_forms():
 
 def test_fp64_tables_reproduce_the_closed_forms():
     """The FP64 polynomial tables of the FP64 pair-owner passes against the
-    oracle's FP64 closed forms and analytic derivatives over the whole table
-    range, including both sides of the 5.35 A switch radius: values and
-    slopes to 1e-13 of the function's largest magnitude (the FP64 rounding
-    of the closed forms is ~1e-16 for these exponentials)."""
+    oracle's FP64 closed forms and analytic derivatives over [1, 6) A, on
+    both sides of the switch radii 5.0, 5.05 and 5.35 A (the radii themselves
+    excluded: there the two sides take different branches).  Exponential
+    functions (degree 8): values and slopes within 1e-13 of the function's
+    largest magnitude.  Pd product polynomials (degree 11): within twice the
+    FP64 rounding noise of the closed forms themselves, measured here as
+    their change under a one-ulp change of r (~2e-11 of the largest
+    magnitude near 5 A), plus 1e-13."""
     import oracle as O
-    tab = A.eam_tables64()
-    assert tab.shape == (3, 100, 10) and np.all(tab[:, :, 9] == 0.0)
-    fns = [O.RHO_H, O.PHI_PDH, O.PHI_H]
+    texp, tpd = A.eam_tables64()
+    assert np.all(texp[:, :, 9] == 0.0)
     r = np.linspace(1.0, 6.0, 20001)[:-1]
+    r = r[np.min(np.abs(r[:, None] - np.array([5.0, 5.05, 5.35])[None]), axis=1) > 1e-9]
     k = np.minimum(np.floor((r - 1.0) / 0.05).astype(int), 99)
     u = (r - 1.0) / 0.05 - k
-    for t, fn in enumerate(fns):
-        c = tab[t, k]
-        v = np.zeros_like(r)
-        d = np.zeros_like(r)
-        for j in range(8, -1, -1):
-            d = d * u + v
-            v = v * u + c[:, j]
-        d *= 20.0
-        ev, ed = O.eam(fn, r), O.eam_d(fn, r)
-        assert np.max(np.abs(v - ev)) <= 1e-13 * np.abs(ev).max(), t
-        assert np.max(np.abs(d - ed)) <= 1e-13 * np.abs(ed).max(), t
+    for tab, fns, deg in ((texp, [O.RHO_H, O.PHI_PDH, O.PHI_H], 8), (tpd, [O.RHO_PD, O.PHI_PD], 11)):
+        for t, fn in enumerate(fns):
+            c = tab[t, k]
+            v = np.zeros_like(r)
+            d = np.zeros_like(r)
+            for j in range(deg, -1, -1):
+                d = d * u + v
+                v = v * u + c[:, j]
+            d *= 20.0
+            ev, ed = O.eam(fn, r), O.eam_d(fn, r)
+            if deg == 8:
+                vbar, dbar = 1e-13 * np.abs(ev).max(), 1e-13 * np.abs(ed).max()
+            else:
+                r1 = r * (1.0 + 2.0 ** -52)
+                vbar = 2.0 * np.max(np.abs(O.eam(fn, r1) - ev)) + 1e-13 * np.abs(ev).max()
+                dbar = 2.0 * np.max(np.abs(O.eam_d(fn, r1) - ed)) + 1e-13 * np.abs(ed).max()
+                assert vbar <= 1e-10 * np.abs(ev).max() and dbar <= 1e-10 * np.abs(ed).max()
+            assert np.max(np.abs(v - ev)) <= vbar, (deg, t)
+            assert np.max(np.abs(d - ed)) <= dbar, (deg, t)
 
 
 def test_serving_request_stream_is_seeded():
