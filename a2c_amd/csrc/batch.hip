@@ -1214,9 +1214,9 @@ size_t a2c_batch_footprint(int32_t n_pd, int32_t n_h, int32_t lmvm_m) {
   const size_t N = (size_t)n_pd + n_h;
   // state/gradient/staging 4N doubles x 4, per-site arrays ~5 doubles, ELL
   // lists ~100 int32 per site, the owned/received halves of two pair lists
-  // per site (~28 wide: 2 + 1 int32 per owned/received slot and a 32-byte
-  // received share), optimizer (7 + 2m) x 4N doubles
-  return N * (4 * 32 + 40 + 100 * 4 + 2 * 28 * (3 * 4 + 32) + (7 + 2 * (size_t)std::max(lmvm_m, 1)) * 32) +
+  // per site (~28 wide: 2 int32 per owned slot, a 32-byte share per received
+  // slot), optimizer (7 + 2m) x 4N doubles
+  return N * (4 * 32 + 40 + 100 * 4 + 2 * 28 * (2 * 4 + 32) + (7 + 2 * (size_t)std::max(lmvm_m, 1)) * 32) +
          ((size_t)1 << 20);
 }
 

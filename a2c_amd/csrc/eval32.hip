@@ -432,6 +432,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(RULE == GH3 
   const Site a = load_site(X, s);
   const float ai = SQ6F * (float)a.s;
   const int mytab = pd ? T_FPD : T_FH;
+  const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
   double rho = 0.0;
 #pragma unroll
   for (int side = 0; side < 2; ++side) {  // Pd neighbors, then H neighbors
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(RULE == GH3 
       }
       rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
       const int rv = L.orv[e];
-      if (rv >= 0) reinterpret_cast<float*>(P.rbuf)[(size_t)rv * P.rows + jl] = theirs;
+      if (rv >= 0) reinterpret_cast<float*>(P.rbuf)[(size_t)rv * P.rows + jl] = (float)(wme * (double)theirs);
     }
   }
   rho = site_sum(rho, K);
@@ -477,11 +478,8 @@ __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lat
     const int list = pd ? side : 2 + side;
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const int n = L.rcnt[row];
-    for (int t = 0; t < n; ++t) {
-      const size_t e = (size_t)t * L.rows + row;
-      const int jl = L.rj[e];
-      rho += (side == 0 ? 1.0 : xo[jl]) * (double)reinterpret_cast<const float*>(L.rbuf)[e];
-    }
+    const float* __restrict__ share = reinterpret_cast<const float*>(L.rbuf);
+    for (int t = 0; t < n; ++t) rho += (double)share[(size_t)t * L.rows + row];  // weighted by their senders
   }
   rho *= S.rule == GH3 ? 1.0 : QW;
   S.rho[s] = rho;
@@ -608,6 +606,7 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
   const Site a = load_site(X, s);
   const float ai = SQ6F * (float)a.s;
   const float Fi = aux[s].x;
+  const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
   double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, gs = 0.0;
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
@@ -657,7 +656,8 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
       gs += w * p.gsi;
       const int rv = L.orv[ei];
       if (rv >= 0)
-        reinterpret_cast<float4*>(P.rbuf)[(size_t)rv * P.rows + jl] = make_float4(-p.g[0], -p.g[1], -p.g[2], p.gsj);
+        reinterpret_cast<float4*>(P.rbuf)[(size_t)rv * P.rows + jl] =
+            make_float4((float)(-wme * p.g[0]), (float)(-wme * p.g[1]), (float)(-wme * p.g[2]), (float)(wme * p.gsj));
     }
   }
   e = site_sum(e, K);
@@ -693,13 +693,11 @@ __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lat
       const int list = pd ? side : 2 + side;
       const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
       const int n = L.rcnt[row];
-      for (int t = 0; t < n; ++t) {
-        const size_t ei = (size_t)t * L.rows + row;
-        const int jl = L.rj[ei];
-        const float4 v = reinterpret_cast<const float4*>(L.rbuf)[ei];
-        const double w = side == 0 ? 1.0 : xo[jl];
-        g0 += w * v.x; g1 += w * v.y; g2 += w * v.z;
-        gs += w * v.w;
+      const float4* __restrict__ share = reinterpret_cast<const float4*>(L.rbuf);
+      for (int t = 0; t < n; ++t) {  // shares arrive weighted by their senders
+        const float4 v = share[(size_t)t * L.rows + row];
+        g0 += v.x; g1 += v.y; g2 += v.z;
+        gs += v.w;
       }
     }
     const double xi = pd ? 1.0 : xo[row];

@@ -466,6 +466,7 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
   const int row = pd ? s : s - S.npd;
   const Site a = load_site(X, s);
   const double ai = SQ6 * a.s;
+  const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
   double rho = 0.0;
 #pragma unroll
   for (int side = 0; side < 2; ++side) {  // Pd neighbors, then H neighbors
@@ -511,7 +512,7 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
       }
       rho += (side == 0 ? 1.0 : xo[jl]) * mine;
       const int rv = L.orv[e];
-      if (rv >= 0) share[(size_t)rv * P.rows + jl] = theirs;
+      if (rv >= 0) share[(size_t)rv * P.rows + jl] = wme * theirs;
     }
   }
   rho = site_sum(rho, K);
@@ -535,7 +536,7 @@ __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ l
     const int n = L.rcnt[row];
     for (int t = 0; t < n; ++t) {
       const size_t e = (size_t)t * L.rows + row;
-      rho += (side == 0 ? 1.0 : xo[L.rj[e]]) * share[e];
+      rho += share[e];  // weighted by its sender
     }
   }
   rho *= S.rule == GH3 ? 1.0 : QW;
@@ -673,6 +674,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int row = pd ? s : s - S.npd;
   const Site a = load_site(X, s);
   const double Fi = Fp[s];
+  const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
   double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, gs = 0.0;
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
@@ -699,7 +701,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_
       g0 += w * pg[0]; g1 += w * pg[1]; g2 += w * pg[2];
       gs += w * psi;
       const int rv = L.orv[ei];
-      if (rv >= 0) share[(size_t)rv * P.rows + jl] = make_double4(-pg[0], -pg[1], -pg[2], psj);
+      if (rv >= 0) share[(size_t)rv * P.rows + jl] = make_double4(-wme * pg[0], -wme * pg[1], -wme * pg[2], wme * psj);
     }
   }
   e = site_sum(e, K);
@@ -737,10 +739,9 @@ __global__ __launch_bounds__(TB) void k_grad_recv64(const LatDev* __restrict__ l
       const int n = L.rcnt[row];
       for (int t = 0; t < n; ++t) {
         const size_t ei = (size_t)t * L.rows + row;
-        const double4 v = share[ei];
-        const double w = side == 0 ? 1.0 : xo[L.rj[ei]];
-        g0 += w * v.x; g1 += w * v.y; g2 += w * v.z;
-        gs += w * v.w;
+        const double4 v = share[ei];  // weighted by its sender
+        g0 += v.x; g1 += v.y; g2 += v.z;
+        gs += v.w;
       }
     }
     const double xi = pd ? 1.0 : xo[row];

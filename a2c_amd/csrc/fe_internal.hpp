@@ -31,14 +31,14 @@ struct Ell {
   int32_t* oj = nullptr;    // owidth * rows: neighbor of owned entry t
   int32_t* orv = nullptr;   // owidth * rows: slot of the pair in the partner's received half
   int32_t* rcnt = nullptr;  // rows: received entries
-  int32_t* rj = nullptr;    // rwidth * rows: neighbor of received entry t
-  // Received shares, written by the partner's owner pass and read by this
-  // row's receive pass of the same evaluation: rwidth * rows entries of a
-  // float (density) or float4 (gradient) in the mixed passes, a double or
-  // double4 in the FP64 passes -- one buffer serves all four.  During the
-  // split it holds the map full-row slot -> received slot.
+  // Received shares, written by the partner's owner pass (already weighted
+  // by the partner's occupancy) and summed by this row's receive pass of the
+  // same evaluation: rwidth * rows entries of a float (density) or float4
+  // (gradient) in the mixed passes, a double or double4 in the FP64 passes --
+  // one buffer serves all four.  During the split it holds the map full-row
+  // slot -> received slot.
   void* rbuf = nullptr;
-  int64_t ocap = 0, rcap = 0;  // allocated entries of oj/orv and of rj
+  int64_t ocap = 0;            // allocated entries of oj/orv
   int64_t bcap = 0;            // allocated bytes of rbuf
   int64_t cap = 0;             // allocated entries of idx (grow-only across rebuilds)
 };

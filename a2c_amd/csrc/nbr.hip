@@ -263,7 +263,6 @@ __global__ void k_split(Ell L, int s_off, int j_off) {
       rpos[e] = -1;
       ++no;
     } else {
-      L.rj[(size_t)nr * L.rows + r] = j;
       rpos[e] = nr;
       ++nr;
     }
@@ -456,12 +455,6 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
       A2C_HIP(c, hipMalloc(&L.oj, sizeof(int32_t) * L.ocap));
       A2C_HIP(c, hipMalloc(&L.orv, sizeof(int32_t) * L.ocap));
     }
-    if (rn > L.rcap) {
-      (void)hipFree(L.rj);
-      L.rj = nullptr;
-      L.rcap = rn + 4 * rows;
-      A2C_HIP(c, hipMalloc(&L.rj, sizeof(int32_t) * L.rcap));
-    }
     if (bn > L.bcap) {
       (void)hipFree(L.rbuf);
       L.rbuf = nullptr;
@@ -492,10 +485,10 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
 
 void free_pair_buffers(Ell& L) {
   (void)hipFree(L.ocnt); (void)hipFree(L.rcnt); (void)hipFree(L.oj); (void)hipFree(L.orv);
-  (void)hipFree(L.rj); (void)hipFree(L.rbuf);
-  L.ocnt = L.rcnt = L.oj = L.orv = L.rj = nullptr;
+  (void)hipFree(L.rbuf);
+  L.ocnt = L.rcnt = L.oj = L.orv = nullptr;
   L.rbuf = nullptr;
-  L.ocap = L.rcap = L.bcap = 0;
+  L.ocap = L.bcap = 0;
   L.owidth = L.rwidth = 0;
 }
 
