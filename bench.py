@@ -174,16 +174,17 @@ def dmd(args):
     fe.set_occupancy(x)
     fe.findNeighbors(X)
     # Relaxation tolerance: the per-component RMS gradient that ||g|| <= 1e-8
-    # means for the 6^3-cell lattice (6,912 DOF), i.e. both tolerances scaled
-    # by sqrt(DOF / 6912); an absolute 1e-8 over 8.4 M DOF sits at the FP64
-    # evaluation's own noise floor.
+    # means for the 6^3-cell lattice (6,912 DOF), i.e. gatol scaled by
+    # sqrt(DOF / 6912); an absolute 1e-8 over 8.4 M DOF sits at the FP64
+    # evaluation's own noise floor.  The mixed -> FP64 switch is the library's
+    # automatic one (1e-8 sqrt(sites)).
     # Widths are capped at 0.5 A (relaxed widths are 0.07-0.16 A): the
     # functional decreases without bound as one width grows past a barrier
     # near 1 A (-3 kB x ln sigma), and over hundreds of thousands of packets
     # an uncapped line search eventually steps across it.
     scale = np.sqrt(4.0 * (npd + nh) / 6912.0)
     prm = A.dmd_params(dt=5e-13, nu=1e13, Qm=0.23, xLb=1e-3, xUb=1 - 1e-16, gamma_bd=1e300,
-                       lmvm={"gatol": 1e-8 * scale, "grtol": 0.0, "mixed_gtol": 1e-5 * scale, "sigma_ub": 0.5})
+                       lmvm={"gatol": 1e-8 * scale, "grtol": 0.0, "sigma_ub": 0.5})
     g = np.zeros(nh)
     fh = np.zeros(nh, np.int32)
     h0 = x.sum()
