@@ -23,11 +23,11 @@ namespace {
 
 a2c_tune_table default_table() {
   // a2c_autotune with the default options on one MI355X (ns per site of a
-  // fused evaluation, K = 1/2/4/8: 40^3 cells mixed 1.81/1.66/1.65/1.83, FP64
-  // 6.38/6.27/6.20/6.68; 6^3 cells x 256 mixed 2.06/1.76/1.69/1.88)
+  // fused evaluation, K = 1/2/4/8: 40^3 cells mixed 1.75/1.61/1.60/1.79, FP64 4.50/4.45/4.49/4.90;
+  // 6^3 cells x 256 mixed 2.02/1.71/1.64/1.85, FP64 5.39/4.50/4.51/4.84)
   static const int32_t max_sites[] = {2660, 6604, 15288, 32003, 71831, 170267, 366357, INT_MAX};
-  static const int32_t lanes_mixed[] = {4, 4, 4, 4, 4, 4, 4, 4};
-  static const int32_t lanes_fp64[] = {4, 4, 4, 2, 4, 4, 4, 4};
+  static const int32_t lanes_mixed[] = {4, 4, 4, 4, 4, 4, 2, 4};
+  static const int32_t lanes_fp64[] = {2, 2, 4, 2, 2, 2, 2, 2};
   a2c_tune_table t;
   std::memset(&t, 0, sizeof t);
   t.n_classes = 8;
