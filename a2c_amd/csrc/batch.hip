@@ -69,6 +69,7 @@ __device__ __forceinline__ double metric_inv(const Metric& m, int s) {
 // on dot products need no host round trip:
 //   amode 0: A = a;  1: A = a * p1;  2: A = a * p1 / p2;
 //         3: A = -min(a / p1, 0.05 / sqrt(p2))   (scaled steepest descent)
+//         4: A = a * gamma_g, per coordinate group g (gscal, see group_of)
 //   bmode 0: B = b;  1: B = b * q1;  2: B = b * (q1 - q2)
 struct SegAx {
   double* z;
@@ -79,6 +80,8 @@ struct SegAx {
   int32_t amode, bmode;
   Metric met;
   int32_t scaled;
+  const double* gscal;  // amode 4: {s.y, y.D y} of each group, then p1, p2 hold the totals
+  int32_t npd;          // Pd sites of the slot (coordinate groups)
   int32_t nsite, blk0;
 };
 struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null)
@@ -88,13 +91,29 @@ struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null
   const double* b2;
   Metric met;
   int32_t k1, k2;
+  int32_t group, npd;  // group >= 0: only the components of that coordinate group
   double* out;
   int32_t nsite, blk0;
 };
+
+// Coordinate groups of the scaled initial inverse Hessian: positions and
+// widths of Pd sites, positions and widths of H sites.  Their curvatures
+// differ by orders of magnitude, so the two-loop recursion scales each group
+// by its own gamma_g = s_g . y_g / y_g . D y_g of the newest pair (a group
+// without positive curvature data takes the total's gamma): measured on 5^3
+// and 6^3 lattices, half the iterations of one scalar gamma.
+constexpr int NGROUP = 4;
+__host__ __device__ inline int group_of(int site, int comp, int npd) {
+  return (site >= npd ? 2 : 0) + (comp == 3 ? 1 : 0);
+}
+__host__ __device__ inline double group_gamma(const double* gs, double sy, double yy, int g) {
+  const double a = gs[2 * g], b = gs[2 * g + 1];
+  return (a > 0.0 && b > 0.0) ? a / b : sy / yy;
+}
 // Device scalars per slot: [0, m) the first-loop dots s_i.q of the two-loop
 // recursion, then s.y and y.D y of the newest pair, the second-loop dot, and
 // g.D g, g.D^2 g of scaled steepest descent.
-constexpr int SCAL_PER_SLOT = 72;  // >= m + 5 for m <= 64
+constexpr int SCAL_PER_SLOT = 80;  // >= m + 5 + 2 NGROUP for m <= 64
 constexpr double SIGMA_STEP = 4.0;  // largest factor by which one line-search trial may change a width
 constexpr double Q_STEP = 0.25;     // largest displacement of a site in one line-search trial (A)
 constexpr int MIXED_MAX_TRIALS = 8;  // line-search trials of a mixed-precision iteration before FP64 takes over
@@ -121,6 +140,13 @@ __global__ __launch_bounds__(TB) void k_seg_axpby(const SegAx* __restrict__ ops,
   if (o.bmode == 1) bb *= *o.q1;
   else if (o.bmode == 2) bb *= *o.q1 - *o.q2;
   if (o.scaled) a *= metric_inv(o.met, s);
+  if (o.amode == 4) {
+    const double sy = *o.p1, yy = *o.p2;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      o.z[i + c] = a * group_gamma(o.gscal, sy, yy, group_of(s, c, o.npd)) * o.x[i + c] + bb * o.y[i + c];
+    return;
+  }
 #pragma unroll
   for (int c = 0; c < 4; ++c) o.z[i + c] = a * o.x[i + c] + bb * o.y[i + c];
 }
@@ -135,11 +161,13 @@ __global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, 
     const size_t i = 4 * (size_t)s;
     const double di = metric_inv(o.met, s);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) v1 += o.a1[i + c] * o.b1[i + c];
+    for (int c = 0; c < 4; ++c)
+      if (o.group < 0 || group_of(s, c, o.npd) == o.group) v1 += o.a1[i + c] * o.b1[i + c];
     if (o.k1) v1 *= o.k1 == 1 ? di : di * di;
     if (o.a2) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) v2 += o.a2[i + c] * o.b2[i + c];
+      for (int c = 0; c < 4; ++c)
+        if (o.group < 0 || group_of(s, c, o.npd) == o.group) v2 += o.a2[i + c] * o.b2[i + c];
       if (o.k2) v2 *= o.k2 == 1 ? di : di * di;
     }
   }
@@ -251,7 +279,7 @@ struct TwoLoop {
   double rho[TL_MAX_PAIRS];
   Metric met;
   double lb, ub;
-  int32_t np, nsite;
+  int32_t np, nsite, npd;
 };
 
 // block sum in a fixed order: wave trees, then the 16 wave partials in order
@@ -271,19 +299,31 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restri
   __shared__ double al[TL_MAX_PAIRS];  // A_t = s_t . q of the first loop
   const TwoLoop& o = ops[blockIdx.x];
   const int n = 4 * o.nsite, np = o.np;
-  // q = g; A_0 = s_0 . q; the scale gamma = s_0 . y_0 / y_0 . D y_0 of H0 = gamma D
-  double a = 0.0, sy = 0.0, yy = 0.0;
+  // q = g; A_0 = s_0 . q; per coordinate group g the scale
+  // gamma_g = s_0 . y_0 / y_0 . D y_0 over the group, H0 = diag(gamma_g) D
+  double a = 0.0, gs[2 * NGROUP] = {};
   for (int i = threadIdx.x; i < n; i += TL_THREADS) {
     const double gi = o.g[i], si = o.S[0][i], yi = o.Y[0][i];
+    const int gr = group_of(i >> 2, i & 3, o.npd);
     o.q[i] = gi;
     a += si * gi;
-    sy += si * yi;
-    yy += yi * yi * metric_inv(o.met, i >> 2);
+#pragma unroll
+    for (int k = 0; k < NGROUP; ++k) {
+      gs[2 * k] += k == gr ? si * yi : 0.0;
+      gs[2 * k + 1] += k == gr ? yi * yi * metric_inv(o.met, i >> 2) : 0.0;
+    }
   }
   double A = tl_sum(a, red);
-  const double gamma = tl_sum(sy, red) / tl_sum(yy, red);
+  double sy = 0.0, yy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 2 * NGROUP; ++k) gs[k] = tl_sum(gs[k], red);
+#pragma unroll
+  for (int k = 0; k < NGROUP; ++k) {
+    sy += gs[2 * k];
+    yy += gs[2 * k + 1];
+  }
   if (threadIdx.x == 0) al[0] = A;
-  // first loop, newest to oldest: q -= rho_t A_t y_t; then q = gamma D q
+  // first loop, newest to oldest: q -= rho_t A_t y_t; then q = diag(gamma_g) D q
   for (int t = 0; t < np; ++t) {
     const double c = -o.rho[t] * A;
     const double* yt = o.Y[t];
@@ -292,7 +332,7 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restri
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += TL_THREADS) {
       double qi = o.q[i] + c * yt[i];
-      if (last) qi = gamma * metric_inv(o.met, i >> 2) * qi;
+      if (last) qi = group_gamma(gs, sy, yy, group_of(i >> 2, i & 3, o.npd)) * metric_inv(o.met, i >> 2) * qi;
       o.q[i] = qi;
       acc += nx[i] * qi;
     }
@@ -570,6 +610,8 @@ int seg_axpby(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
     o.scaled = 0;
     o.amode = o.bmode = 0;
     o.p1 = o.p2 = o.q1 = o.q2 = nullptr;
+    o.gscal = nullptr;
+    o.npd = 0;
     fill(o, s);
   };
   SegAx* ops;
@@ -603,6 +645,8 @@ int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<do
     o.a2 = o.b2 = nullptr;
     o.met = Metric{};
     o.k1 = o.k2 = 0;
+    o.group = -1;
+    o.npd = 0;
     fill(o, s);
     o.out = b->dres + 2 * (&s - b->slots.data());
   };
@@ -628,6 +672,8 @@ int seg_dot_dev(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
     o.a2 = o.b2 = nullptr;
     o.met = Metric{};
     o.k1 = o.k2 = 0;
+    o.group = -1;
+    o.npd = 0;
     fill(o, s);
   };
   SegDot* ops;
@@ -870,7 +916,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
       }
       o.met = s.metric();
       o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
-      o.np = s.npairs; o.nsite = s.N();
+      o.np = s.npairs; o.nsite = s.N(); o.npd = s.fe->desc.n_pd;
     }
     const size_t need = (ops.size() * sizeof(TwoLoop) + 255) & ~(size_t)255;
     if (b->arena.used + need > b->arena.cap)
@@ -904,15 +950,20 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
           }))
         return rc;
     }
-    // H0 = gamma D, gamma = s.y / y.D y of the newest pair
-    if (int rc = seg_dot_dev(b, tl, [&](SegDot& o, Slot& s) {
-          const int nw = (s.head - 1 + s.m) % s.m;
-          o.a1 = s.S[nw]; o.b1 = s.Y[nw]; o.a2 = s.Y[nw]; o.b2 = s.Y[nw];
-          o.met = s.metric(); o.k2 = 1; o.out = scal(s) + s.m;
-        }))
-      return rc;
+    // H0 = diag(gamma_g) D, gamma_g = s.y / y.D y of the newest pair over
+    // coordinate group g (the totals, at m and m + 1, stand in for a group
+    // without positive curvature data)
+    for (int g = -1; g < NGROUP; ++g)
+      if (int rc = seg_dot_dev(b, tl, [&](SegDot& o, Slot& s) {
+            const int nw = (s.head - 1 + s.m) % s.m;
+            o.a1 = s.S[nw]; o.b1 = s.Y[nw]; o.a2 = s.Y[nw]; o.b2 = s.Y[nw];
+            o.met = s.metric(); o.k2 = 1; o.group = g; o.npd = s.fe->desc.n_pd;
+            o.out = scal(s) + (g < 0 ? s.m : s.m + 5 + 2 * g);
+          }))
+        return rc;
     if (int rc = seg_axpby(b, tl, [&](SegAx& o, Slot& s) {
-          o.z = s.q; o.x = s.q; o.y = s.q; o.a = 1.0; o.amode = 2; o.p1 = scal(s) + s.m; o.p2 = scal(s) + s.m + 1;
+          o.z = s.q; o.x = s.q; o.y = s.q; o.a = 1.0; o.amode = 4; o.p1 = scal(s) + s.m; o.p2 = scal(s) + s.m + 1;
+          o.gscal = scal(s) + s.m + 5; o.npd = s.fe->desc.n_pd;
           o.b = 0.0; o.met = s.metric(); o.scaled = 1;
         }))
       return rc;

@@ -112,7 +112,9 @@ def test_graph_replay_matches_direct_issue_bitwise(precision):
             out[graphs] = ([B.report(sl) for sl in slots], [B.get_state(sl)[0] for sl in slots], B.graph_stats())
     (r0, X0, st0), (r1, X1, st1) = out[False], out[True]
     assert st0["replays"] == 0 and st0["captures"] == 0
-    assert st1["replays"] > 10 * st1["captures"] > 0, st1  # steady state runs from graphs
+    # steady state runs from graphs: each captured sequence recurs several times
+    assert st1["replays"] > 4 * st1["captures"] > 0, st1
+    assert st1["replays"] > st1["direct"], st1
     for a, b, xa, xb in zip(r0, r1, X0, X1):
         assert a == b and a["reason"] > 0
         assert np.array_equal(xa, xb)
