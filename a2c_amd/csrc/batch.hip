@@ -69,7 +69,6 @@ __device__ __forceinline__ double metric_inv(const Metric& m, int s) {
 // on dot products need no host round trip:
 //   amode 0: A = a;  1: A = a * p1;  2: A = a * p1 / p2;
 //         3: A = -min(a / p1, 0.05 / sqrt(p2))   (scaled steepest descent)
-//         4: A = a * gamma_g, per coordinate group g (gscal, see group_of)
 //   bmode 0: B = b;  1: B = b * q1;  2: B = b * (q1 - q2)
 struct SegAx {
   double* z;
@@ -80,8 +79,6 @@ struct SegAx {
   int32_t amode, bmode;
   Metric met;
   int32_t scaled;
-  const double* gscal;  // amode 4: {s.y, y.D y} of each group, then p1, p2 hold the totals
-  int32_t npd;          // Pd sites of the slot (coordinate groups)
   int32_t nsite, blk0;
 };
 struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null)
@@ -91,7 +88,6 @@ struct SegDot {  // out[0] = a1 . D^k1 b1, out[1] = a2 . D^k2 b2 (a2 may be null
   const double* b2;
   Metric met;
   int32_t k1, k2;
-  int32_t group, npd;  // group >= 0: only the components of that coordinate group
   double* out;
   int32_t nsite, blk0;
 };
@@ -111,19 +107,21 @@ __host__ __device__ inline double group_gamma(const double* gs, double sy, doubl
   return (a > 0.0 && b > 0.0) ? a / b : sy / yy;
 }
 // Device scalars per slot: [0, m) the first-loop dots s_i.q of the two-loop
-// recursion, then s.y and y.D y of the newest pair, the second-loop dot, and
-// g.D g, g.D^2 g of scaled steepest descent.
+// recursion, m + 2 the second-loop dot, m + 3 and m + 4 g.D g and g.D^2 g of
+// scaled steepest descent, [m + 5, m + 5 + 2 NGROUP) s.y and y.D y of the
+// newest pair per coordinate group (m and m + 1 are unused).
 constexpr int SCAL_PER_SLOT = 80;  // >= m + 5 + 2 NGROUP for m <= 64
 constexpr double SIGMA_STEP = 4.0;  // largest factor by which one line-search trial may change a width
 constexpr double Q_STEP = 0.25;     // largest displacement of a site in one line-search trial (A)
 constexpr int MIXED_MAX_TRIALS = 8;  // line-search trials of a mixed-precision iteration before FP64 takes over
 
 struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero the active part of d,
-                   // 3 project a trial point and form the step direction it realises
-  double* z;        // x (0), gp (1), d (2), trial x (3)
-  const double* x;  // x (1, 2), x_old (3)
-  const double* y;  // g (1), d (3)
-  double* w;        // realised direction (3)
+                   // 3 form a trial point x_old + stp d, projected, and the step direction it realises;
+                   // 4 (no bound) copy a point and its gradient: z = x, w = y
+  double* z;        // x (0), gp (1), d (2), trial x (3), copy of x (4)
+  const double* x;  // x (1, 2, 4), x_old (3)
+  const double* y;  // g (1, 4), d (3)
+  double* w;        // realised direction (3), copy of g (4)
   double lb, ub, stp;
   int32_t nsite, blk0;
 };
@@ -140,13 +138,6 @@ __global__ __launch_bounds__(TB) void k_seg_axpby(const SegAx* __restrict__ ops,
   if (o.bmode == 1) bb *= *o.q1;
   else if (o.bmode == 2) bb *= *o.q1 - *o.q2;
   if (o.scaled) a *= metric_inv(o.met, s);
-  if (o.amode == 4) {
-    const double sy = *o.p1, yy = *o.p2;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      o.z[i + c] = a * group_gamma(o.gscal, sy, yy, group_of(s, c, o.npd)) * o.x[i + c] + bb * o.y[i + c];
-    return;
-  }
 #pragma unroll
   for (int c = 0; c < 4; ++c) o.z[i + c] = a * o.x[i + c] + bb * o.y[i + c];
 }
@@ -161,13 +152,11 @@ __global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, 
     const size_t i = 4 * (size_t)s;
     const double di = metric_inv(o.met, s);
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (o.group < 0 || group_of(s, c, o.npd) == o.group) v1 += o.a1[i + c] * o.b1[i + c];
+    for (int c = 0; c < 4; ++c) v1 += o.a1[i + c] * o.b1[i + c];
     if (o.k1) v1 *= o.k1 == 1 ? di : di * di;
     if (o.a2) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (o.group < 0 || group_of(s, c, o.npd) == o.group) v2 += o.a2[i + c] * o.b2[i + c];
+      for (int c = 0; c < 4; ++c) v2 += o.a2[i + c] * o.b2[i + c];
       if (o.k2) v2 *= o.k2 == 1 ? di : di * di;
     }
   }
@@ -214,6 +203,187 @@ __global__ __launch_bounds__(1024) void k_seg_dot_final(const SegDot* __restrict
   }
 }
 
+// block sum in a fixed order: wave trees, then the NT / 64 wave partials in order
+template <int NT>
+__device__ __forceinline__ double bsum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  __syncthreads();  // the previous sum's readers are done with red
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) s += red[w];
+  return s;
+}
+
+// Two-loop recursion of slots too large for one workgroup: one grid pass per
+// recursion step for all such slots at once.  A pass applies the update of
+// its step and forms block partials of the dot product the next step needs
+// (k_seg_final reduces them in a fixed order), so the recursion over np
+// pairs is 2 np + 1 passes, each reading q and two vectors, instead of a
+// dot and an axpy per step.
+struct SegTL {
+  double* q;
+  const double* v;      // pass 0: g (q = g); else the update vector (y_t, then s_t)
+  const double* w;      // dot partner of the updated q (pass 0: s_0)
+  const double* y0;     // pass 0: y_0 (group sums of the initial scale)
+  const double* ca;     // update q += c v, c = rho (*ca - (cb ? *cb : 0))
+  const double* cb;
+  double rho;
+  int32_t mode;         // 0 update; 1 update, then q = diag(gamma_g) D q; 2 final: d = -(updated q),
+                        // widths at an active bound not moved, dot = g . d
+  const double* gscal;  // mode 1: {s.y, y.D y} of the newest pair per coordinate group
+  Metric met;
+  int32_t npd;
+  double* d;            // mode 2
+  const double* g;
+  const double* x;
+  double lb, ub;
+  double* out;          // the reduced dot
+  double* out2;         // pass 0: the 2 NGROUP group sums
+  int32_t nsite, blk0;
+};
+constexpr int TL_PART = 1 + 2 * NGROUP;  // partial sums per block of pass 0
+
+__global__ __launch_bounds__(TB) void k_seg_tl0(const SegTL* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                                double* __restrict__ part, int nblk) {
+  __shared__ double red[TB / 64];
+  const SegTL& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  double a = 0.0, gs[2 * NGROUP] = {};
+  if (s < o.nsite) {
+    const size_t i = 4 * (size_t)s;
+    const double di = metric_inv(o.met, s);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double gi = o.v[i + c], si = o.w[i + c], yi = o.y0[i + c];
+      const int gr = group_of(s, c, o.npd);
+      o.q[i + c] = gi;
+      a += si * gi;
+#pragma unroll
+      for (int k = 0; k < NGROUP; ++k) {
+        gs[2 * k] += k == gr ? si * yi : 0.0;
+        gs[2 * k + 1] += k == gr ? yi * yi * di : 0.0;
+      }
+    }
+  }
+  a = bsum<TB>(a, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+#pragma unroll
+  for (int k = 0; k < 2 * NGROUP; ++k) {
+    const double v = bsum<TB>(gs[k], red);
+    if (threadIdx.x == 0) part[(size_t)(1 + k) * nblk + blockIdx.x] = v;
+  }
+}
+
+__global__ __launch_bounds__(TB) void k_seg_tl(const SegTL* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                               double* __restrict__ part) {
+  __shared__ double red[TB / 64];
+  const SegTL& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  double acc = 0.0;
+  if (s < o.nsite) {
+    const size_t i = 4 * (size_t)s;
+    const double c = o.rho * (*o.ca - (o.cb ? *o.cb : 0.0));
+    if (o.mode == 2) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double di = -(o.q[i + k] + c * o.v[i + k]);
+        if (k == 3) {
+          const double xs = o.x[i + 3];
+          if ((xs <= o.lb && di < 0.0) || (xs >= o.ub && di > 0.0)) di = 0.0;
+        }
+        o.d[i + k] = di;
+        acc += o.g[i + k] * di;
+      }
+    } else {
+      double sy = 0.0, yy = 0.0;
+      if (o.mode == 1) {
+#pragma unroll
+        for (int k = 0; k < NGROUP; ++k) {
+          sy += o.gscal[2 * k];
+          yy += o.gscal[2 * k + 1];
+        }
+      }
+      const double di = metric_inv(o.met, s);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double qi = o.q[i + k] + c * o.v[i + k];
+        if (o.mode == 1) qi = group_gamma(o.gscal, sy, yy, group_of(s, k, o.npd)) * di * qi;
+        o.q[i + k] = qi;
+        acc += o.w[i + k] * qi;
+      }
+    }
+  }
+  acc = bsum<TB>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// The new curvature pair of an accepted step and the projected gradient in
+// one pass: s = x - x_old, y = g - g_old, gp = P g; dots s.y and gp.gp.
+struct SegPair {
+  const double* x;
+  const double* xo;
+  const double* g;
+  const double* go;
+  double* S;
+  double* Y;
+  double* gp;
+  double lb, ub;
+  double* out;   // s.y
+  double* out2;  // gp.gp
+  int32_t nsite, blk0;
+};
+
+__global__ __launch_bounds__(TB) void k_seg_pair(const SegPair* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                                 double* __restrict__ part, int nblk) {
+  __shared__ double red[TB / 64];
+  const SegPair& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  double sy = 0.0, gg = 0.0;
+  if (s < o.nsite) {
+    const size_t i = 4 * (size_t)s;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double gi = o.g[i + c];
+      const double si = o.x[i + c] - o.xo[i + c], yi = gi - o.go[i + c];
+      o.S[i + c] = si;
+      o.Y[i + c] = yi;
+      sy += si * yi;
+      double pi = gi;
+      if (c == 3) {
+        const double xs = o.x[i + 3];
+        if ((xs <= o.lb && gi > 0.0) || (xs >= o.ub && gi < 0.0)) pi = 0.0;
+      }
+      o.gp[i + c] = pi;
+      gg += pi * pi;
+    }
+  }
+  sy = bsum<TB>(sy, red);
+  gg = bsum<TB>(gg, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sy;
+    part[nblk + blockIdx.x] = gg;
+  }
+}
+
+// one block per op: fixed-order sums of the op's NOUT block partials
+// (the first to out, the others to out2)
+template <class Op, int NOUT>
+__global__ __launch_bounds__(256) void k_seg_final(const Op* __restrict__ ops, const double* __restrict__ part,
+                                                   int nblk) {
+  __shared__ double red[4];
+  const Op& o = ops[blockIdx.x];
+  const int n = site_blocks(o.nsite);
+#pragma unroll
+  for (int k = 0; k < NOUT; ++k) {
+    double v = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) v += part[(size_t)k * nblk + o.blk0 + j];
+    v = bsum<256>(v, red);
+    if (threadIdx.x == 0) (k == 0 ? o.out[0] : o.out2[k - 1]) = v;
+  }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ ops, const int32_t* __restrict__ b2l) {
   const SegBound& o = ops[b2l[blockIdx.x]];
@@ -231,6 +401,12 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
   } else if (MODE == 2) {
     const double xs = o.x[i + 3], ds = o.z[i + 3];
     if ((xs <= o.lb && ds < 0.0) || (xs >= o.ub && ds > 0.0)) o.z[i + 3] = 0.0;
+  } else if (MODE == 4) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      o.z[i + c] = o.x[i + c];
+      o.w[i + c] = o.y[i + c];
+    }
   } else {
     // Trial safeguards (they bend the search path, not the fixed point): a
     // site's position moves at most Q_STEP, so that a packet whose metric
@@ -243,19 +419,18 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
     const double d0 = o.y[i], d1 = o.y[i + 1], d2 = o.y[i + 2];
     const double len = o.stp * sqrt(d0 * d0 + d1 * d1 + d2 * d2);
     const double sc = len > Q_STEP ? Q_STEP / len : 1.0;  // a site moves at most Q_STEP per trial
-    o.w[i] = sc * d0;
-    o.w[i + 1] = sc * d1;
-    o.w[i + 2] = sc * d2;
-    if (sc < 1.0) {
-      o.z[i] = o.x[i] + o.stp * o.w[i];
-      o.z[i + 1] = o.x[i + 1] + o.stp * o.w[i + 1];
-      o.z[i + 2] = o.x[i + 2] + o.stp * o.w[i + 2];
-    }
-    const double so = o.x[i + 3];
+    const double w0 = sc * d0, w1 = sc * d1, w2 = sc * d2;
+    o.w[i] = w0;
+    o.w[i + 1] = w1;
+    o.w[i + 2] = w2;
+    o.z[i] = o.x[i] + o.stp * w0;
+    o.z[i + 1] = o.x[i + 1] + o.stp * w1;
+    o.z[i + 2] = o.x[i + 2] + o.stp * w2;
+    const double so = o.x[i + 3], ds = o.y[i + 3];
     const double lo = fmax(o.lb, so * (1.0 / SIGMA_STEP)), hi = fmin(o.ub, so * SIGMA_STEP);
-    const double xs = o.z[i + 3], cs = fmin(fmax(xs, lo), hi);
+    const double xs = so + o.stp * ds, cs = fmin(fmax(xs, lo), hi);
     o.z[i + 3] = cs;
-    o.w[i + 3] = cs == xs ? o.y[i + 3] : (cs - so) / o.stp;
+    o.w[i + 3] = cs == xs ? ds : (cs - so) / o.stp;
   }
 }
 
@@ -282,17 +457,7 @@ struct TwoLoop {
   int32_t np, nsite, npd;
 };
 
-// block sum in a fixed order: wave trees, then the 16 wave partials in order
-__device__ __forceinline__ double tl_sum(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  __syncthreads();  // the previous sum's readers are done with red
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double s = 0.0;
-#pragma unroll
-  for (int w = 0; w < TL_THREADS / 64; ++w) s += red[w];
-  return s;
-}
+__device__ __forceinline__ double tl_sum(double v, double* red) { return bsum<TL_THREADS>(v, red); }
 
 __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restrict__ ops) {
   __shared__ double red[TL_THREADS / 64];
@@ -450,7 +615,7 @@ struct a2c_batch {
   double* dres = nullptr;  // 2 doubles per slot (device dot results read by the host)
   double* hres = nullptr;  // pinned mirror
   double* dscal = nullptr;  // SCAL_PER_SLOT doubles per slot: optimizer scalars that stay on the device
-  double* dpart = nullptr;  // 2 * max blocks
+  double* dpart = nullptr;  // TL_PART * max blocks
   size_t part_cap = 0;      // blocks
   float4* dtab = nullptr;
   double* dtab64 = nullptr;
@@ -565,13 +730,14 @@ int bsync(a2c_batch* b) {
   return A2C_OK;
 }
 
-// make sure the partial-sum buffer holds nblk blocks (x2 for two dots)
+// make sure the partial-sum buffer holds nblk blocks (x TL_PART: up to that
+// many sums per block)
 int ensure_part(a2c_batch* b, size_t nblk) {
   if (nblk <= b->part_cap) return A2C_OK;
   if (int rc = bsync(b)) return rc;
   (void)hipFree(b->dpart);
   b->dpart = nullptr;
-  B_HIP(b, hipMalloc(&b->dpart, sizeof(double) * 2 * nblk));
+  B_HIP(b, hipMalloc(&b->dpart, sizeof(double) * TL_PART * nblk));
   b->part_cap = nblk;
   return A2C_OK;
 }
@@ -610,8 +776,6 @@ int seg_axpby(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
     o.scaled = 0;
     o.amode = o.bmode = 0;
     o.p1 = o.p2 = o.q1 = o.q2 = nullptr;
-    o.gscal = nullptr;
-    o.npd = 0;
     fill(o, s);
   };
   SegAx* ops;
@@ -645,8 +809,6 @@ int seg_dot(a2c_batch* b, const std::vector<int>& ids, Fill fill, std::vector<do
     o.a2 = o.b2 = nullptr;
     o.met = Metric{};
     o.k1 = o.k2 = 0;
-    o.group = -1;
-    o.npd = 0;
     fill(o, s);
     o.out = b->dres + 2 * (&s - b->slots.data());
   };
@@ -672,8 +834,6 @@ int seg_dot_dev(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
     o.a2 = o.b2 = nullptr;
     o.met = Metric{};
     o.k1 = o.k2 = 0;
-    o.group = -1;
-    o.npd = 0;
     fill(o, s);
   };
   SegDot* ops;
@@ -681,6 +841,75 @@ int seg_dot_dev(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
   int nblk;
   if (int rc = seg_launch_prep<SegDot>(b, ids, fill2, &ops, &map, &nblk)) return rc;
   enqueue_dot(b, ops, map, nblk, (unsigned)ids.size());
+  return A2C_OK;
+}
+
+// one pass of the segmented two-loop recursion (first: pass 0) and the
+// fixed-order reduction of its dots
+template <class Fill>
+int seg_tl(a2c_batch* b, const std::vector<int>& ids, bool first, Fill fill) {
+  if (ids.empty()) return A2C_OK;
+  auto fill0 = [&](SegTL& o, Slot& s) {
+    o = SegTL{};
+    fill(o, s);
+  };
+  SegTL* ops;
+  int32_t* map;
+  int nblk;
+  if (int rc = seg_launch_prep<SegTL>(b, ids, fill0, &ops, &map, &nblk)) return rc;
+  double* part = b->dpart;
+  const unsigned nops = (unsigned)ids.size();
+  if (first) {
+    if (nblk)
+      enqueue(b, {8, u64(ops), u64(map), u64(part), (uint64_t)nblk}, [=](hipStream_t st) {
+        hipLaunchKernelGGL(k_seg_tl0, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk);
+      });
+    enqueue(b, {9, u64(ops), u64(part), (uint64_t)nblk, nops}, [=](hipStream_t st) {
+      hipLaunchKernelGGL((k_seg_final<SegTL, TL_PART>), dim3(nops), dim3(256), 0, st, ops, part, nblk);
+    });
+  } else {
+    if (nblk)
+      enqueue(b, {10, u64(ops), u64(map), u64(part), (uint64_t)nblk}, [=](hipStream_t st) {
+        hipLaunchKernelGGL(k_seg_tl, dim3(nblk), dim3(TB), 0, st, ops, map, part);
+      });
+    enqueue(b, {11, u64(ops), u64(part), (uint64_t)nblk, nops}, [=](hipStream_t st) {
+      hipLaunchKernelGGL((k_seg_final<SegTL, 1>), dim3(nops), dim3(256), 0, st, ops, part, nblk);
+    });
+  }
+  return A2C_OK;
+}
+
+// s, y, gp of the accepted step and the dots s.y, gp.gp per slot, returned in
+// out[2k], out[2k + 1] (one round trip)
+int seg_pair(a2c_batch* b, const std::vector<int>& ids, std::vector<double>& out) {
+  out.assign(2 * ids.size(), 0.0);
+  if (ids.empty()) return A2C_OK;
+  auto fill = [&](SegPair& o, Slot& s) {
+    o.x = s.x; o.xo = s.xo; o.g = s.g; o.go = s.go;
+    o.S = s.S[s.head]; o.Y = s.Y[s.head]; o.gp = s.gp;
+    o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+    o.out = b->dres + 2 * (&s - b->slots.data());
+    o.out2 = o.out + 1;
+  };
+  SegPair* ops;
+  int32_t* map;
+  int nblk;
+  if (int rc = seg_launch_prep<SegPair>(b, ids, fill, &ops, &map, &nblk)) return rc;
+  double* part = b->dpart;
+  const unsigned nops = (unsigned)ids.size();
+  if (nblk)
+    enqueue(b, {12, u64(ops), u64(map), u64(part), (uint64_t)nblk}, [=](hipStream_t st) {
+      hipLaunchKernelGGL(k_seg_pair, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk);
+    });
+  enqueue(b, {13, u64(ops), u64(part), (uint64_t)nblk, nops}, [=](hipStream_t st) {
+    hipLaunchKernelGGL((k_seg_final<SegPair, 2>), dim3(nops), dim3(256), 0, st, ops, part, nblk);
+  });
+  enqueue_d2h(b, b->hres, b->dres, sizeof(double) * 2 * b->slots.size());
+  if (int rc = bsync(b)) return rc;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    out[2 * k] = b->hres[2 * ids[k]];
+    out[2 * k + 1] = b->hres[2 * ids[k] + 1];
+  }
   return A2C_OK;
 }
 
@@ -796,11 +1025,13 @@ auto lmvm_state = [](Slot& s, const double** X, double** G) {
 };
 
 // Norm of the projected gradient below which a mixed-precision slot continues
-// in FP64: mixed_gtol, or automatically 1e-8 sqrt(N) -- ten times the
-// difference of the mixed and FP64 gradients at relaxed states (measured
-// 9e-10 sqrt(N) for 1,728 to 110,592 sites).
+// in FP64: mixed_gtol, or automatically 3e-9 sqrt(N) -- about three times
+// the difference of the mixed and FP64 gradients at relaxed states (measured
+// 9e-10 sqrt(N) for 1,728 to 110,592 sites).  On 64 lattices of 4,096 to
+// 110,592 sites, 3e-9 / 5e-9 / 1e-8 / 2e-8 sqrt(N) took 3,104 / 3,414 /
+// 3,850 / 4,423 FP64 evaluations at about the same total count.
 double mixed_switch_gnorm(const Slot& s) {
-  return s.opts.mixed_gtol > 0.0 ? s.opts.mixed_gtol : 1e-8 * std::sqrt((double)s.N());
+  return s.opts.mixed_gtol > 0.0 ? s.opts.mixed_gtol : 3e-9 * std::sqrt((double)s.N());
 }
 
 int converged_reason(const Slot& s) {
@@ -887,19 +1118,17 @@ int lmvm_init(a2c_batch* b, const std::vector<int>& ids) {
 // d = -H g for the listed slots (two-loop recursion, or scaled steepest
 // descent for slots with no pairs or marked steepest)
 int directions(a2c_batch* b, const std::vector<int>& ids) {
-  std::vector<int> tl, sd, small, rest;  // segmented two-loop, steepest descent, fused two-loop; tl + sd
+  std::vector<int> tl, sd, small;  // segmented two-loop, steepest descent, fused two-loop
   int maxp = 0;
   for (int i : ids) {
     Slot& s = b->slots[i];
     if (s.steepest || s.npairs == 0) {
       s.npairs = 0;
       sd.push_back(i);
-      rest.push_back(i);
     } else if (b->fused_two_loop && s.N() <= TL_MAX_SITES) {
       small.push_back(i);
     } else {
       tl.push_back(i);
-      rest.push_back(i);
       maxp = std::max(maxp, s.npairs);
     }
   }
@@ -932,58 +1161,43 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
   auto scal = [b](const Slot& s) { return b->dscal + SCAL_PER_SLOT * (&s - b->slots.data()); };
   if (!tl.empty()) {
     // Two-loop recursion with every dot product and coefficient on the
-    // device (no host round trip inside the recursion).
-    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.q; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; }))
-      return rc;
-    for (int t = 0; t < maxp; ++t) {  // newest to oldest: A_i = s_i.q, q -= rho_i A_i y_i
-      std::vector<int> sub;
-      for (int i : tl)
-        if (t < b->slots[i].npairs) sub.push_back(i);
-      auto pidx = [t](const Slot& s) { return (s.head - 1 - t + s.m) % s.m; };
-      if (int rc = seg_dot_dev(b, sub, [&](SegDot& o, Slot& s) {
-            o.a1 = s.S[pidx(s)]; o.b1 = s.q; o.out = scal(s) + pidx(s);
-          }))
-        return rc;
-      if (int rc = seg_axpby(b, sub, [&](SegAx& o, Slot& s) {
-            o.z = s.q; o.x = s.q; o.y = s.Y[pidx(s)]; o.a = 1.0;
-            o.b = -s.rho[pidx(s)]; o.bmode = 1; o.q1 = scal(s) + pidx(s);
-          }))
-        return rc;
-    }
-    // H0 = diag(gamma_g) D, gamma_g = s.y / y.D y of the newest pair over
-    // coordinate group g (the totals, at m and m + 1, stand in for a group
-    // without positive curvature data)
-    for (int g = -1; g < NGROUP; ++g)
-      if (int rc = seg_dot_dev(b, tl, [&](SegDot& o, Slot& s) {
-            const int nw = (s.head - 1 + s.m) % s.m;
-            o.a1 = s.S[nw]; o.b1 = s.Y[nw]; o.a2 = s.Y[nw]; o.b2 = s.Y[nw];
-            o.met = s.metric(); o.k2 = 1; o.group = g; o.npd = s.fe->desc.n_pd;
-            o.out = scal(s) + (g < 0 ? s.m : s.m + 5 + 2 * g);
-          }))
-        return rc;
-    if (int rc = seg_axpby(b, tl, [&](SegAx& o, Slot& s) {
-          o.z = s.q; o.x = s.q; o.y = s.q; o.a = 1.0; o.amode = 4; o.p1 = scal(s) + s.m; o.p2 = scal(s) + s.m + 1;
-          o.gscal = scal(s) + s.m + 5; o.npd = s.fe->desc.n_pd;
-          o.b = 0.0; o.met = s.metric(); o.scaled = 1;
+    // device (no host round trip inside the recursion): pass 0, then
+    // passes 1..np of the first loop and np+1..2np of the second; a slot
+    // with fewer pairs than the batch's largest is done after its 2np.
+    auto pidx = [](const Slot& s, int t) { return (s.head - 1 - t + s.m) % s.m; };
+    if (int rc = seg_tl(b, tl, true, [&](SegTL& o, Slot& s) {
+          o.q = s.q; o.v = s.g; o.w = s.S[pidx(s, 0)]; o.y0 = s.Y[pidx(s, 0)];
+          o.met = s.metric(); o.npd = s.fe->desc.n_pd;
+          o.out = scal(s) + pidx(s, 0); o.out2 = scal(s) + s.m + 5;
         }))
       return rc;
-    for (int t = maxp - 1; t >= 0; --t) {  // oldest to newest: q += rho_i (A_i - y_i.q) s_i
+    for (int p = 1; p <= 2 * maxp; ++p) {
       std::vector<int> sub;
       for (int i : tl)
-        if (t < b->slots[i].npairs) sub.push_back(i);
-      auto pidx = [t](const Slot& s) { return (s.head - 1 - t + s.m) % s.m; };
-      if (int rc = seg_dot_dev(b, sub, [&](SegDot& o, Slot& s) {
-            o.a1 = s.Y[pidx(s)]; o.b1 = s.q; o.out = scal(s) + s.m + 2;
-          }))
-        return rc;
-      if (int rc = seg_axpby(b, sub, [&](SegAx& o, Slot& s) {
-            o.z = s.q; o.x = s.q; o.y = s.S[pidx(s)]; o.a = 1.0;
-            o.b = s.rho[pidx(s)]; o.bmode = 2; o.q1 = scal(s) + pidx(s); o.q2 = scal(s) + s.m + 2;
+        if (p <= 2 * b->slots[i].npairs) sub.push_back(i);
+      if (int rc = seg_tl(b, sub, false, [&](SegTL& o, Slot& s) {
+            const int np = s.npairs;
+            o.q = s.q;
+            if (p <= np) {  // first loop, newest to oldest: q -= rho_t A_t y_t, A_t = s_t . q
+              const int t = p - 1;
+              o.v = s.Y[pidx(s, t)]; o.rho = -s.rho[pidx(s, t)]; o.ca = scal(s) + pidx(s, t);
+              if (t + 1 < np) { o.w = s.S[pidx(s, t + 1)]; o.out = scal(s) + pidx(s, t + 1); }
+              else {  // then q = diag(gamma_g) D q; B = y_{np-1} . q
+                o.mode = 1; o.gscal = scal(s) + s.m + 5; o.met = s.metric(); o.npd = s.fe->desc.n_pd;
+                o.w = s.Y[pidx(s, np - 1)]; o.out = scal(s) + s.m + 2;
+              }
+            } else {  // second loop, oldest to newest: q += rho_t (A_t - B_t) s_t, B_t = y_t . q
+              const int t = 2 * np - p;
+              o.v = s.S[pidx(s, t)]; o.rho = s.rho[pidx(s, t)]; o.ca = scal(s) + pidx(s, t); o.cb = scal(s) + s.m + 2;
+              if (t > 0) { o.w = s.Y[pidx(s, t - 1)]; o.out = scal(s) + s.m + 2; }
+              else {  // d = -q, widths at an active bound not moved; g . d
+                o.mode = 2; o.d = s.d; o.g = s.g; o.x = s.x; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+                o.out = b->dres + 2 * (&s - b->slots.data());
+              }
+            }
           }))
         return rc;
     }
-    if (int rc = seg_axpby(b, tl, [](SegAx& o, Slot& s) { o.z = s.d; o.x = s.q; o.y = s.q; o.a = -1.0; o.b = 0.0; }))
-      return rc;
   }
   if (!sd.empty()) {
     // scaled steepest descent d = -delta D g: TAO's initial scale
@@ -1000,18 +1214,19 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
         }))
       return rc;
   }
-  if (int rc = seg_bound<2>(b, rest, [](SegBound& o, Slot& s) {
+  if (int rc = seg_bound<2>(b, sd, [](SegBound& o, Slot& s) {
         o.z = s.d; o.x = s.x; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
       }))
     return rc;
-  if (!rest.empty()) {  // its copy of the results also brings the fused slots' g . d
-    if (int rc = seg_dot(b, rest, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
-    for (size_t k = 0; k < rest.size(); ++k) b->slots[rest[k]].gd = v[2 * k];
+  if (!sd.empty()) {  // its copy of the results also brings the two-loop slots' g . d
+    if (int rc = seg_dot(b, sd, [](SegDot& o, Slot& s) { o.a1 = s.g; o.b1 = s.d; }, v)) return rc;
+    for (size_t k = 0; k < sd.size(); ++k) b->slots[sd[k]].gd = v[2 * k];
   } else {
     enqueue_d2h(b, b->hres, b->dres, sizeof(double) * 2 * b->slots.size());
     if (int rc = bsync(b)) return rc;
   }
   for (int i : small) b->slots[i].gd = b->hres[2 * i];
+  for (int i : tl) b->slots[i].gd = b->hres[2 * i];
   return A2C_OK;
 }
 
@@ -1035,8 +1250,8 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
   if (go.empty()) return A2C_OK;
   // line search setup (projected search: trial points are clipped onto the
   // sigma bounds, and the slope is taken along the projected step)
-  if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.xo; o.x = s.x; o.y = s.x; o.a = 1.0; o.b = 0.0; })) return rc;
-  if (int rc = seg_axpby(b, go, [](SegAx& o, Slot& s) { o.z = s.go; o.x = s.g; o.y = s.g; o.a = 1.0; o.b = 0.0; })) return rc;
+  if (int rc = seg_bound<4>(b, go, [](SegBound& o, Slot& s) { o.z = s.xo; o.x = s.x; o.w = s.go; o.y = s.g; }))
+    return rc;
   for (size_t k = 0; k < go.size(); ++k) {
     Slot& s = b->slots[go[k]];
     s.fold = s.f;
@@ -1058,8 +1273,6 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
     for (int i : srch) (b->slots[i].rep.evals >= b->slots[i].opts.max_funcs ? cap : run).push_back(i);
     for (int i : cap) b->slots[i].lsstat = -1;
     if (run.empty()) break;
-    if (int rc = seg_axpby(b, run, [](SegAx& o, Slot& s) { o.z = s.x; o.x = s.xo; o.y = s.d; o.a = 1.0; o.b = s.stp; }))
-      return rc;
     if (int rc = seg_bound<3>(b, run, [](SegBound& o, Slot& s) {
           o.z = s.x; o.x = s.xo; o.y = s.d; o.w = s.q;
           o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub; o.stp = s.stp;
@@ -1091,8 +1304,8 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
     (s.lsstat >= 0 && decreased ? ok : bad).push_back(i);
   }
   if (!bad.empty()) {
-    if (int rc = seg_axpby(b, bad, [](SegAx& o, Slot& s) { o.z = s.x; o.x = s.xo; o.y = s.xo; o.a = 1.0; o.b = 0.0; })) return rc;
-    if (int rc = seg_axpby(b, bad, [](SegAx& o, Slot& s) { o.z = s.g; o.x = s.go; o.y = s.go; o.a = 1.0; o.b = 0.0; })) return rc;
+    if (int rc = seg_bound<4>(b, bad, [](SegBound& o, Slot& s) { o.z = s.x; o.x = s.xo; o.w = s.g; o.y = s.go; }))
+      return rc;
     for (int i : bad) {
       Slot& s = b->slots[i];
       s.f = s.fold;
@@ -1113,19 +1326,8 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
     if (s.rep.reason) s.running = false;
   }
   if (!ok.empty()) {
-    if (int rc = seg_axpby(b, ok, [](SegAx& o, Slot& s) { o.z = s.S[s.head]; o.x = s.x; o.y = s.xo; o.a = 1.0; o.b = -1.0; }))
-      return rc;
-    if (int rc = seg_axpby(b, ok, [](SegAx& o, Slot& s) { o.z = s.Y[s.head]; o.x = s.g; o.y = s.go; o.a = 1.0; o.b = -1.0; }))
-      return rc;
-    // s.y of the new pair and the projected-gradient norm in one round trip
-    if (int rc = seg_bound<1>(b, ok, [](SegBound& o, Slot& s) {
-          o.z = s.gp; o.x = s.x; o.y = s.g; o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
-        }))
-      return rc;
-    if (int rc = seg_dot(b, ok, [](SegDot& o, Slot& s) {
-          o.a1 = s.S[s.head]; o.b1 = s.Y[s.head]; o.a2 = s.gp; o.b2 = s.gp;
-        }, v))
-      return rc;
+    // the new pair, s.y and the projected-gradient norm in one round trip
+    if (int rc = seg_pair(b, ok, v)) return rc;
     for (size_t k = 0; k < ok.size(); ++k) {
       Slot& s = b->slots[ok[k]];
       s.gnorm = std::sqrt(v[2 * k + 1]);
@@ -1461,7 +1663,7 @@ extern "C" void a2c_lmvm_opts_default(a2c_lmvm_opts* o) {
   o->gttol = 0.0;
   o->sigma_lb = 1e-6;  // formVariableBounds, minimizer.cpp:815,822
   o->mixed_switch = 1;
-  o->mixed_gtol = 0.0;  // automatic: 1e-8 sqrt(N)
+  o->mixed_gtol = 0.0;  // automatic: 3e-9 sqrt(N)
   o->sigma_ub = HUGE_VAL;  // formVariableBounds leaves sigma unbounded above
   o->precond = 1;
 }

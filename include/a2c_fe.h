@@ -139,9 +139,9 @@ typedef struct a2c_lmvm_opts {
   double sigma_lb;      /* lower bound on sigma (1e-6); q unbounded */
   int32_t mixed_switch; /* MIXED precision: switch to FP64 evals once
                            ||proj g|| <= mixed_gtol (1) */
-  double mixed_gtol;    /* 0 (default): automatic, 1e-8 sqrt(N sites) -- ten
-                           times the measured difference of the mixed and
-                           FP64 gradients at relaxed states */
+  double mixed_gtol;    /* 0 (default): automatic, 3e-9 sqrt(N sites) --
+                           about three times the measured difference of the
+                           mixed and FP64 gradients at relaxed states */
   double sigma_ub;      /* upper bound on sigma (+inf, as formVariableBounds,
                            minimizer.cpp:804-830).  The functional is
                            unbounded below as sigma -> inf (the entropy term
