@@ -35,7 +35,7 @@ POINTS = {"r12": 12, "gh3": 27}  # quadrature points per pair
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--steps", type=int, default=1000, help="timed evaluations (the workload's 1000 repeats)")
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--cells", type=int, default=40)
     p.add_argument("--precision", choices=["mixed", "fp64"], default=os.environ.get("A2C_BENCH_PREC", "mixed"))
