@@ -106,7 +106,8 @@ class AutotuneOpts(C.Structure):
 
 class AutotuneReport(C.Structure):
     _fields_ = [("sites", C.c_int32 * TUNE_MAX), ("copies", C.c_int32 * TUNE_MAX),
-                ("ns_per_site", C.c_double * 4 * 2 * TUNE_MAX)]
+                ("ns_per_site", C.c_double * 4 * 2 * TUNE_MAX), ("us_alone", C.c_double * 4 * 2 * TUNE_MAX),
+                ("latency", C.c_int32 * 2 * TUNE_MAX)]
 
 
 class DmdParams(C.Structure):
@@ -701,8 +702,10 @@ def tune_set(table=None):
 
 def autotune(cells=None, target_sites=None, reps=None, apply=False, device=0):
     """Measure lanes per site for each lattice size (include/a2c_tune.h).
-    Returns (table, report) where report lists, per size, sites, copies and
-    ns per site for K = 1, 2, 4, 8 in mixed and FP64 precision."""
+    Returns (table, report) where report lists, per size, sites, copies, ns
+    per site of a fused launch of the copies and microseconds per evaluation
+    of one lattice alone, for K = 1, 2, 4, 8 in mixed and FP64 precision, and
+    whether the lone-lattice time chose K ("latency")."""
     o = AutotuneOpts()
     lib.a2c_autotune_opts_default(C.byref(o))
     o.device, o.apply = device, int(bool(apply))
@@ -721,7 +724,9 @@ def autotune(cells=None, target_sites=None, reps=None, apply=False, device=0):
     rows = []
     for i in range(o.n_sizes):
         rows.append({"cells": o.cells[i], "sites": r.sites[i], "copies": r.copies[i],
-                     "ns_per_site": {"mixed": list(r.ns_per_site[i][0]), "fp64": list(r.ns_per_site[i][1])}})
+                     "ns_per_site": {"mixed": list(r.ns_per_site[i][0]), "fp64": list(r.ns_per_site[i][1])},
+                     "us_alone": {"mixed": list(r.us_alone[i][0]), "fp64": list(r.us_alone[i][1])},
+                     "latency": {"mixed": bool(r.latency[i][0]), "fp64": bool(r.latency[i][1])}})
     return _table_dict(t), rows
 
 

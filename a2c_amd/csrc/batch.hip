@@ -459,23 +459,52 @@ struct TwoLoop {
 
 __device__ __forceinline__ double tl_sum(double v, double* red) { return bsum<TL_THREADS>(v, red); }
 
+// Each thread takes elements i0 + u TL_THREADS (u < TL_U) of a pass at a time
+// and loads them all before it stores any: a slot's vectors are distinct
+// arrays, but a store to q could otherwise hold back the next element's
+// loads, and a pass would pay one memory latency per element.
+constexpr int TL_U = 4;
+
 __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restrict__ ops) {
   __shared__ double red[TL_THREADS / 64];
   __shared__ double al[TL_MAX_PAIRS];  // A_t = s_t . q of the first loop
   const TwoLoop& o = ops[blockIdx.x];
-  const int n = 4 * o.nsite, np = o.np;
+  const int n = 4 * o.nsite, np = o.np, npd = o.npd;
+  const Metric met = o.met;
+  double* __restrict__ q = o.q;
+  const double* __restrict__ g = o.g;
   // q = g; A_0 = s_0 . q; per coordinate group g the scale
   // gamma_g = s_0 . y_0 / y_0 . D y_0 over the group, H0 = diag(gamma_g) D
   double a = 0.0, gs[2 * NGROUP] = {};
-  for (int i = threadIdx.x; i < n; i += TL_THREADS) {
-    const double gi = o.g[i], si = o.S[0][i], yi = o.Y[0][i];
-    const int gr = group_of(i >> 2, i & 3, o.npd);
-    o.q[i] = gi;
-    a += si * gi;
+  {
+    const double* __restrict__ s0 = o.S[0];
+    const double* __restrict__ y0 = o.Y[0];
+    for (int i0 = threadIdx.x; i0 < n; i0 += TL_U * TL_THREADS) {
+      double gv[TL_U], sv[TL_U], yv[TL_U];
 #pragma unroll
-    for (int k = 0; k < NGROUP; ++k) {
-      gs[2 * k] += k == gr ? si * yi : 0.0;
-      gs[2 * k + 1] += k == gr ? yi * yi * metric_inv(o.met, i >> 2) : 0.0;
+      for (int u = 0; u < TL_U; ++u) {
+        const int i = i0 + u * TL_THREADS;
+        if (i < n) {
+          gv[u] = g[i];
+          sv[u] = s0[i];
+          yv[u] = y0[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < TL_U; ++u) {
+        const int i = i0 + u * TL_THREADS;
+        if (i < n) {
+          const double gi = gv[u], si = sv[u], yi = yv[u];
+          const int gr = group_of(i >> 2, i & 3, npd);
+          q[i] = gi;
+          a += si * gi;
+#pragma unroll
+          for (int k = 0; k < NGROUP; ++k) {
+            gs[2 * k] += k == gr ? si * yi : 0.0;
+            gs[2 * k + 1] += k == gr ? yi * yi * metric_inv(met, i >> 2) : 0.0;
+          }
+        }
+      }
     }
   }
   double A = tl_sum(a, red);
@@ -488,44 +517,74 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restri
     yy += gs[2 * k + 1];
   }
   if (threadIdx.x == 0) al[0] = A;
+  // q += c v, then (scale) q = diag(gamma_g) D q; returns this thread's part of w . q
+  auto update = [&](const double* __restrict__ v, const double* __restrict__ w, double c, bool scale) {
+    double acc = 0.0;
+    for (int i0 = threadIdx.x; i0 < n; i0 += TL_U * TL_THREADS) {
+      double qv[TL_U], vv[TL_U], wv[TL_U];
+#pragma unroll
+      for (int u = 0; u < TL_U; ++u) {
+        const int i = i0 + u * TL_THREADS;
+        if (i < n) {
+          qv[u] = q[i];
+          vv[u] = v[i];
+          wv[u] = w[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < TL_U; ++u) {
+        const int i = i0 + u * TL_THREADS;
+        if (i < n) {
+          double qi = qv[u] + c * vv[u];
+          if (scale) qi = group_gamma(gs, sy, yy, group_of(i >> 2, i & 3, npd)) * metric_inv(met, i >> 2) * qi;
+          q[i] = qi;
+          acc += wv[u] * qi;
+        }
+      }
+    }
+    return acc;
+  };
   // first loop, newest to oldest: q -= rho_t A_t y_t; then q = diag(gamma_g) D q
   for (int t = 0; t < np; ++t) {
-    const double c = -o.rho[t] * A;
-    const double* yt = o.Y[t];
     const bool last = t + 1 == np;
-    const double* nx = last ? o.Y[np - 1] : o.S[t + 1];  // next dot: A_{t+1}, or B_{np-1} = y . q
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += TL_THREADS) {
-      double qi = o.q[i] + c * yt[i];
-      if (last) qi = group_gamma(gs, sy, yy, group_of(i >> 2, i & 3, o.npd)) * metric_inv(o.met, i >> 2) * qi;
-      o.q[i] = qi;
-      acc += nx[i] * qi;
-    }
-    A = tl_sum(acc, red);
+    // next dot: A_{t+1}, or B_{np-1} = y . q
+    A = tl_sum(update(o.Y[t], last ? o.Y[np - 1] : o.S[t + 1], -o.rho[t] * A, last), red);
     if (!last && threadIdx.x == 0) al[t + 1] = A;
   }
   // second loop, oldest to newest: q += rho_t (A_t - B_t) s_t, B_t = y_t . q
   double B = A;
-  for (int t = np - 1; t >= 0; --t) {
-    const double c = o.rho[t] * (al[t] - B);
-    const double* st = o.S[t];
+  for (int t = np - 1; t > 0; --t) B = tl_sum(update(o.S[t], o.Y[t - 1], o.rho[t] * (al[t] - B), false), red);
+  {  // t = 0: d = -q, sigma components at an active bound zeroed; g . d
+    const double c = o.rho[0] * (al[0] - B);
+    const double* __restrict__ st = o.S[0];
+    const double* __restrict__ x = o.x;
+    double* __restrict__ d = o.d;
+    const double lb = o.lb, ub = o.ub;
     double acc = 0.0;
-    if (t > 0) {
-      const double* yn = o.Y[t - 1];
-      for (int i = threadIdx.x; i < n; i += TL_THREADS) {
-        const double qi = o.q[i] + c * st[i];
-        o.q[i] = qi;
-        acc += yn[i] * qi;
-      }
-    } else {  // d = -q, sigma components at an active bound zeroed; g . d
-      for (int i = threadIdx.x; i < n; i += TL_THREADS) {
-        double di = -(o.q[i] + c * st[i]);
-        if ((i & 3) == 3) {
-          const double xs = o.x[i];
-          if ((xs <= o.lb && di < 0.0) || (xs >= o.ub && di > 0.0)) di = 0.0;
+    for (int i0 = threadIdx.x; i0 < n; i0 += TL_U * TL_THREADS) {
+      double qv[TL_U], sv[TL_U], gv[TL_U], xv[TL_U];
+#pragma unroll
+      for (int u = 0; u < TL_U; ++u) {
+        const int i = i0 + u * TL_THREADS;
+        if (i < n) {
+          qv[u] = q[i];
+          sv[u] = st[i];
+          gv[u] = g[i];
+          xv[u] = (i & 3) == 3 ? x[i] : 0.0;
         }
-        o.d[i] = di;
-        acc += o.g[i] * di;
+      }
+#pragma unroll
+      for (int u = 0; u < TL_U; ++u) {
+        const int i = i0 + u * TL_THREADS;
+        if (i < n) {
+          double di = -(qv[u] + c * sv[u]);
+          if ((i & 3) == 3) {
+            const double xs = xv[u];
+            if ((xs <= lb && di < 0.0) || (xs >= ub && di > 0.0)) di = 0.0;
+          }
+          d[i] = di;
+          acc += gv[u] * di;
+        }
       }
     }
     B = tl_sum(acc, red);

@@ -2,8 +2,8 @@
 // size and precision) and the autotuner that measures it.
 //
 // The built-in table is the outcome of a2c_autotune on one MI355X with the
-// default options (fused batched evaluations of 6^3..40^3-cell lattices
-// filling about one million sites per launch).
+// default options (evaluations of 6^3..40^3-cell lattices, in fused launches
+// filling about one million sites and alone).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,12 +22,16 @@ namespace a2c {
 namespace {
 
 a2c_tune_table default_table() {
-  // a2c_autotune with the default options on one MI355X (ns per site of a
-  // fused evaluation, K = 1/2/4/8: 40^3 cells mixed 1.75/1.61/1.60/1.79, FP64 4.50/4.45/4.49/4.90;
-  // 6^3 cells x 256 mixed 2.02/1.71/1.64/1.85, FP64 5.39/4.50/4.51/4.84)
+  // a2c_autotune with the default options on one MI355X.  Up to 11^3 cells a
+  // lattice alone fills fewer SIMDs than the device has, and K = 8 makes it
+  // finish soonest (6^3 cells alone, K = 1/2/4/8: mixed 307/157/111/92 us,
+  // FP64 702/283/176/136 us per evaluation), at a cost of 4-11% per site in
+  // launches of 256 copies; larger lattices take the K of the best
+  // throughput (ns per site, 40^3 cells: mixed 1.75/1.58/1.60/1.80, FP64
+  // 4.50/4.46/4.48/4.90).
   static const int32_t max_sites[] = {2660, 6604, 15288, 32003, 71831, 170267, 366357, INT_MAX};
-  static const int32_t lanes_mixed[] = {4, 4, 4, 4, 4, 4, 2, 4};
-  static const int32_t lanes_fp64[] = {2, 2, 4, 2, 2, 2, 2, 2};
+  static const int32_t lanes_mixed[] = {8, 8, 8, 4, 4, 4, 2, 2};
+  static const int32_t lanes_fp64[] = {8, 8, 8, 4, 4, 4, 2, 2};
   a2c_tune_table t;
   std::memset(&t, 0, sizeof t);
   t.n_classes = 8;
@@ -112,6 +116,10 @@ int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_r
   res.n_classes = o->n_sizes;
   const a2c_tune_table saved = a2c::g_table;
   const int lanes[4] = {1, 2, 4, 8};
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, o->device) != hipSuccess || ncu < 1)
+    return A2C_EHIP;
+  const int64_t simds = 4 * (int64_t)ncu;
   int rc = A2C_OK;
   for (int i = 0; i < o->n_sizes && rc == A2C_OK; ++i) {
     const int n = o->cells[i];
@@ -152,22 +160,36 @@ int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_r
     }
     for (int p = 0; p < 2 && rc == A2C_OK; ++p) {
       const int prec = p == 0 ? A2C_PREC_MIXED : A2C_PREC_FP64;
-      double best = HUGE_VAL;
-      int bestk = 1;
+      double best = HUGE_VAL, best1 = HUGE_VAL;
+      int bestk = 1, bestk1 = 1;
       for (int li = 0; li < 4 && rc == A2C_OK; ++li) {
         a2c::g_table = a2c::uniform(lanes[li]);
-        for (int w = 0; w < 2 && rc == A2C_OK; ++w) rc = a2c_batch_eval(b, copies, slots.data(), prec, f.data());
-        const auto t0 = std::chrono::steady_clock::now();
-        for (int r = 0; r < o->reps && rc == A2C_OK; ++r) rc = a2c_batch_eval(b, copies, slots.data(), prec, f.data());
-        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        const double ns = s * 1e9 / ((double)o->reps * copies * N);
-        rep.ns_per_site[i][p][li] = ns;
-        if (ns < best) {
-          best = ns;
-          bestk = lanes[li];
+        for (int alone = 0; alone < 2 && rc == A2C_OK; ++alone) {
+          const int n = alone ? 1 : copies, reps = alone ? 4 * o->reps : o->reps;
+          for (int w = 0; w < 2 && rc == A2C_OK; ++w) rc = a2c_batch_eval(b, n, slots.data(), prec, f.data());
+          const auto t0 = std::chrono::steady_clock::now();
+          for (int r = 0; r < reps && rc == A2C_OK; ++r) rc = a2c_batch_eval(b, n, slots.data(), prec, f.data());
+          const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+          if (alone) {
+            rep.us_alone[i][p][li] = s * 1e6;
+            if (s < best1) {
+              best1 = s;
+              bestk1 = lanes[li];
+            }
+          } else {
+            const double ns = s * 1e9 / ((double)copies * N);
+            rep.ns_per_site[i][p][li] = ns;
+            if (ns < best) {
+              best = ns;
+              bestk = lanes[li];
+            }
+          }
         }
       }
-      (p == 0 ? res.lanes_mixed : res.lanes_fp64)[i] = bestk;
+      // one wave = 64 lanes: a lattice alone under bestk fills fewer SIMDs than the device has
+      const bool lat = (int64_t)N * bestk < 64 * simds;
+      rep.latency[i][p] = lat;
+      (p == 0 ? res.lanes_mixed : res.lanes_fp64)[i] = lat ? bestk1 : bestk;
     }
     a2c_batch_destroy(b);
   }

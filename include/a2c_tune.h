@@ -53,14 +53,21 @@ typedef struct a2c_autotune_report {
   int32_t copies[A2C_TUNE_MAX_CLASSES];         /* lattices per fused launch */
   double ns_per_site[A2C_TUNE_MAX_CLASSES][2][4]; /* [size][mixed, fp64][K = 1, 2, 4, 8]: wall time of
                                                      one fused evaluation per site */
+  double us_alone[A2C_TUNE_MAX_CLASSES][2][4];    /* the same, one lattice per launch: microseconds
+                                                     per evaluation */
+  int32_t latency[A2C_TUNE_MAX_CLASSES][2];       /* 1: the class took the lone-lattice choice */
 } a2c_autotune_report;
 
 void a2c_autotune_opts_default(a2c_autotune_opts* o);
-/* Time fused batched evaluations (objective and gradient, synthetic Pd-H
- * lattices of the batched workload's recipe) of every representative size
- * under every K and both precisions; per size and precision the fastest K
- * wins.  Class boundaries lie at the geometric means of neighboring sizes.
- * out receives the table; report (may be null) the measurements. */
+/* Time evaluations (objective and gradient, synthetic Pd-H lattices of the
+ * batched workload's recipe) of every representative size under every K and
+ * both precisions, in fused launches of many copies (throughput) and of one
+ * lattice alone (latency).  Per size and precision the K of the best
+ * throughput wins, unless one lattice under that K would give the device
+ * fewer waves than it has SIMDs: such a lattice alone leaves SIMDs idle, and
+ * the K that makes it finish soonest wins.  Class boundaries lie at the
+ * geometric means of neighboring sizes.  out receives the table; report (may
+ * be null) the measurements. */
 int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_report* report);
 
 #ifdef __cplusplus

@@ -30,10 +30,16 @@ def test_autotune_returns_a_valid_table_and_leaves_the_default(restore_table):
     for r in rows:
         assert r["sites"] == 8 * r["cells"] ** 3 and r["copies"] >= 1
         for p in ("mixed", "fp64"):
-            assert all(t > 0 for t in r["ns_per_site"][p])
-    for i, r in enumerate(rows):  # the table holds the fastest K of each size
-        assert table["lanes_mixed"][i] == LANES[int(np.argmin(r["ns_per_site"]["mixed"]))]
-        assert table["lanes_fp64"][i] == LANES[int(np.argmin(r["ns_per_site"]["fp64"]))]
+            assert all(t > 0 for t in r["ns_per_site"][p]) and all(t > 0 for t in r["us_alone"][p])
+    for i, r in enumerate(rows):
+        # the fastest K of each size: per site in a full launch, or, for a
+        # lattice that alone gives fewer waves than the device has SIMDs
+        # (1,024 on MI355X), the K that makes it finish soonest
+        for p in ("mixed", "fp64"):
+            k_thr = LANES[int(np.argmin(r["ns_per_site"][p]))]
+            assert r["latency"][p] == (r["sites"] * k_thr < 64 * 1024)
+            k = LANES[int(np.argmin(r["us_alone"][p]))] if r["latency"][p] else k_thr
+            assert table["lanes_" + p][i] == k
     A.autotune(cells=[4], target_sites=1000, reps=1, apply=True)
     assert A.tune_get() != before or before["max_sites"] == [2**31 - 1]
 
