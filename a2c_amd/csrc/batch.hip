@@ -116,12 +116,11 @@ constexpr double Q_STEP = 0.25;     // largest displacement of a site in one lin
 constexpr int MIXED_MAX_TRIALS = 8;  // line-search trials of a mixed-precision iteration before FP64 takes over
 
 struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 zero the active part of d,
-                   // 3 form a trial point x_old + stp d, projected, and the step direction it realises;
-                   // 4 (no bound) copy a point and its gradient: z = x, w = y
-  double* z;        // x (0), gp (1), d (2), trial x (3), copy of x (4)
-  const double* x;  // x (1, 2, 4), x_old (3)
-  const double* y;  // g (1, 4), d (3)
-  double* w;        // realised direction (3), copy of g (4)
+                   // 3 form a trial point x_old + stp d, projected, and the step direction it realises
+  double* z;        // x (0), gp (1), d (2), trial x (3)
+  const double* x;  // x (1, 2), x_old (3)
+  const double* y;  // g (1), d (3)
+  double* w;        // realised direction (3)
   double lb, ub, stp;
   int32_t nsite, blk0;
 };
@@ -401,12 +400,6 @@ __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ o
   } else if (MODE == 2) {
     const double xs = o.x[i + 3], ds = o.z[i + 3];
     if ((xs <= o.lb && ds < 0.0) || (xs >= o.ub && ds > 0.0)) o.z[i + 3] = 0.0;
-  } else if (MODE == 4) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      o.z[i + c] = o.x[i + c];
-      o.w[i + c] = o.y[i + c];
-    }
   } else {
     // Trial safeguards (they bend the search path, not the fixed point): a
     // site's position moves at most Q_STEP, so that a packet whose metric
@@ -1309,10 +1302,12 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
   if (go.empty()) return A2C_OK;
   // line search setup (projected search: trial points are clipped onto the
   // sigma bounds, and the slope is taken along the projected step)
-  if (int rc = seg_bound<4>(b, go, [](SegBound& o, Slot& s) { o.z = s.xo; o.x = s.x; o.w = s.go; o.y = s.g; }))
-    return rc;
   for (size_t k = 0; k < go.size(); ++k) {
     Slot& s = b->slots[go[k]];
+    // the iteration's start point and gradient become x_old, g_old by a swap
+    // of buffer roles (every trial overwrites all of x and g)
+    std::swap(s.x, s.xo);
+    std::swap(s.g, s.go);
     s.fold = s.f;
     s.ls = MoreThuente();
     s.ls.stpmax = 1e20;
@@ -1363,10 +1358,10 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
     (s.lsstat >= 0 && decreased ? ok : bad).push_back(i);
   }
   if (!bad.empty()) {
-    if (int rc = seg_bound<4>(b, bad, [](SegBound& o, Slot& s) { o.z = s.x; o.x = s.xo; o.w = s.g; o.y = s.go; }))
-      return rc;
     for (int i : bad) {
       Slot& s = b->slots[i];
+      std::swap(s.x, s.xo);  // back to the iteration's start
+      std::swap(s.g, s.go);
       s.f = s.fold;
       if (s.lsstat == -1) { s.rep.reason = -3; s.running = false; }
       else if (s.mixed && s.opts.mixed_switch) {  // incl. non-finite mixed trials
