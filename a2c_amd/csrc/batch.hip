@@ -1077,13 +1077,14 @@ auto lmvm_state = [](Slot& s, const double** X, double** G) {
 };
 
 // Norm of the projected gradient below which a mixed-precision slot continues
-// in FP64: mixed_gtol, or automatically 3e-9 sqrt(N) -- about three times
-// the difference of the mixed and FP64 gradients at relaxed states (measured
+// in FP64: mixed_gtol, or automatically 2e-9 sqrt(N) -- about twice the
+// difference of the mixed and FP64 gradients at relaxed states (measured
 // 9e-10 sqrt(N) for 1,728 to 110,592 sites).  On 64 lattices of 4,096 to
-// 110,592 sites, 3e-9 / 5e-9 / 1e-8 / 2e-8 sqrt(N) took 3,104 / 3,414 /
-// 3,850 / 4,423 FP64 evaluations at about the same total count.
+// 110,592 sites, 1.5e-9 / 2e-9 / 3e-9 / 5e-9 / 1e-8 / 2e-8 sqrt(N) took
+// 2,870 / 3,042 / 3,317 / 3,414 / 3,850 / 4,423 FP64 evaluations of about
+// 8,000; below 2e-9 the mixed phase itself starts to need more of them.
 double mixed_switch_gnorm(const Slot& s) {
-  return s.opts.mixed_gtol > 0.0 ? s.opts.mixed_gtol : 3e-9 * std::sqrt((double)s.N());
+  return s.opts.mixed_gtol > 0.0 ? s.opts.mixed_gtol : 2e-9 * std::sqrt((double)s.N());
 }
 
 int converged_reason(const Slot& s) {
@@ -1717,7 +1718,7 @@ extern "C" void a2c_lmvm_opts_default(a2c_lmvm_opts* o) {
   o->gttol = 0.0;
   o->sigma_lb = 1e-6;  // formVariableBounds, minimizer.cpp:815,822
   o->mixed_switch = 1;
-  o->mixed_gtol = 0.0;  // automatic: 3e-9 sqrt(N)
+  o->mixed_gtol = 0.0;  // automatic: 2e-9 sqrt(N)
   o->sigma_ub = HUGE_VAL;  // formVariableBounds leaves sigma unbounded above
   o->precond = 1;
 }

@@ -139,8 +139,8 @@ typedef struct a2c_lmvm_opts {
   double sigma_lb;      /* lower bound on sigma (1e-6); q unbounded */
   int32_t mixed_switch; /* MIXED precision: switch to FP64 evals once
                            ||proj g|| <= mixed_gtol (1) */
-  double mixed_gtol;    /* 0 (default): automatic, 3e-9 sqrt(N sites) --
-                           about three times the measured difference of the
+  double mixed_gtol;    /* 0 (default): automatic, 2e-9 sqrt(N sites) --
+                           about twice the measured difference of the
                            mixed and FP64 gradients at relaxed states */
   double sigma_ub;      /* upper bound on sigma (+inf, as formVariableBounds,
                            minimizer.cpp:804-830).  The functional is
