@@ -478,8 +478,8 @@ __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lat
     const int list = pd ? side : 2 + side;
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const int n = L.rcnt[row];
-    const float* __restrict__ share = reinterpret_cast<const float*>(L.rbuf);
-    for (int t = 0; t < n; ++t) rho += (double)share[(size_t)t * L.rows + row];  // weighted by their senders
+    // shares arrive weighted by their senders
+    for_row_shares<RECV_RU_SCALAR>(reinterpret_cast<const float*>(L.rbuf), L.rows, row, n, [&](float v) { rho += (double)v; });
   }
   rho *= S.rule == GH3 ? 1.0 : QW;
   S.rho[s] = rho;
@@ -693,12 +693,11 @@ __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lat
       const int list = pd ? side : 2 + side;
       const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
       const int n = L.rcnt[row];
-      const float4* __restrict__ share = reinterpret_cast<const float4*>(L.rbuf);
-      for (int t = 0; t < n; ++t) {  // shares arrive weighted by their senders
-        const float4 v = share[(size_t)t * L.rows + row];
+      // shares arrive weighted by their senders
+      for_row_shares<RECV_RU_WIDE>(reinterpret_cast<const float4*>(L.rbuf), L.rows, row, n, [&](const float4& v) {
         g0 += v.x; g1 += v.y; g2 += v.z;
         gs += v.w;
-      }
+      });
     }
     const double xi = pd ? 1.0 : xo[row];
     const double sg = S.X[4 * (size_t)s + 3];

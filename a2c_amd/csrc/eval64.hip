@@ -532,12 +532,9 @@ __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ l
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
     const Ell& L = list_of(S, pd ? side : 2 + side);
-    const double* __restrict__ share = reinterpret_cast<const double*>(L.rbuf);
     const int n = L.rcnt[row];
-    for (int t = 0; t < n; ++t) {
-      const size_t e = (size_t)t * L.rows + row;
-      rho += share[e];  // weighted by its sender
-    }
+    // shares arrive weighted by their senders
+    for_row_shares<RECV_RU_SCALAR>(reinterpret_cast<const double*>(L.rbuf), L.rows, row, n, [&](double v) { rho += v; });
   }
   rho *= S.rule == GH3 ? 1.0 : QW;
   S.rho[s] = rho;
@@ -735,14 +732,12 @@ __global__ __launch_bounds__(TB) void k_grad_recv64(const LatDev* __restrict__ l
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const Ell& L = list_of(S, pd ? side : 2 + side);
-      const double4* __restrict__ share = reinterpret_cast<const double4*>(L.rbuf);
       const int n = L.rcnt[row];
-      for (int t = 0; t < n; ++t) {
-        const size_t ei = (size_t)t * L.rows + row;
-        const double4 v = share[ei];  // weighted by its sender
+      // shares arrive weighted by their senders
+      for_row_shares<RECV_RU_WIDE>(reinterpret_cast<const double4*>(L.rbuf), L.rows, row, n, [&](const double4& v) {
         g0 += v.x; g1 += v.y; g2 += v.z;
         gs += v.w;
-      }
+      });
     }
     const double xi = pd ? 1.0 : xo[row];
     const double sg = S.X[4 * (size_t)s + 3];

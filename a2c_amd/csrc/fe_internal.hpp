@@ -74,6 +74,32 @@ using DevSys = LatDev;
 constexpr int SITE_BLOCK = 256;  // sites per block of every site-parallel kernel
 __host__ __device__ inline int site_blocks(int n) { return (n + SITE_BLOCK - 1) / SITE_BLOCK; }
 
+// The n received shares of one row of a slot-major buffer (share t at
+// base[t * rows + row]) passed to add in slot order.  The loads of RU
+// consecutive shares are issued before any of them is added, so a row costs
+// one memory latency per RU shares rather than one per share; the order of
+// the additions, and so the result, is that of a plain loop.
+template <int RU, class T, class Add>
+__device__ __forceinline__ void for_row_shares(const T* __restrict__ base, size_t rows, int row, int n, Add add) {
+  for (int t = 0; t < n; t += RU) {
+    T v[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+      if (t + u < n) v[u] = base[(size_t)(t + u) * rows + row];
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+      if (t + u < n) add(v[u]);
+  }
+}
+// Loads in flight per lane: 8 for the density pass's scalar shares (faster
+// at every size measured), 2 for the gradient pass's shares of 16 or 32
+// bytes.  Measured on
+// the mixed gradient receive pass, RU = 1/2/4/8: a 6^3-cell lattice (latency-
+// bound) 19.6/15.0/12.2/10.9 us, a 40^3-cell one (streaming at HBM speed,
+// where more rows in flight and the registers they hold cost waves)
+// 74.6/76.3/83.0/87.9 us.
+constexpr int RECV_RU_SCALAR = 8, RECV_RU_WIDE = 2;
+
 // Lanes per site of the pair-owner passes.  A small lattice gives too few
 // waves to fill the GPU with one lane per site, and each lane would walk its
 // ~40 owned pairs x 12 points serially; splitting a site's pairs over K lanes
