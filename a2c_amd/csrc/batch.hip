@@ -451,15 +451,36 @@ struct TwoLoop {
 };
 
 __device__ __forceinline__ double tl_sum(double v, double* red) { return bsum<TL_THREADS>(v, red); }
+// K block sums at once (one pair of barriers): each the same fixed order as tl_sum
+template <int K>
+__device__ __forceinline__ void tl_sum_n(double (&v)[K], double* red) {
+  constexpr int W = TL_THREADS / 64;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
+  __syncthreads();  // the previous sum's readers are done with red
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k * W + (threadIdx.x >> 6)] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) s += red[k * W + w];
+    v[k] = s;
+  }
+}
 
 // Each thread takes elements i0 + u TL_THREADS (u < TL_U) of a pass at a time
 // and loads them all before it stores any: a slot's vectors are distinct
 // arrays, but a store to q could otherwise hold back the next element's
-// loads, and a pass would pay one memory latency per element.
-constexpr int TL_U = 4;
+// loads, and a pass would pay one memory latency per element.  With TL_U = 8
+// a slot of up to 2,048 sites makes one round trip to memory per pass.
+constexpr int TL_U = 8;
 
 __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restrict__ ops) {
-  __shared__ double red[TL_THREADS / 64];
+  __shared__ double red[(1 + 2 * NGROUP) * (TL_THREADS / 64)];
   __shared__ double al[TL_MAX_PAIRS];  // A_t = s_t . q of the first loop
   const TwoLoop& o = ops[blockIdx.x];
   const int n = 4 * o.nsite, np = o.np, npd = o.npd;
@@ -500,15 +521,21 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restri
       }
     }
   }
-  double A = tl_sum(a, red);
-  double sy = 0.0, yy = 0.0;
+  double sums[1 + 2 * NGROUP] = {a};
 #pragma unroll
-  for (int k = 0; k < 2 * NGROUP; ++k) gs[k] = tl_sum(gs[k], red);
+  for (int k = 0; k < 2 * NGROUP; ++k) sums[1 + k] = gs[k];
+  tl_sum_n(sums, red);
+  double A = sums[0];
+  double sy = 0.0, yy = 0.0, gam[NGROUP];
+#pragma unroll
+  for (int k = 0; k < 2 * NGROUP; ++k) gs[k] = sums[1 + k];
 #pragma unroll
   for (int k = 0; k < NGROUP; ++k) {
     sy += gs[2 * k];
     yy += gs[2 * k + 1];
   }
+#pragma unroll
+  for (int k = 0; k < NGROUP; ++k) gam[k] = group_gamma(gs, sy, yy, k);
   if (threadIdx.x == 0) al[0] = A;
   // q += c v, then (scale) q = diag(gamma_g) D q; returns this thread's part of w . q
   auto update = [&](const double* __restrict__ v, const double* __restrict__ w, double c, bool scale) {
@@ -529,7 +556,13 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restri
         const int i = i0 + u * TL_THREADS;
         if (i < n) {
           double qi = qv[u] + c * vv[u];
-          if (scale) qi = group_gamma(gs, sy, yy, group_of(i >> 2, i & 3, npd)) * metric_inv(met, i >> 2) * qi;
+          if (scale) {
+            const int gr = group_of(i >> 2, i & 3, npd);
+            double gm = gam[0];
+#pragma unroll
+            for (int k = 1; k < NGROUP; ++k) gm = gr == k ? gam[k] : gm;
+            qi = gm * metric_inv(met, i >> 2) * qi;
+          }
           q[i] = qi;
           acc += wv[u] * qi;
         }
@@ -583,6 +616,135 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop(const TwoLoop* __restri
     B = tl_sum(acc, red);
   }
   if (threadIdx.x == 0) o.out[0] = B;
+}
+
+// The same recursion in vector-free form, for np <= VF_PAIRS: one pass forms
+// every dot product the two loops need -- s_t . g, s_a . y_b for a older
+// than b (the first loop pairs each s with the newer y's, the second each y
+// with the older s's), and the group sums of the initial scale -- and the
+// loops then run on those scalars; a second pass forms r = diag(gamma_g) D
+// (g - sum_t alpha_t y_t) and y_t . r, a third d = -(r + sum_t c_t s_t).
+// Three passes and three block reductions instead of 2 np + 1 of each.
+constexpr int VF_PAIRS = 5;
+constexpr int VF_TRI = VF_PAIRS * (VF_PAIRS - 1) / 2;
+constexpr int VF_SUMS = VF_PAIRS + VF_TRI + 2 * NGROUP;
+__device__ __forceinline__ constexpr int vf_tri(int a, int b) { return VF_PAIRS + a * (a - 1) / 2 + b; }  // a > b
+
+__global__ __launch_bounds__(TL_THREADS) void k_two_loop_vf(const TwoLoop* __restrict__ ops) {
+  __shared__ double red[VF_SUMS * (TL_THREADS / 64)];
+  const TwoLoop& o = ops[blockIdx.x];
+  const int n = 4 * o.nsite, np = o.np, npd = o.npd;
+  const Metric met = o.met;
+  const double* __restrict__ g = o.g;
+  double* __restrict__ q = o.q;
+  const double* S[VF_PAIRS];
+  const double* Y[VF_PAIRS];
+  double rho[VF_PAIRS];
+#pragma unroll
+  for (int t = 0; t < VF_PAIRS; ++t) {
+    S[t] = t < np ? o.S[t] : nullptr;
+    Y[t] = t < np ? o.Y[t] : nullptr;
+    rho[t] = t < np ? o.rho[t] : 0.0;
+  }
+  // pass 1: [0, np) s_t . g; vf_tri(a, b) s_a . y_b; then s_0 . y_0 and
+  // y_0 . D y_0 per coordinate group
+  double acc[VF_SUMS] = {};
+  for (int i = threadIdx.x; i < n; i += TL_THREADS) {
+    double sv[VF_PAIRS], yv[VF_PAIRS];
+    const double gi = g[i];
+#pragma unroll
+    for (int t = 0; t < VF_PAIRS; ++t)
+      if (t < np) {
+        sv[t] = S[t][i];
+        yv[t] = Y[t][i];
+      }
+#pragma unroll
+    for (int a = 0; a < VF_PAIRS; ++a)
+      if (a < np) {
+        acc[a] += sv[a] * gi;
+#pragma unroll
+        for (int b = 0; b < a; ++b) acc[vf_tri(a, b)] += sv[a] * yv[b];
+      }
+    const int gr = group_of(i >> 2, i & 3, npd);
+    const double di = metric_inv(met, i >> 2);
+#pragma unroll
+    for (int k = 0; k < NGROUP; ++k) {
+      acc[VF_PAIRS + VF_TRI + 2 * k] += k == gr ? sv[0] * yv[0] : 0.0;
+      acc[VF_PAIRS + VF_TRI + 2 * k + 1] += k == gr ? yv[0] * yv[0] * di : 0.0;
+    }
+  }
+  tl_sum_n(acc, red);
+  // first loop, newest to oldest: alpha_t = rho_t s_t . (g - sum_{u<t} alpha_u y_u)
+  double al[VF_PAIRS];
+#pragma unroll
+  for (int t = 0; t < VF_PAIRS; ++t) {
+    double v = acc[t];
+#pragma unroll
+    for (int u = 0; u < t; ++u) v -= al[u] * acc[vf_tri(t, u)];
+    al[t] = t < np ? rho[t] * v : 0.0;
+  }
+  double sy = 0.0, yy = 0.0, gam[NGROUP];
+#pragma unroll
+  for (int k = 0; k < NGROUP; ++k) {
+    sy += acc[VF_PAIRS + VF_TRI + 2 * k];
+    yy += acc[VF_PAIRS + VF_TRI + 2 * k + 1];
+  }
+#pragma unroll
+  for (int k = 0; k < NGROUP; ++k) gam[k] = group_gamma(acc + VF_PAIRS + VF_TRI, sy, yy, k);
+  // pass 2: r = diag(gamma_g) D (g - sum_t alpha_t y_t) into q; y_t . r
+  double yr[VF_PAIRS] = {};
+  for (int i = threadIdx.x; i < n; i += TL_THREADS) {
+    double yv[VF_PAIRS];
+#pragma unroll
+    for (int t = 0; t < VF_PAIRS; ++t)
+      if (t < np) yv[t] = Y[t][i];
+    double qi = g[i];
+#pragma unroll
+    for (int t = 0; t < VF_PAIRS; ++t)
+      if (t < np) qi -= al[t] * yv[t];
+    const int gr = group_of(i >> 2, i & 3, npd);
+    double gm = gam[0];
+#pragma unroll
+    for (int k = 1; k < NGROUP; ++k) gm = gr == k ? gam[k] : gm;
+    const double ri = gm * metric_inv(met, i >> 2) * qi;
+    q[i] = ri;
+#pragma unroll
+    for (int t = 0; t < VF_PAIRS; ++t)
+      if (t < np) yr[t] += yv[t] * ri;
+  }
+  tl_sum_n(yr, red);
+  // second loop, oldest to newest: beta_t = rho_t y_t . (r + sum_{k>t} c_k s_k), c_t = alpha_t - beta_t
+  double c[VF_PAIRS];
+#pragma unroll
+  for (int t = VF_PAIRS - 1; t >= 0; --t) {
+    double v = yr[t];
+#pragma unroll
+    for (int k = t + 1; k < VF_PAIRS; ++k)
+      if (k < np) v += c[k] * acc[vf_tri(k, t)];
+    c[t] = t < np ? al[t] - rho[t] * v : 0.0;
+  }
+  // pass 3: d = -(r + sum_t c_t s_t) (oldest first), widths at an active bound not moved; g . d
+  const double* __restrict__ x = o.x;
+  double* __restrict__ d = o.d;
+  const double lb = o.lb, ub = o.ub;
+  double gd = 0.0;
+  for (int i = threadIdx.x; i < n; i += TL_THREADS) {
+    double sv[VF_PAIRS];
+#pragma unroll
+    for (int t = 0; t < VF_PAIRS; ++t)
+      if (t < np) sv[t] = S[t][i];
+    const double gi = g[i], xs = (i & 3) == 3 ? x[i] : 0.0;
+    double ri = q[i];
+#pragma unroll
+    for (int t = VF_PAIRS - 1; t >= 0; --t)
+      if (t < np) ri += c[t] * sv[t];
+    double di = -ri;
+    if ((i & 3) == 3 && ((xs <= lb && di < 0.0) || (xs >= ub && di > 0.0))) di = 0.0;
+    d[i] = di;
+    gd += gi * di;
+  }
+  gd = tl_sum(gd, red);
+  if (threadIdx.x == 0) o.out[0] = gd;
 }
 
 }  // namespace
@@ -1185,13 +1347,18 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
       maxp = std::max(maxp, s.npairs);
     }
   }
-  if (!small.empty()) {
-    std::vector<TwoLoop> ops(small.size());
-    for (size_t k = 0; k < small.size(); ++k) {
-      Slot& s = b->slots[small[k]];
+  // fused recursion: vector-free for up to VF_PAIRS pairs, one pass per step above
+  for (int vf = 1; vf >= 0; --vf) {
+    std::vector<int> part;
+    for (int i : small)
+      if ((b->slots[i].npairs <= VF_PAIRS) == (vf == 1)) part.push_back(i);
+    if (part.empty()) continue;
+    std::vector<TwoLoop> ops(part.size());
+    for (size_t k = 0; k < part.size(); ++k) {
+      Slot& s = b->slots[part[k]];
       TwoLoop& o = ops[k];
       o.g = s.g; o.q = s.q; o.d = s.d; o.x = s.x;
-      o.out = b->dres + 2 * small[k];
+      o.out = b->dres + 2 * part[k];
       for (int t = 0; t < s.npairs; ++t) {
         const int p = (s.head - 1 - t + s.m) % s.m;
         o.S[t] = s.S[p]; o.Y[t] = s.Y[p]; o.rho[t] = s.rho[p];
@@ -1206,8 +1373,9 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
     const TwoLoop* dops = b->arena.push(ops.data(), ops.size());
     if (!dops) return berr(b, A2C_ENOMEM, "staging arena too small");
     const unsigned nops = (unsigned)ops.size();
-    enqueue(b, {6, u64(dops), nops}, [=](hipStream_t st) {
-      hipLaunchKernelGGL(k_two_loop, dim3(nops), dim3(TL_THREADS), 0, st, dops);
+    enqueue(b, {6, (uint64_t)vf, u64(dops), nops}, [=](hipStream_t st) {
+      if (vf) hipLaunchKernelGGL(k_two_loop_vf, dim3(nops), dim3(TL_THREADS), 0, st, dops);
+      else hipLaunchKernelGGL(k_two_loop, dim3(nops), dim3(TL_THREADS), 0, st, dops);
     });
   }
   std::vector<double> v;
