@@ -215,6 +215,28 @@ __device__ __forceinline__ double bsum(double v, double* red) {
   return s;
 }
 
+// K block sums at once (one pair of barriers), each in the fixed order of
+// bsum; red holds K * NT / 64 doubles
+template <int NT, int K>
+__device__ __forceinline__ void bsum_n(double (&v)[K], double* red) {
+  constexpr int W = NT / 64;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
+  __syncthreads();  // the previous sum's readers are done with red
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k * W + (threadIdx.x >> 6)] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) s += red[k * W + w];
+    v[k] = s;
+  }
+}
+
 // Two-loop recursion of slots too large for one workgroup: one grid pass per
 // recursion step for all such slots at once.  A pass applies the update of
 // its step and forms block partials of the dot product the next step needs
@@ -451,26 +473,8 @@ struct TwoLoop {
 };
 
 __device__ __forceinline__ double tl_sum(double v, double* red) { return bsum<TL_THREADS>(v, red); }
-// K block sums at once (one pair of barriers): each the same fixed order as tl_sum
 template <int K>
-__device__ __forceinline__ void tl_sum_n(double (&v)[K], double* red) {
-  constexpr int W = TL_THREADS / 64;
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
-  __syncthreads();  // the previous sum's readers are done with red
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < K; ++k) red[k * W + (threadIdx.x >> 6)] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) s += red[k * W + w];
-    v[k] = s;
-  }
-}
+__device__ __forceinline__ void tl_sum_n(double (&v)[K], double* red) { bsum_n<TL_THREADS>(v, red); }
 
 // Each thread takes elements i0 + u TL_THREADS (u < TL_U) of a pass at a time
 // and loads them all before it stores any: a slot's vectors are distinct
@@ -630,6 +634,49 @@ constexpr int VF_TRI = VF_PAIRS * (VF_PAIRS - 1) / 2;
 constexpr int VF_SUMS = VF_PAIRS + VF_TRI + 2 * NGROUP;
 __device__ __forceinline__ constexpr int vf_tri(int a, int b) { return VF_PAIRS + a * (a - 1) / 2 + b; }  // a > b
 
+// The loops on the scalars.  acc: the VF_SUMS sums of the first pass.
+// First loop, newest to oldest: alpha_t = rho_t s_t . (g - sum_{u<t} alpha_u y_u).
+__device__ __forceinline__ void vf_alpha(const double* acc, const double* rho, int np, double (&al)[VF_PAIRS]) {
+#pragma unroll
+  for (int t = 0; t < VF_PAIRS; ++t) {
+    double v = acc[t];
+#pragma unroll
+    for (int u = 0; u < t; ++u) v -= al[u] * acc[vf_tri(t, u)];
+    al[t] = t < np ? rho[t] * v : 0.0;
+  }
+}
+// Scale of each coordinate group from its sums (the totals where it has none).
+__device__ __forceinline__ void vf_gamma(const double* acc, double (&gam)[NGROUP]) {
+  const double* gs = acc + VF_PAIRS + VF_TRI;
+  double sy = 0.0, yy = 0.0;
+#pragma unroll
+  for (int k = 0; k < NGROUP; ++k) {
+    sy += gs[2 * k];
+    yy += gs[2 * k + 1];
+  }
+#pragma unroll
+  for (int k = 0; k < NGROUP; ++k) gam[k] = group_gamma(gs, sy, yy, k);
+}
+__device__ __forceinline__ double vf_pick(const double (&gam)[NGROUP], int gr) {
+  double gm = gam[0];
+#pragma unroll
+  for (int k = 1; k < NGROUP; ++k) gm = gr == k ? gam[k] : gm;
+  return gm;
+}
+// Second loop, oldest to newest: beta_t = rho_t y_t . (r + sum_{k>t} c_k s_k),
+// c_t = alpha_t - beta_t; yr: y_t . r.
+__device__ __forceinline__ void vf_coef(const double* acc, const double* yr, const double* rho, int np,
+                                        const double (&al)[VF_PAIRS], double (&c)[VF_PAIRS]) {
+#pragma unroll
+  for (int t = VF_PAIRS - 1; t >= 0; --t) {
+    double v = yr[t];
+#pragma unroll
+    for (int k = t + 1; k < VF_PAIRS; ++k)
+      if (k < np) v += c[k] * acc[vf_tri(k, t)];
+    c[t] = t < np ? al[t] - rho[t] * v : 0.0;
+  }
+}
+
 __global__ __launch_bounds__(TL_THREADS) void k_two_loop_vf(const TwoLoop* __restrict__ ops) {
   __shared__ double red[VF_SUMS * (TL_THREADS / 64)];
   const TwoLoop& o = ops[blockIdx.x];
@@ -674,23 +721,9 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop_vf(const TwoLoop* __res
     }
   }
   tl_sum_n(acc, red);
-  // first loop, newest to oldest: alpha_t = rho_t s_t . (g - sum_{u<t} alpha_u y_u)
-  double al[VF_PAIRS];
-#pragma unroll
-  for (int t = 0; t < VF_PAIRS; ++t) {
-    double v = acc[t];
-#pragma unroll
-    for (int u = 0; u < t; ++u) v -= al[u] * acc[vf_tri(t, u)];
-    al[t] = t < np ? rho[t] * v : 0.0;
-  }
-  double sy = 0.0, yy = 0.0, gam[NGROUP];
-#pragma unroll
-  for (int k = 0; k < NGROUP; ++k) {
-    sy += acc[VF_PAIRS + VF_TRI + 2 * k];
-    yy += acc[VF_PAIRS + VF_TRI + 2 * k + 1];
-  }
-#pragma unroll
-  for (int k = 0; k < NGROUP; ++k) gam[k] = group_gamma(acc + VF_PAIRS + VF_TRI, sy, yy, k);
+  double al[VF_PAIRS], gam[NGROUP];
+  vf_alpha(acc, rho, np, al);
+  vf_gamma(acc, gam);
   // pass 2: r = diag(gamma_g) D (g - sum_t alpha_t y_t) into q; y_t . r
   double yr[VF_PAIRS] = {};
   for (int i = threadIdx.x; i < n; i += TL_THREADS) {
@@ -702,27 +735,15 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop_vf(const TwoLoop* __res
 #pragma unroll
     for (int t = 0; t < VF_PAIRS; ++t)
       if (t < np) qi -= al[t] * yv[t];
-    const int gr = group_of(i >> 2, i & 3, npd);
-    double gm = gam[0];
-#pragma unroll
-    for (int k = 1; k < NGROUP; ++k) gm = gr == k ? gam[k] : gm;
-    const double ri = gm * metric_inv(met, i >> 2) * qi;
+    const double ri = vf_pick(gam, group_of(i >> 2, i & 3, npd)) * metric_inv(met, i >> 2) * qi;
     q[i] = ri;
 #pragma unroll
     for (int t = 0; t < VF_PAIRS; ++t)
       if (t < np) yr[t] += yv[t] * ri;
   }
   tl_sum_n(yr, red);
-  // second loop, oldest to newest: beta_t = rho_t y_t . (r + sum_{k>t} c_k s_k), c_t = alpha_t - beta_t
   double c[VF_PAIRS];
-#pragma unroll
-  for (int t = VF_PAIRS - 1; t >= 0; --t) {
-    double v = yr[t];
-#pragma unroll
-    for (int k = t + 1; k < VF_PAIRS; ++k)
-      if (k < np) v += c[k] * acc[vf_tri(k, t)];
-    c[t] = t < np ? al[t] - rho[t] * v : 0.0;
-  }
+  vf_coef(acc, yr, rho, np, al, c);
   // pass 3: d = -(r + sum_t c_t s_t) (oldest first), widths at an active bound not moved; g . d
   const double* __restrict__ x = o.x;
   double* __restrict__ d = o.d;
@@ -745,6 +766,155 @@ __global__ __launch_bounds__(TL_THREADS) void k_two_loop_vf(const TwoLoop* __res
   }
   gd = tl_sum(gd, red);
   if (threadIdx.x == 0) o.out[0] = gd;
+}
+
+// The vector-free recursion for slots too large for one workgroup: the three
+// passes as grid launches over all such slots, each followed by a
+// fixed-order reduction of its block partials into the slot's device
+// scalars (the VF_SUMS first-pass sums, then the y_t . r), which the next
+// pass reads to run the loops on the scalars itself.
+struct SegVF {
+  const double* g;
+  const double* S[VF_PAIRS];  // step t = 0 is the newest pair
+  const double* Y[VF_PAIRS];
+  double rho[VF_PAIRS];
+  double* q;                  // r
+  double* d;
+  const double* x;
+  double* sums;               // VF_SUMS + VF_PAIRS device scalars
+  double* out;                // g . d
+  Metric met;
+  double lb, ub;
+  int32_t np, npd;
+  int32_t nsite, blk0;
+};
+
+__global__ __launch_bounds__(TB) void k_seg_vf1(const SegVF* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                                double* __restrict__ part, int nblk) {
+  __shared__ double red[VF_SUMS * (TB / 64)];
+  const SegVF& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  const int np = o.np;
+  double acc[VF_SUMS] = {};
+  if (s < o.nsite) {
+    const size_t i0 = 4 * (size_t)s;
+    const double di = metric_inv(o.met, s);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const size_t i = i0 + c;
+      double sv[VF_PAIRS], yv[VF_PAIRS];
+      const double gi = o.g[i];
+#pragma unroll
+      for (int t = 0; t < VF_PAIRS; ++t)
+        if (t < np) {
+          sv[t] = o.S[t][i];
+          yv[t] = o.Y[t][i];
+        }
+#pragma unroll
+      for (int a = 0; a < VF_PAIRS; ++a)
+        if (a < np) {
+          acc[a] += sv[a] * gi;
+#pragma unroll
+          for (int b = 0; b < a; ++b) acc[vf_tri(a, b)] += sv[a] * yv[b];
+        }
+      const int gr = group_of(s, c, o.npd);
+#pragma unroll
+      for (int k = 0; k < NGROUP; ++k) {
+        acc[VF_PAIRS + VF_TRI + 2 * k] += k == gr ? sv[0] * yv[0] : 0.0;
+        acc[VF_PAIRS + VF_TRI + 2 * k + 1] += k == gr ? yv[0] * yv[0] * di : 0.0;
+      }
+    }
+  }
+  bsum_n<TB>(acc, red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < VF_SUMS; ++k) part[(size_t)k * nblk + blockIdx.x] = acc[k];
+}
+
+__global__ __launch_bounds__(TB) void k_seg_vf2(const SegVF* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                                double* __restrict__ part, int nblk) {
+  __shared__ double red[VF_PAIRS * (TB / 64)];
+  const SegVF& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  const int np = o.np;
+  double al[VF_PAIRS], gam[NGROUP];
+  vf_alpha(o.sums, o.rho, np, al);
+  vf_gamma(o.sums, gam);
+  double yr[VF_PAIRS] = {};
+  if (s < o.nsite) {
+    const size_t i0 = 4 * (size_t)s;
+    const double di = metric_inv(o.met, s);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const size_t i = i0 + c;
+      double yv[VF_PAIRS];
+#pragma unroll
+      for (int t = 0; t < VF_PAIRS; ++t)
+        if (t < np) yv[t] = o.Y[t][i];
+      double qi = o.g[i];
+#pragma unroll
+      for (int t = 0; t < VF_PAIRS; ++t)
+        if (t < np) qi -= al[t] * yv[t];
+      const double ri = vf_pick(gam, group_of(s, c, o.npd)) * di * qi;
+      o.q[i] = ri;
+#pragma unroll
+      for (int t = 0; t < VF_PAIRS; ++t)
+        if (t < np) yr[t] += yv[t] * ri;
+    }
+  }
+  bsum_n<TB>(yr, red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < VF_PAIRS; ++k) part[(size_t)k * nblk + blockIdx.x] = yr[k];
+}
+
+__global__ __launch_bounds__(TB) void k_seg_vf3(const SegVF* __restrict__ ops, const int32_t* __restrict__ b2l,
+                                                double* __restrict__ part) {
+  __shared__ double red[TB / 64];
+  const SegVF& o = ops[b2l[blockIdx.x]];
+  const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
+  const int np = o.np;
+  double al[VF_PAIRS], c[VF_PAIRS];
+  vf_alpha(o.sums, o.rho, np, al);
+  vf_coef(o.sums, o.sums + VF_SUMS, o.rho, np, al, c);
+  double gd = 0.0;
+  if (s < o.nsite) {
+    const size_t i0 = 4 * (size_t)s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t i = i0 + k;
+      double ri = o.q[i];
+#pragma unroll
+      for (int t = VF_PAIRS - 1; t >= 0; --t)
+        if (t < np) ri += c[t] * o.S[t][i];
+      double di = -ri;
+      if (k == 3) {
+        const double xs = o.x[i];
+        if ((xs <= o.lb && di < 0.0) || (xs >= o.ub && di > 0.0)) di = 0.0;
+      }
+      o.d[i] = di;
+      gd += o.g[i] * di;
+    }
+  }
+  gd = bsum<TB>(gd, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = gd;
+}
+
+// one block per op: fixed-order sums of the K block partials of pass PASS
+template <int K, int PASS>
+__global__ __launch_bounds__(256) void k_seg_vf_final(const SegVF* __restrict__ ops, const double* __restrict__ part,
+                                                      int nblk) {
+  __shared__ double red[4];
+  const SegVF& o = ops[blockIdx.x];
+  const int n = site_blocks(o.nsite);
+  double* out = PASS == 1 ? o.sums : (PASS == 2 ? o.sums + VF_SUMS : o.out);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double v = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) v += part[(size_t)k * nblk + o.blk0 + j];
+    v = bsum<256>(v, red);
+    if (threadIdx.x == 0) out[k] = v;
+  }
 }
 
 }  // namespace
@@ -824,12 +994,13 @@ struct a2c_batch {
   a2c::GraphCache graphs;
   int qrc = 0;  // first error of a queued operation
   bool fused_two_loop = true;  // small slots run the two-loop recursion in one workgroup
+  bool vector_free = true;     // slots of at most VF_PAIRS pairs take the vector-free form
   double* dF = nullptr;    // objective per slot (device)
   double* hF = nullptr;    // pinned mirror
   double* dres = nullptr;  // 2 doubles per slot (device dot results read by the host)
   double* hres = nullptr;  // pinned mirror
   double* dscal = nullptr;  // SCAL_PER_SLOT doubles per slot: optimizer scalars that stay on the device
-  double* dpart = nullptr;  // TL_PART * max blocks
+  double* dpart = nullptr;  // PART_PER_BLOCK * max blocks
   size_t part_cap = 0;      // blocks
   float4* dtab = nullptr;
   double* dtab64 = nullptr;
@@ -944,14 +1115,15 @@ int bsync(a2c_batch* b) {
   return A2C_OK;
 }
 
-// make sure the partial-sum buffer holds nblk blocks (x TL_PART: up to that
-// many sums per block)
+// make sure the partial-sum buffer holds nblk blocks (x PART_PER_BLOCK: up
+// to that many sums per block)
+constexpr int PART_PER_BLOCK = VF_SUMS > TL_PART ? VF_SUMS : TL_PART;
 int ensure_part(a2c_batch* b, size_t nblk) {
   if (nblk <= b->part_cap) return A2C_OK;
   if (int rc = bsync(b)) return rc;
   (void)hipFree(b->dpart);
   b->dpart = nullptr;
-  B_HIP(b, hipMalloc(&b->dpart, sizeof(double) * TL_PART * nblk));
+  B_HIP(b, hipMalloc(&b->dpart, sizeof(double) * PART_PER_BLOCK * nblk));
   b->part_cap = nblk;
   return A2C_OK;
 }
@@ -1090,6 +1262,44 @@ int seg_tl(a2c_batch* b, const std::vector<int>& ids, bool first, Fill fill) {
       hipLaunchKernelGGL((k_seg_final<SegTL, 1>), dim3(nops), dim3(256), 0, st, ops, part, nblk);
     });
   }
+  return A2C_OK;
+}
+
+// the three passes of the segmented vector-free recursion and their reductions
+template <class Fill>
+int seg_vf(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
+  if (ids.empty()) return A2C_OK;
+  auto fill0 = [&](SegVF& o, Slot& s) {
+    o = SegVF{};
+    fill(o, s);
+  };
+  SegVF* ops;
+  int32_t* map;
+  int nblk;
+  if (int rc = seg_launch_prep<SegVF>(b, ids, fill0, &ops, &map, &nblk)) return rc;
+  double* part = b->dpart;
+  const unsigned nops = (unsigned)ids.size();
+  if (nblk)
+    enqueue(b, {14, u64(ops), u64(map), u64(part), (uint64_t)nblk}, [=](hipStream_t st) {
+      hipLaunchKernelGGL(k_seg_vf1, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk);
+    });
+  enqueue(b, {15, u64(ops), u64(part), (uint64_t)nblk, nops}, [=](hipStream_t st) {
+    hipLaunchKernelGGL((k_seg_vf_final<VF_SUMS, 1>), dim3(nops), dim3(256), 0, st, ops, part, nblk);
+  });
+  if (nblk)
+    enqueue(b, {16, u64(ops), u64(map), u64(part), (uint64_t)nblk}, [=](hipStream_t st) {
+      hipLaunchKernelGGL(k_seg_vf2, dim3(nblk), dim3(TB), 0, st, ops, map, part, nblk);
+    });
+  enqueue(b, {17, u64(ops), u64(part), (uint64_t)nblk, nops}, [=](hipStream_t st) {
+    hipLaunchKernelGGL((k_seg_vf_final<VF_PAIRS, 2>), dim3(nops), dim3(256), 0, st, ops, part, nblk);
+  });
+  if (nblk)
+    enqueue(b, {18, u64(ops), u64(map), u64(part), (uint64_t)nblk}, [=](hipStream_t st) {
+      hipLaunchKernelGGL(k_seg_vf3, dim3(nblk), dim3(TB), 0, st, ops, map, part);
+    });
+  enqueue(b, {19, u64(ops), u64(part), (uint64_t)nblk, nops}, [=](hipStream_t st) {
+    hipLaunchKernelGGL((k_seg_vf_final<1, 3>), dim3(nops), dim3(256), 0, st, ops, part, nblk);
+  });
   return A2C_OK;
 }
 
@@ -1333,7 +1543,7 @@ int lmvm_init(a2c_batch* b, const std::vector<int>& ids) {
 // d = -H g for the listed slots (two-loop recursion, or scaled steepest
 // descent for slots with no pairs or marked steepest)
 int directions(a2c_batch* b, const std::vector<int>& ids) {
-  std::vector<int> tl, sd, small;  // segmented two-loop, steepest descent, fused two-loop
+  std::vector<int> tl, tlvf, sd, small;  // segmented two-loop (per step / vector-free), steepest descent, fused
   int maxp = 0;
   for (int i : ids) {
     Slot& s = b->slots[i];
@@ -1342,6 +1552,8 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
       sd.push_back(i);
     } else if (b->fused_two_loop && s.N() <= TL_MAX_SITES) {
       small.push_back(i);
+    } else if (b->vector_free && s.npairs <= VF_PAIRS) {
+      tlvf.push_back(i);
     } else {
       tl.push_back(i);
       maxp = std::max(maxp, s.npairs);
@@ -1351,7 +1563,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
   for (int vf = 1; vf >= 0; --vf) {
     std::vector<int> part;
     for (int i : small)
-      if ((b->slots[i].npairs <= VF_PAIRS) == (vf == 1)) part.push_back(i);
+      if ((b->vector_free && b->slots[i].npairs <= VF_PAIRS) == (vf == 1)) part.push_back(i);
     if (part.empty()) continue;
     std::vector<TwoLoop> ops(part.size());
     for (size_t k = 0; k < part.size(); ++k) {
@@ -1380,6 +1592,18 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
   }
   std::vector<double> v;
   auto scal = [b](const Slot& s) { return b->dscal + SCAL_PER_SLOT * (&s - b->slots.data()); };
+  static_assert(VF_SUMS + VF_PAIRS <= SCAL_PER_SLOT, "device scalars of the vector-free recursion");
+  if (int rc = seg_vf(b, tlvf, [&](SegVF& o, Slot& s) {
+        o.g = s.g; o.q = s.q; o.d = s.d; o.x = s.x;
+        for (int t = 0; t < s.npairs; ++t) {
+          const int p = (s.head - 1 - t + s.m) % s.m;
+          o.S[t] = s.S[p]; o.Y[t] = s.Y[p]; o.rho[t] = s.rho[p];
+        }
+        o.np = s.npairs; o.npd = s.fe->desc.n_pd; o.met = s.metric();
+        o.lb = s.opts.sigma_lb; o.ub = s.opts.sigma_ub;
+        o.sums = scal(s); o.out = b->dres + 2 * (&s - b->slots.data());
+      }))
+    return rc;
   if (!tl.empty()) {
     // Two-loop recursion with every dot product and coefficient on the
     // device (no host round trip inside the recursion): pass 0, then
@@ -1448,6 +1672,7 @@ int directions(a2c_batch* b, const std::vector<int>& ids) {
   }
   for (int i : small) b->slots[i].gd = b->hres[2 * i];
   for (int i : tl) b->slots[i].gd = b->hres[2 * i];
+  for (int i : tlvf) b->slots[i].gd = b->hres[2 * i];
   return A2C_OK;
 }
 
@@ -1647,8 +1872,13 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
   a2c_batch* b = new (std::nothrow) a2c_batch;
   if (!b) return A2C_ENOMEM;
   b->desc = *desc;
-  const char* tl = std::getenv("A2C_TWO_LOOP");  // "segmented": every slot takes the segmented kernels
-  b->fused_two_loop = !(tl && std::string(tl) == "segmented");
+  // A2C_TWO_LOOP: "segmented" sends every slot through the segmented kernels,
+  // "perstep" every recursion through the pass-per-step form (both may be given,
+  // comma-separated)
+  const char* tl = std::getenv("A2C_TWO_LOOP");
+  const std::string tls = tl ? tl : "";
+  b->fused_two_loop = tls.find("segmented") == std::string::npos;
+  b->vector_free = tls.find("perstep") == std::string::npos;
   auto fail = [&](int rc) { a2c_batch_destroy(b); return rc; };
   if (hipSetDevice(desc->device) != hipSuccess) return fail(A2C_EHIP);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(A2C_EHIP);

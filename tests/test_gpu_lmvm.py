@@ -138,28 +138,33 @@ def test_preconditioned_and_plain_metrics_reach_the_same_state():
     print("iterations: precond %d, plain %d" % (r1["iterations"], r0["iterations"]))
 
 
-def test_fused_two_loop_matches_the_segmented_recursion(monkeypatch):
-    """Slots of up to 8,192 sites run the two-loop recursion in one workgroup;
-    A2C_TWO_LOOP=segmented sends them through the segmented kernels that
-    larger slots use.  Same iterations; states agree to rounding after a few
-    iterations and to 1e-6 A at convergence."""
+@pytest.mark.parametrize("m", [5, 8])
+def test_two_loop_forms_agree(monkeypatch, m):
+    """The two-loop recursion has four implementations: in one workgroup
+    (slots of up to 8,192 sites) or as grid passes (larger slots; forced by
+    A2C_TWO_LOOP=segmented), each in vector-free form (up to 5 pairs: one pass
+    of dot products, the loops on scalars) or with one pass per step (more
+    pairs; forced by A2C_TWO_LOOP=perstep).  All four take the same
+    iterations; states agree to rounding after a few iterations and to 1e-6 A
+    at convergence.  m = 8 exercises both forms within one relaxation."""
     X, x, box, per, npd, nh = specimen(4, seed=7, disp=0.03, sigma=(0.08, 0.15), x=(0.05, 0.5))
     out = {}
-    for mode in ("fused", "segmented"):
-        if mode == "segmented":
-            monkeypatch.setenv("A2C_TWO_LOOP", "segmented")
+    for mode in ("", "segmented", "perstep", "segmented,perstep"):
+        monkeypatch.setenv("A2C_TWO_LOOP", mode)
         fe = A.FreeEntropy(npd, nh, box, per, T=400.0, precision="fp64")
         fe.set_occupancy(x)
         fe.findNeighbors(X)
-        out[mode] = (fe.minimizeFreeEntropy(X, max_it=6), fe.minimizeFreeEntropy(X, gatol=1e-9, grtol=0.0))
+        out[mode] = (fe.minimizeFreeEntropy(X, max_it=8, m=m),
+                     fe.minimizeFreeEntropy(X, gatol=1e-9, grtol=0.0, m=m))
         fe.close()
-    (Xa, ra), (Xc, rc) = out["fused"][0], out["segmented"][0]
-    assert ra["iterations"] == rc["iterations"] == 6
-    assert np.max(np.abs(Xa - Xc)) <= 1e-9
-    (Xa, ra), (Xc, rc) = out["fused"][1], out["segmented"][1]
-    assert ra["reason"] > 0 and rc["reason"] > 0
-    dq, ds = compare(Xa, Xc)
-    assert dq <= 1e-6 and ds <= 1e-6
+    (Xa, ra), (Xb, rb) = out[""]
+    assert ra["iterations"] == 8 and rb["reason"] > 0
+    for mode in ("segmented", "perstep", "segmented,perstep"):
+        (Xc, rc), (Xd, rd) = out[mode]
+        assert rc["iterations"] == 8 and np.max(np.abs(Xa - Xc)) <= 1e-9, mode
+        assert rd["reason"] > 0
+        dq, ds = compare(Xb, Xd)
+        assert dq <= 1e-6 and ds <= 1e-6, mode
 
 
 def test_relaxation_of_a_lattice_above_the_fused_size():
