@@ -1121,9 +1121,9 @@ constexpr int PART_PER_BLOCK = VF_SUMS > TL_PART ? VF_SUMS : TL_PART;
 int ensure_part(a2c_batch* b, size_t nblk) {
   if (nblk <= b->part_cap) return A2C_OK;
   if (int rc = bsync(b)) return rc;
-  (void)hipFree(b->dpart);
+  (void)dfree(b->dpart);
   b->dpart = nullptr;
-  B_HIP(b, hipMalloc(&b->dpart, sizeof(double) * PART_PER_BLOCK * nblk));
+  B_HIP(b, dmalloc(&b->dpart, sizeof(double) * PART_PER_BLOCK * nblk));
   b->part_cap = nblk;
   return A2C_OK;
 }
@@ -1420,9 +1420,9 @@ int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf sta
 
 // ---------------------------------------------------------------- optimizer
 void lmvm_free(Slot& s) {
-  for (double* p : {s.x, s.g, s.gp, s.d, s.xo, s.go, s.q}) (void)hipFree(p);
-  for (double* p : s.S) (void)hipFree(p);
-  for (double* p : s.Y) (void)hipFree(p);
+  for (double* p : {s.x, s.g, s.gp, s.d, s.xo, s.go, s.q}) (void)dfree(p);
+  for (double* p : s.S) (void)dfree(p);
+  for (double* p : s.Y) (void)dfree(p);
   s.x = s.g = s.gp = s.d = s.xo = s.go = s.q = nullptr;
   s.S.clear();
   s.Y.clear();
@@ -1431,12 +1431,12 @@ void lmvm_free(Slot& s) {
 int lmvm_alloc(a2c_batch* b, Slot& s) {
   lmvm_free(s);
   const size_t n = std::max<size_t>(4 * (size_t)s.N(), 4);
-  for (double** p : {&s.x, &s.g, &s.gp, &s.d, &s.xo, &s.go, &s.q}) B_HIP(b, hipMalloc(p, n * sizeof(double)));
+  for (double** p : {&s.x, &s.g, &s.gp, &s.d, &s.xo, &s.go, &s.q}) B_HIP(b, dmalloc(p, n * sizeof(double)));
   s.S.assign(s.m, nullptr);
   s.Y.assign(s.m, nullptr);
   for (int i = 0; i < s.m; ++i) {
-    B_HIP(b, hipMalloc(&s.S[i], n * sizeof(double)));
-    B_HIP(b, hipMalloc(&s.Y[i], n * sizeof(double)));
+    B_HIP(b, dmalloc(&s.S[i], n * sizeof(double)));
+    B_HIP(b, dmalloc(&s.Y[i], n * sizeof(double)));
   }
   s.rho.assign(s.m, 0.0);
   s.alpha.assign(s.m, 0.0);
@@ -1885,18 +1885,18 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
   const size_t cap = desc->max_lattices;
   b->slots.resize(cap);
   b->arena.cap = (size_t)32 << 20;
-  bool ok = hipHostMalloc(&b->arena.h, b->arena.cap) == hipSuccess && hipMalloc(&b->arena.d, b->arena.cap) == hipSuccess &&
-            hipMalloc(&b->dF, 8 * cap) == hipSuccess && hipHostMalloc(&b->hF, 8 * cap) == hipSuccess &&
-            hipMalloc(&b->dres, 16 * cap) == hipSuccess && hipHostMalloc(&b->hres, 16 * cap) == hipSuccess &&
-            hipMalloc(&b->dscal, sizeof(double) * SCAL_PER_SLOT * cap) == hipSuccess;
+  bool ok = hipHostMalloc(&b->arena.h, b->arena.cap) == hipSuccess && dmalloc(&b->arena.d, b->arena.cap) == hipSuccess &&
+            dmalloc(&b->dF, 8 * cap) == hipSuccess && hipHostMalloc(&b->hF, 8 * cap) == hipSuccess &&
+            dmalloc(&b->dres, 16 * cap) == hipSuccess && hipHostMalloc(&b->hres, 16 * cap) == hipSuccess &&
+            dmalloc(&b->dscal, sizeof(double) * SCAL_PER_SLOT * cap) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
   std::vector<float4> t;
   host_tables(t);
-  if (hipMalloc(&b->dtab, sizeof(float4) * t.size()) != hipSuccess) return fail(A2C_ENOMEM);
+  if (dmalloc(&b->dtab, sizeof(float4) * t.size()) != hipSuccess) return fail(A2C_ENOMEM);
   if (hipMemcpy(b->dtab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice) != hipSuccess) return fail(A2C_EHIP);
   std::vector<double> t64;
   host_tables64(t64);
-  if (hipMalloc(&b->dtab64, sizeof(double) * t64.size()) != hipSuccess) return fail(A2C_ENOMEM);
+  if (dmalloc(&b->dtab64, sizeof(double) * t64.size()) != hipSuccess) return fail(A2C_ENOMEM);
   if (hipMemcpy(b->dtab64, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(A2C_EHIP);
   *out = b;
@@ -1911,16 +1911,16 @@ void a2c_batch_destroy(a2c_batch* b) {
     lmvm_free(s);
     if (s.owned) a2c_fe_destroy(s.fe);
   }
-  (void)hipFree(b->arena.d);
+  (void)dfree(b->arena.d);
   if (b->arena.h) (void)hipHostFree(b->arena.h);
-  (void)hipFree(b->dF);
+  (void)dfree(b->dF);
   if (b->hF) (void)hipHostFree(b->hF);
-  (void)hipFree(b->dres);
-  (void)hipFree(b->dscal);
+  (void)dfree(b->dres);
+  (void)dfree(b->dscal);
   if (b->hres) (void)hipHostFree(b->hres);
-  (void)hipFree(b->dpart);
-  (void)hipFree(b->dtab);
-  (void)hipFree(b->dtab64);
+  (void)dfree(b->dpart);
+  (void)dfree(b->dtab);
+  (void)dfree(b->dtab64);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

@@ -87,14 +87,14 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(A2C_EHIP);
   const size_t N = std::max(c->N, 1), nh = std::max(desc->n_h, 1);
   c->nblk_red = (int)((N + 255) / 256);
-  bool ok = hipMalloc(&c->dX, 32 * N) == hipSuccess && hipMalloc(&c->dG, 32 * N) == hipSuccess &&
-            hipMalloc(&c->dx, 8 * nh) == hipSuccess && hipMalloc(&c->drho, 8 * N) == hipSuccess &&
-            hipMalloc(&c->dFp, 8 * N) == hipSuccess && hipMalloc(&c->demb, 8 * N) == hipSuccess &&
-            hipMalloc(&c->dpart, 8 * (size_t)c->nblk_red) == hipSuccess && hipMalloc(&c->dF, 64) == hipSuccess &&
+  bool ok = dmalloc(&c->dX, 32 * N) == hipSuccess && dmalloc(&c->dG, 32 * N) == hipSuccess &&
+            dmalloc(&c->dx, 8 * nh) == hipSuccess && dmalloc(&c->drho, 8 * N) == hipSuccess &&
+            dmalloc(&c->dFp, 8 * N) == hipSuccess && dmalloc(&c->demb, 8 * N) == hipSuccess &&
+            dmalloc(&c->dpart, 8 * (size_t)c->nblk_red) == hipSuccess && dmalloc(&c->dF, 64) == hipSuccess &&
             hipHostMalloc(&c->hX, 32 * N) == hipSuccess && hipHostMalloc(&c->hG, 32 * N) == hipSuccess &&
-            hipHostMalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && hipMalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
-            hipMalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess &&
-            hipMalloc(&c->dtab64, sizeof(double) * T64_SIZE) == hipSuccess;
+            hipHostMalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && dmalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
+            dmalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess &&
+            dmalloc(&c->dtab64, sizeof(double) * T64_SIZE) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
   {
     std::vector<double> t64;
@@ -114,20 +114,20 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
 void a2c_fe_destroy(a2c_fe_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
-  hipFree(c->dX); hipFree(c->dG); hipFree(c->dx); hipFree(c->drho); hipFree(c->dFp);
-  hipFree(c->demb); hipFree(c->dpart); hipFree(c->dF); hipFree(c->dpdpdnum); hipFree(c->dtab); hipFree(c->daux);
-  hipFree(c->dtab64);
+  dfree(c->dX); dfree(c->dG); dfree(c->dx); dfree(c->drho); dfree(c->dFp);
+  dfree(c->demb); dfree(c->dpart); dfree(c->dF); dfree(c->dpdpdnum); dfree(c->dtab); dfree(c->daux);
+  dfree(c->dtab64);
   for (int w = 0; w < 5; ++w) {
-    hipFree(c->ell[w].idx); hipFree(c->ell[w].cnt);
+    dfree(c->ell[w].idx); dfree(c->ell[w].cnt);
     a2c::free_pair_buffers(c->ell[w]);
   }
-  hipFree(c->depair);
-  hipFree(c->nb_cid); hipFree(c->nb_order); hipFree(c->nb_cc); hipFree(c->nb_cursor); hipFree(c->nb_misc);
+  dfree(c->depair);
+  dfree(c->nb_cid); dfree(c->nb_order); dfree(c->nb_cc); dfree(c->nb_cursor); dfree(c->nb_misc);
   if (c->hX) hipHostFree(c->hX);
   if (c->hG) hipHostFree(c->hG);
   if (c->hlat) hipHostFree(c->hlat);
-  hipFree(c->dlat);
-  hipFree(c->db2l);
+  dfree(c->dlat);
+  dfree(c->db2l);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }

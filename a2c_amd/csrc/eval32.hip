@@ -754,8 +754,8 @@ void host_tables(std::vector<float4>& t) {
 int build_tables(a2c_fe_ctx* c) {
   std::vector<float4> t;
   host_tables(t);
-  if (!c->dtab) A2C_HIP(c, hipMalloc(&c->dtab, sizeof(float4) * t.size()));
-  if (!c->daux) A2C_HIP(c, hipMalloc(&c->daux, sizeof(float2) * std::max(c->N, 1)));
+  if (!c->dtab) A2C_HIP(c, dmalloc(&c->dtab, sizeof(float4) * t.size()));
+  if (!c->daux) A2C_HIP(c, dmalloc(&c->daux, sizeof(float2) * std::max(c->N, 1)));
   A2C_HIP(c, hipMemcpy(c->dtab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice));
   return A2C_OK;
 }

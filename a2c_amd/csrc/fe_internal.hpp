@@ -10,6 +10,15 @@
 
 namespace a2c {
 
+// Device buffers of the library (devmem.cpp): a caching allocator with
+// hipMalloc/hipFree semantics (dfree synchronises the block's device first).
+hipError_t dmalloc(void** ptr, size_t bytes);
+template <class T>
+inline hipError_t dmalloc(T** ptr, size_t bytes) {
+  return dmalloc(reinterpret_cast<void**>(ptr), bytes);
+}
+hipError_t dfree(const void* ptr);
+
 constexpr double KB = 8.6173324e-5;  // Boltzmann constant, eV/K (minimizer.cpp:92)
 constexpr int NBR_CAP = 256;          // per-site neighbor cap of the build kernel
 

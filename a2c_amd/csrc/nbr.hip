@@ -321,16 +321,16 @@ int nbr_build(a2c_fe_ctx* c) {
   if (ncell > (1LL << 28)) return set_err(c, A2C_EINVAL, "neighbor grid too large");
   // scratch buffers persist across rebuilds (grow-only)
   if (!c->nb_cid) {
-    A2C_HIP(c, hipMalloc(&c->nb_cid, sizeof(int) * std::max(N, 1)));
-    A2C_HIP(c, hipMalloc(&c->nb_order, sizeof(int) * std::max(N, 1)));
-    A2C_HIP(c, hipMalloc(&c->nb_misc, sizeof(int) * 8));
+    A2C_HIP(c, dmalloc(&c->nb_cid, sizeof(int) * std::max(N, 1)));
+    A2C_HIP(c, dmalloc(&c->nb_order, sizeof(int) * std::max(N, 1)));
+    A2C_HIP(c, dmalloc(&c->nb_misc, sizeof(int) * 8));
   }
   if (ncell + 1 > c->nb_cap_cells) {
-    (void)hipFree(c->nb_cc);
-    (void)hipFree(c->nb_cursor);
+    (void)dfree(c->nb_cc);
+    (void)dfree(c->nb_cursor);
     c->nb_cc = c->nb_cursor = nullptr;
-    A2C_HIP(c, hipMalloc(&c->nb_cc, sizeof(int) * (ncell + 1)));
-    A2C_HIP(c, hipMalloc(&c->nb_cursor, sizeof(int) * (ncell + 1)));
+    A2C_HIP(c, dmalloc(&c->nb_cc, sizeof(int) * (ncell + 1)));
+    A2C_HIP(c, dmalloc(&c->nb_cursor, sizeof(int) * (ncell + 1)));
     c->nb_cap_cells = ncell + 1;
   }
   int *dcid = c->nb_cid, *dcc = c->nb_cc, *dorder = c->nb_order, *dmisc = c->nb_misc, *dcursor = c->nb_cursor;
@@ -356,7 +356,7 @@ int nbr_build(a2c_fe_ctx* c) {
   for (int w = 0; w < 5; ++w) {
     Ell& L = c->ell[w];
     L.rows = (w < 2) ? npd : nh;
-    if (!L.cnt) A2C_HIP(c, hipMalloc(&L.cnt, sizeof(int32_t) * std::max(L.rows, 1)));
+    if (!L.cnt) A2C_HIP(c, dmalloc(&L.cnt, sizeof(int32_t) * std::max(L.rows, 1)));
     A2C_HIP(c, hipMemsetAsync(L.cnt, 0, sizeof(int32_t) * std::max(L.rows, 1), c->stream));
   }
   NbrArgs a{};
@@ -386,16 +386,16 @@ int nbr_build(a2c_fe_ctx* c) {
     L.width = hm[w];
     const int64_t n = (int64_t)std::max(hm[w], 1) * std::max(L.rows, 1);
     if (n > L.cap) {  // grow with slack so small changes of the widest row reuse the arrays
-      (void)hipFree(L.idx);
+      (void)dfree(L.idx);
       L.idx = nullptr;
       L.cap = (int64_t)(std::max(hm[w], 1) + 4) * std::max(L.rows, 1);
-      A2C_HIP(c, hipMalloc(&L.idx, sizeof(int32_t) * L.cap));
+      A2C_HIP(c, dmalloc(&L.idx, sizeof(int32_t) * L.cap));
     }
     A2C_HIP(c, hipMemsetAsync(L.idx, 0xff, sizeof(int32_t) * n, c->stream));
     a.idx[w] = L.idx;
     a.width[w] = hm[w];
   }
-  if (!c->dpdpdnum) A2C_HIP(c, hipMalloc(&c->dpdpdnum, sizeof(int32_t) * 3 * std::max(npd, 1)));
+  if (!c->dpdpdnum) A2C_HIP(c, dmalloc(&c->dpdpdnum, sizeof(int32_t) * 3 * std::max(npd, 1)));
   a.pdpdnum = c->dpdpdnum;
   if (N > 0) {
     hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), 0, c->stream, a);
@@ -420,8 +420,8 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
   for (int w = 0; w < 4; ++w) {
     Ell& L = c->ell[w];
     if (!L.ocnt) {
-      A2C_HIP(c, hipMalloc(&L.ocnt, sizeof(int32_t) * std::max(L.rows, 1)));
-      A2C_HIP(c, hipMalloc(&L.rcnt, sizeof(int32_t) * std::max(L.rows, 1)));
+      A2C_HIP(c, dmalloc(&L.ocnt, sizeof(int32_t) * std::max(L.rows, 1)));
+      A2C_HIP(c, dmalloc(&L.rcnt, sizeof(int32_t) * std::max(L.rows, 1)));
     }
   }
   A2C_HIP(c, hipMemsetAsync(c->nb_misc, 0, sizeof(int) * 8, c->stream));
@@ -449,17 +449,17 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
     const int64_t on = (int64_t)std::max(L.owidth, 1) * rows, rn = (int64_t)std::max(L.rwidth, 1) * rows;
     const int64_t bn = std::max<int64_t>(32 * rn, 4 * (int64_t)std::max(L.width, 1) * rows);
     if (on > L.ocap) {
-      (void)hipFree(L.oj); (void)hipFree(L.orv);
+      (void)dfree(L.oj); (void)dfree(L.orv);
       L.oj = L.orv = nullptr;
       L.ocap = on + 4 * rows;  // slack: small changes of the widest half reuse the arrays
-      A2C_HIP(c, hipMalloc(&L.oj, sizeof(int32_t) * L.ocap));
-      A2C_HIP(c, hipMalloc(&L.orv, sizeof(int32_t) * L.ocap));
+      A2C_HIP(c, dmalloc(&L.oj, sizeof(int32_t) * L.ocap));
+      A2C_HIP(c, dmalloc(&L.orv, sizeof(int32_t) * L.ocap));
     }
     if (bn > L.bcap) {
-      (void)hipFree(L.rbuf);
+      (void)dfree(L.rbuf);
       L.rbuf = nullptr;
       L.bcap = bn + 32 * 4 * rows;
-      A2C_HIP(c, hipMalloc(&L.rbuf, L.bcap));
+      A2C_HIP(c, dmalloc(&L.rbuf, L.bcap));
     }
   }
   // 3. split, then the partner slots (needs every list's split)
@@ -477,15 +477,15 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
     hipLaunchKernelGGL(k_orv, dim3((L.rows + TB - 1) / TB), dim3(TB), 0, c->stream, L, c->ell[partner[w]]);
     A2C_HIP(c, hipGetLastError());
   }
-  if (!c->depair) A2C_HIP(c, hipMalloc(&c->depair, sizeof(double) * std::max(c->N, 1)));
+  if (!c->depair) A2C_HIP(c, dmalloc(&c->depair, sizeof(double) * std::max(c->N, 1)));
   A2C_HIP(c, hipStreamSynchronize(c->stream));
   c->lat_dirty = true;
   return A2C_OK;
 }
 
 void free_pair_buffers(Ell& L) {
-  (void)hipFree(L.ocnt); (void)hipFree(L.rcnt); (void)hipFree(L.oj); (void)hipFree(L.orv);
-  (void)hipFree(L.rbuf);
+  (void)dfree(L.ocnt); (void)dfree(L.rcnt); (void)dfree(L.oj); (void)dfree(L.orv);
+  (void)dfree(L.rbuf);
   L.ocnt = L.rcnt = L.oj = L.orv = nullptr;
   L.rbuf = nullptr;
   L.ocap = L.bcap = 0;
