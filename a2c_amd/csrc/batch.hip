@@ -1885,9 +1885,9 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
   const size_t cap = desc->max_lattices;
   b->slots.resize(cap);
   b->arena.cap = (size_t)32 << 20;
-  bool ok = hipHostMalloc(&b->arena.h, b->arena.cap) == hipSuccess && dmalloc(&b->arena.d, b->arena.cap) == hipSuccess &&
-            dmalloc(&b->dF, 8 * cap) == hipSuccess && hipHostMalloc(&b->hF, 8 * cap) == hipSuccess &&
-            dmalloc(&b->dres, 16 * cap) == hipSuccess && hipHostMalloc(&b->hres, 16 * cap) == hipSuccess &&
+  bool ok = hmalloc(&b->arena.h, b->arena.cap) == hipSuccess && dmalloc(&b->arena.d, b->arena.cap) == hipSuccess &&
+            dmalloc(&b->dF, 8 * cap) == hipSuccess && hmalloc(&b->hF, 8 * cap) == hipSuccess &&
+            dmalloc(&b->dres, 16 * cap) == hipSuccess && hmalloc(&b->hres, 16 * cap) == hipSuccess &&
             dmalloc(&b->dscal, sizeof(double) * SCAL_PER_SLOT * cap) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
   std::vector<float4> t;
@@ -1912,12 +1912,12 @@ void a2c_batch_destroy(a2c_batch* b) {
     if (s.owned) a2c_fe_destroy(s.fe);
   }
   (void)dfree(b->arena.d);
-  if (b->arena.h) (void)hipHostFree(b->arena.h);
+  if (b->arena.h) (void)hfree(b->arena.h);
   (void)dfree(b->dF);
-  if (b->hF) (void)hipHostFree(b->hF);
+  if (b->hF) (void)hfree(b->hF);
   (void)dfree(b->dres);
   (void)dfree(b->dscal);
-  if (b->hres) (void)hipHostFree(b->hres);
+  if (b->hres) (void)hfree(b->hres);
   (void)dfree(b->dpart);
   (void)dfree(b->dtab);
   (void)dfree(b->dtab64);
@@ -1964,7 +1964,7 @@ int a2c_batch_add(a2c_batch* b, const a2c_fe_desc* lat, const double* X, const d
   d.fd_eps = b->desc.fd_eps;
   if (d.max_neighbors == 0) d.max_neighbors = b->desc.max_neighbors;
   a2c_fe_ctx* fe = nullptr;
-  int rc = a2c_fe_create(&d, &fe);
+  int rc = fe_create(&d, b->stream, &fe);
   if (rc) return berr(b, rc, "lattice context creation failed");
   if ((rc = a2c_fe_set_occupancy(fe, x)) || (rc = a2c_fe_build_neighbors(fe, X, nullptr))) {
     b->err = a2c_fe_last_error(fe);

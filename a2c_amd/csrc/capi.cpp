@@ -65,7 +65,11 @@ void a2c_fe_desc_default(a2c_fe_desc* d) {
   d->max_neighbors = NBR_CAP;
 }
 
-int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
+int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) { return a2c::fe_create(desc, nullptr, out); }
+
+}  // extern "C"
+
+int a2c::fe_create(const a2c_fe_desc* desc, hipStream_t shared, a2c_fe_ctx** out) {
   if (!desc || !out) return A2C_EINVAL;
   *out = nullptr;
   if (desc->n_pd < 0 || desc->n_h < 0 || (int64_t)desc->n_pd + desc->n_h > (1 << 30)) return A2C_EINVAL;
@@ -84,15 +88,20 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
   c->N = desc->n_pd + desc->n_h;
   auto fail = [&](int code) { a2c_fe_destroy(c); return code; };
   if (hipSetDevice(desc->device) != hipSuccess) return fail(A2C_EHIP);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(A2C_EHIP);
+  if (shared) {
+    c->stream = shared;
+    c->own_stream = false;
+  } else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    return fail(A2C_EHIP);
+  }
   const size_t N = std::max(c->N, 1), nh = std::max(desc->n_h, 1);
   c->nblk_red = (int)((N + 255) / 256);
   bool ok = dmalloc(&c->dX, 32 * N) == hipSuccess && dmalloc(&c->dG, 32 * N) == hipSuccess &&
             dmalloc(&c->dx, 8 * nh) == hipSuccess && dmalloc(&c->drho, 8 * N) == hipSuccess &&
             dmalloc(&c->dFp, 8 * N) == hipSuccess && dmalloc(&c->demb, 8 * N) == hipSuccess &&
             dmalloc(&c->dpart, 8 * (size_t)c->nblk_red) == hipSuccess && dmalloc(&c->dF, 64) == hipSuccess &&
-            hipHostMalloc(&c->hX, 32 * N) == hipSuccess && hipHostMalloc(&c->hG, 32 * N) == hipSuccess &&
-            hipHostMalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && dmalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
+            hmalloc(&c->hX, 32 * N) == hipSuccess && hmalloc(&c->hG, 32 * N) == hipSuccess &&
+            hmalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && dmalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
             dmalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess &&
             dmalloc(&c->dtab64, sizeof(double) * T64_SIZE) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
@@ -111,6 +120,8 @@ int a2c_fe_create(const a2c_fe_desc* desc, a2c_fe_ctx** out) {
   return A2C_OK;
 }
 
+extern "C" {
+
 void a2c_fe_destroy(a2c_fe_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
@@ -123,12 +134,12 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   }
   dfree(c->depair);
   dfree(c->nb_cid); dfree(c->nb_order); dfree(c->nb_cc); dfree(c->nb_cursor); dfree(c->nb_misc);
-  if (c->hX) hipHostFree(c->hX);
-  if (c->hG) hipHostFree(c->hG);
-  if (c->hlat) hipHostFree(c->hlat);
+  if (c->hX) hfree(c->hX);
+  if (c->hG) hfree(c->hG);
+  if (c->hlat) hfree(c->hlat);
   dfree(c->dlat);
   dfree(c->db2l);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
   delete c;
 }
 

@@ -13,7 +13,10 @@
 // Release synchronises the block's device first, as hipFree does: a block
 // may still be read by work in flight on any stream, and it must not be
 // handed out again before that work ends.  Memory stays with the process
-// (reused, never returned to the device).
+// (reused, never returned to the device).  Pinned host staging buffers
+// (hmalloc / hfree) are cached the same way, whole (hipHostMalloc pins
+// pages, ~0.1-0.5 ms a call; release waits for the device, whose copies
+// may still read them).
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -51,6 +54,8 @@ struct Pool {
   std::map<std::pair<int, size_t>, std::vector<void*>> free_blocks;  // (device, rounded size)
   std::unordered_map<const void*, std::pair<int, size_t>> owner;      // block -> (device, rounded size)
   std::map<int, Slab> slab;                                           // current slab per device
+  std::map<size_t, std::vector<void*>> free_host;                     // pinned host, by rounded size
+  std::unordered_map<const void*, size_t> host_owner;
 };
 
 Pool& pool() {
@@ -112,6 +117,40 @@ hipError_t dfree(const void* ptr) {
   std::lock_guard<std::mutex> lk(P.mu);
   P.owner.erase(ptr);
   P.free_blocks[o].push_back(const_cast<void*>(ptr));
+  return e;
+}
+
+hipError_t hmalloc(void** ptr, size_t bytes) {
+  if (!ptr) return hipErrorInvalidValue;
+  *ptr = nullptr;
+  const size_t sz = round_size(bytes);
+  Pool& P = pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.free_host.find(sz);
+  if (it != P.free_host.end() && !it->second.empty()) {
+    *ptr = it->second.back();
+    it->second.pop_back();
+  } else if (hipError_t e = hipHostMalloc(ptr, sz)) {
+    return e;
+  }
+  P.host_owner[*ptr] = sz;
+  return hipSuccess;
+}
+
+hipError_t hfree(const void* ptr) {
+  if (!ptr) return hipSuccess;
+  Pool& P = pool();
+  size_t sz = 0;
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.host_owner.find(ptr);
+    if (it == P.host_owner.end()) return hipErrorInvalidValue;
+    sz = it->second;
+  }
+  const hipError_t e = hipDeviceSynchronize();  // copies in flight may still read it
+  std::lock_guard<std::mutex> lk(P.mu);
+  P.host_owner.erase(ptr);
+  P.free_host[sz].push_back(const_cast<void*>(ptr));
   return e;
 }
 

@@ -18,6 +18,12 @@ inline hipError_t dmalloc(T** ptr, size_t bytes) {
   return dmalloc(reinterpret_cast<void**>(ptr), bytes);
 }
 hipError_t dfree(const void* ptr);
+hipError_t hmalloc(void** ptr, size_t bytes);  // pinned host staging, cached alike
+template <class T>
+inline hipError_t hmalloc(T** ptr, size_t bytes) {
+  return hmalloc(reinterpret_cast<void**>(ptr), bytes);
+}
+hipError_t hfree(const void* ptr);
 
 constexpr double KB = 8.6173324e-5;  // Boltzmann constant, eV/K (minimizer.cpp:92)
 constexpr int NBR_CAP = 256;          // per-site neighbor cap of the build kernel
@@ -157,11 +163,17 @@ __device__ inline int xcd_block() {
 
 struct Timing;
 
+// a2c_fe_create on a stream of the caller's (a batch's own lattices share
+// its stream: creating and destroying one per lattice cost ~1.3 ms), or on a
+// stream of its own when shared is null
+int fe_create(const a2c_fe_desc* desc, hipStream_t shared, a2c_fe_ctx** out);
+
 }  // namespace a2c
 
 struct a2c_fe_ctx {
   a2c_fe_desc desc;
   int32_t N = 0;
+  bool own_stream = true;
   hipStream_t stream = nullptr;
   std::string err;
 
