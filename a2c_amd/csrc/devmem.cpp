@@ -13,7 +13,10 @@
 // Release synchronises the block's device first, as hipFree does: a block
 // may still be read by work in flight on any stream, and it must not be
 // handed out again before that work ends.  Memory stays with the process
-// (reused, never returned to the device).  Pinned host staging buffers
+// and is reused; when the device runs out, the cached blocks that are whole
+// allocations (requests above 16 MiB) go back to it and the request is tried
+// once more, so a loop over many lattice sizes cannot exhaust HBM through
+// the cache's large blocks.  Pinned host staging buffers
 // (hmalloc / hfree) are cached the same way, whole (hipHostMalloc pins
 // pages, ~0.1-0.5 ms a call; release waits for the device, whose copies
 // may still read them).
@@ -63,6 +66,19 @@ Pool& pool() {
   return *p;
 }
 
+// hipMalloc, and on failure once more after returning the device's cached
+// whole allocations (caller holds the pool lock; those blocks are idle)
+hipError_t device_alloc(Pool& P, int dev, void** out, size_t bytes) {
+  if (hipMalloc(out, bytes) == hipSuccess) return hipSuccess;
+  (void)hipGetLastError();
+  for (auto it = P.free_blocks.begin(); it != P.free_blocks.end(); ++it)
+    if (it->first.first == dev && it->first.second > CARVE_MAX) {
+      for (void* b : it->second) (void)hipFree(b);
+      it->second.clear();
+    }
+  return hipMalloc(out, bytes);
+}
+
 }  // namespace
 
 hipError_t dmalloc(void** ptr, size_t bytes) {
@@ -82,12 +98,12 @@ hipError_t dmalloc(void** ptr, size_t bytes) {
   }
   void* blk = nullptr;
   if (sz > CARVE_MAX) {
-    if (hipError_t e = hipMalloc(&blk, sz)) return e;
+    if (hipError_t e = device_alloc(P, dev, &blk, sz)) return e;
   } else {
     Slab& s = P.slab[dev];
     if (!s.base || s.used + sz > SLAB) {  // the rest of a full slab stays unused
       void* b = nullptr;
-      if (hipError_t e = hipMalloc(&b, SLAB)) return e;
+      if (hipError_t e = device_alloc(P, dev, &b, SLAB)) return e;
       s.base = static_cast<char*>(b);
       s.used = 0;
     }
