@@ -10,13 +10,16 @@
 //
 // Optimizer.  The single-lattice minimizer (a2c_fe_minimize) runs on this
 // code with a batch of one, so it is the same algorithm: L-BFGS two-loop
-// recursion (m pairs, H0 = s'y/y'y), active-set handling of the sigma >= lb
-// bound, Moré-Thuente line search with approximate Wolfe acceptance in FP64,
-// steepest-descent restart on failure, TAO's gatol/grtol/gttol tests, and the
-// mixed -> FP64 switch.  In a batch every step is one launch over all slots
-// that need it (segmented kernels), and each scalar decision is taken per slot
-// on the host after one synchronisation per round.  Slots can be added and
-// started between iterations (continuous batching).
+// recursion (m pairs; initial inverse Hessian diag(gamma_g) D, D the
+// occupancy metric and gamma_g = s.y / y.D y of the newest pair over each
+// coordinate group), active-set handling of the sigma bounds, Moré-Thuente
+// line search with approximate Wolfe acceptance in FP64, steepest-descent
+// restart on failure, TAO's gatol/grtol/gttol tests, and the mixed -> FP64
+// switch.  In a batch every step is one launch over all slots that need it
+// (segmented kernels; the recursion of a small slot in one workgroup), and
+// each scalar decision is taken per slot on the host after one
+// synchronisation per round.  Slots can be added and started between
+// iterations (continuous batching).
 //
 // Launch sequences.  Every operation between two host synchronisations is
 // queued with a key made of its kernel and all its argument values; the

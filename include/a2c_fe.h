@@ -148,10 +148,13 @@ typedef struct a2c_lmvm_opts {
                            -3 kB x ln sigma outgrows the vanishing pair terms
                            beyond a few A), so long relaxations of large
                            lattices may set a finite cap. */
-  int32_t precond;      /* 1: initial inverse Hessian gamma * diag(1/w), w = 1
-                           for Pd sites and the occupancy x for H sites (every
-                           term of an H packet carries x, so its curvature
-                           does); 0: gamma * I as TAO's BLMVM (1) */
+  int32_t precond;      /* 1: initial inverse Hessian diag(gamma_g) diag(1/w),
+                           w = 1 for Pd sites and the occupancy x for H sites
+                           (every term of an H packet carries x, so its
+                           curvature does); 0: w = 1 (1).  gamma_g = s.y /
+                           y.diag(1/w) y of the newest pair over coordinate
+                           group g -- Pd positions, Pd widths, H positions, H
+                           widths (TAO's BLMVM scales by one gamma) */
 } a2c_lmvm_opts;
 
 typedef struct a2c_lmvm_report {
