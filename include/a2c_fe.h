@@ -15,6 +15,12 @@
  *
  * Threading: one context per host thread.  Calls are synchronous on the
  * context's HIP stream unless named *_device / *_async.
+ *
+ * Memory: the device buffers of contexts and batches come from a
+ * process-wide cache; destroying a context returns them to the cache for
+ * the next context, not to the device (large cached blocks go back to the
+ * device when an allocation would otherwise fail).  a2c_dev_alloc memory is
+ * allocated and freed directly.
  */
 #ifndef A2C_FE_H
 #define A2C_FE_H
