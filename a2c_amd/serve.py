@@ -174,6 +174,10 @@ def main():
         if not a.no_baseline:
             d1, w1 = serve_one_at_a_time([dict(r) for r in reqs])
             row["one_at_a_time_fp64"] = summarize(d1, w1)
+            # relaxed objectives of the served (mixed) requests against the FP64 ones
+            f64 = {d["id"]: d["report"]["f"] for d in d1}
+            row["relaxed_f_rel_vs_fp64"] = max(abs(d["report"]["f"] - f64[d["id"]]) / abs(f64[d["id"]])
+                                               for d in done)
         print(json.dumps(row), flush=True)
         out.append(row)
     return out
