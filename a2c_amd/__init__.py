@@ -106,7 +106,7 @@ class AutotuneOpts(C.Structure):
 
 class AutotuneReport(C.Structure):
     _fields_ = [("sites", C.c_int32 * TUNE_MAX), ("copies", C.c_int32 * TUNE_MAX),
-                ("ns_per_site", C.c_double * 4 * 2 * TUNE_MAX), ("us_alone", C.c_double * 4 * 2 * TUNE_MAX),
+                ("ns_per_site", C.c_double * 5 * 2 * TUNE_MAX), ("us_alone", C.c_double * 5 * 2 * TUNE_MAX),
                 ("latency", C.c_int32 * 2 * TUNE_MAX)]
 
 
@@ -704,7 +704,7 @@ def autotune(cells=None, target_sites=None, reps=None, apply=False, device=0):
     """Measure lanes per site for each lattice size (include/a2c_tune.h).
     Returns (table, report) where report lists, per size, sites, copies, ns
     per site of a fused launch of the copies and microseconds per evaluation
-    of one lattice alone, for K = 1, 2, 4, 8 in mixed and FP64 precision, and
+    of one lattice alone, for K = 1, 2, 4, 8, 16 in mixed and FP64 precision, and
     whether the lone-lattice time chose K ("latency")."""
     o = AutotuneOpts()
     lib.a2c_autotune_opts_default(C.byref(o))

@@ -124,7 +124,7 @@ constexpr int RECV_RU_SCALAR = 8, RECV_RU_WIDE = 2;
 // a2c_tune.h (lattice size class and precision), so it depends only on the
 // lattice: its results are the same in any batch.
 int lanes_per_site(int n_sites, bool fp64);
-constexpr int MAX_LANES_PER_SITE = 8;
+constexpr int MAX_LANES_PER_SITE = 16;
 
 // Fixed-order sum over the K lanes of a site (K a power of two <= 64, lanes
 // contiguous within a wave); the total lands in the group's first lane.

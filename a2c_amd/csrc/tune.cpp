@@ -23,15 +23,18 @@ namespace {
 
 a2c_tune_table default_table() {
   // a2c_autotune with the default options on one MI355X.  Up to 11^3 cells a
-  // lattice alone fills fewer SIMDs than the device has, and K = 8 makes it
-  // finish soonest (6^3 cells alone, K = 1/2/4/8: mixed 307/157/111/92 us,
-  // FP64 702/283/176/136 us per evaluation), at a cost of 4-11% per site in
-  // launches of 256 copies; larger lattices take the K of the best
-  // throughput (ns per site, 40^3 cells: mixed 1.75/1.58/1.60/1.80, FP64
-  // 4.50/4.46/4.48/4.90).
+  // lattice alone fills fewer SIMDs than the device has and takes the K that
+  // finishes it soonest within 25% of the best throughput: 6^3 cells, K =
+  // 1/2/4/8/16, alone mixed 294/143/98/80/67 us and FP64 687/270/165/124/96
+  // us per evaluation, in launches of 256 copies mixed 1.99/1.75/1.67/1.85/
+  // 2.30 and FP64 5.39/4.72/4.70/5.11/6.04 ns per site (K = 16 costs 37%
+  // there).  Larger lattices take the fastest K alone within 5% of the best
+  // throughput (24^3 cells FP64: 4.13/4.18 ns per site at K = 2/4 in a full
+  // launch, 602/528 us alone; 40^3 cells mixed 1.74/1.58/1.60/1.79/2.33 ns
+  // per site, 871/807/818/915/1193 us alone).
   static const int32_t max_sites[] = {2660, 6604, 15288, 32003, 71831, 170267, 366357, INT_MAX};
   static const int32_t lanes_mixed[] = {8, 8, 8, 4, 4, 4, 2, 2};
-  static const int32_t lanes_fp64[] = {8, 8, 8, 4, 4, 4, 2, 2};
+  static const int32_t lanes_fp64[] = {8, 8, 8, 4, 4, 4, 4, 2};
   a2c_tune_table t;
   std::memset(&t, 0, sizeof t);
   t.n_classes = 8;
@@ -45,8 +48,16 @@ a2c_tune_table default_table() {
 
 a2c_tune_table g_table = default_table();
 
-bool valid_lanes(int k) { return k == 1 || k == 2 || k == 4 || k == 8; }
-static_assert(MAX_LANES_PER_SITE == 8, "the table allows up to 8 lanes per site");
+constexpr int NLANES = 5;
+// Largest per-site cost in a full launch that a choice for the lone-lattice
+// time may add: for a lattice that alone fills fewer SIMDs than the device
+// has, and for one that fills them (where near-equal throughputs differ
+// alone: at 24^3 cells FP64, K = 2 beat K = 4 by 1% per site in a full
+// launch and lost by 14% alone, in the tail of a lockstep run).
+constexpr double LATENCY_COST = 1.25, THROUGHPUT_SLACK = 1.05;
+const int LANES[NLANES] = {1, 2, 4, 8, 16};
+bool valid_lanes(int k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16; }
+static_assert(MAX_LANES_PER_SITE == 16, "the table allows up to 16 lanes per site");
 
 bool valid(const a2c_tune_table& t) {
   if (t.n_classes < 1 || t.n_classes > A2C_TUNE_MAX_CLASSES) return false;
@@ -115,7 +126,6 @@ int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_r
   std::memset(&res, 0, sizeof res);
   res.n_classes = o->n_sizes;
   const a2c_tune_table saved = a2c::g_table;
-  const int lanes[4] = {1, 2, 4, 8};
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, o->device) != hipSuccess || ncu < 1)
     return A2C_EHIP;
@@ -160,10 +170,10 @@ int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_r
     }
     for (int p = 0; p < 2 && rc == A2C_OK; ++p) {
       const int prec = p == 0 ? A2C_PREC_MIXED : A2C_PREC_FP64;
-      double best = HUGE_VAL, best1 = HUGE_VAL;
-      int bestk = 1, bestk1 = 1;
-      for (int li = 0; li < 4 && rc == A2C_OK; ++li) {
-        a2c::g_table = a2c::uniform(lanes[li]);
+      double best = HUGE_VAL;
+      int bestk = 1;
+      for (int li = 0; li < a2c::NLANES && rc == A2C_OK; ++li) {
+        a2c::g_table = a2c::uniform(a2c::LANES[li]);
         for (int alone = 0; alone < 2 && rc == A2C_OK; ++alone) {
           const int n = alone ? 1 : copies, reps = alone ? 4 * o->reps : o->reps;
           for (int w = 0; w < 2 && rc == A2C_OK; ++w) rc = a2c_batch_eval(b, n, slots.data(), prec, f.data());
@@ -172,24 +182,30 @@ int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_r
           const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
           if (alone) {
             rep.us_alone[i][p][li] = s * 1e6;
-            if (s < best1) {
-              best1 = s;
-              bestk1 = lanes[li];
-            }
           } else {
             const double ns = s * 1e9 / ((double)copies * N);
             rep.ns_per_site[i][p][li] = ns;
             if (ns < best) {
               best = ns;
-              bestk = lanes[li];
+              bestk = a2c::LANES[li];
             }
           }
         }
       }
-      // one wave = 64 lanes: a lattice alone under bestk fills fewer SIMDs than the device has
-      const bool lat = (int64_t)N * bestk < 64 * simds;
-      rep.latency[i][p] = lat;
-      (p == 0 ? res.lanes_mixed : res.lanes_fp64)[i] = lat ? bestk1 : bestk;
+      // The K that finishes the lattice soonest alone, among those within a
+      // slack of the best throughput: LATENCY_COST for a lattice that alone
+      // under the best-throughput K gives fewer waves (64 lanes) than the
+      // device has SIMDs, THROUGHPUT_SLACK otherwise.
+      const double slack = (int64_t)N * bestk < 64 * simds ? a2c::LATENCY_COST : a2c::THROUGHPUT_SLACK;
+      int k = bestk;
+      double t = HUGE_VAL;
+      for (int li = 0; li < a2c::NLANES; ++li)
+        if (rep.ns_per_site[i][p][li] <= slack * best && rep.us_alone[i][p][li] < t) {
+          t = rep.us_alone[i][p][li];
+          k = a2c::LANES[li];
+        }
+      rep.latency[i][p] = k != bestk;
+      (p == 0 ? res.lanes_mixed : res.lanes_fp64)[i] = k;
     }
     a2c_batch_destroy(b);
   }

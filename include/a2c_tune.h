@@ -3,7 +3,7 @@
  * autotuner that fills it.  No counterpart in the reference (its CPU loops
  * have no launch shapes).
  *
- * The pair-owner kernels give each site K lanes (K in {1, 2, 4, 8}); lane p
+ * The pair-owner kernels give each site K lanes (K in {1, 2, 4, 8, 16}); lane p
  * of a site takes its owned neighbor slots p, p + K, p + 2K, ... and the K
  * partial sums are combined by a fixed shuffle tree.  K is chosen per
  * lattice from its site count and the evaluation precision through a
@@ -28,7 +28,7 @@ typedef struct a2c_tune_table {
   int32_t n_classes;                         /* 1..A2C_TUNE_MAX_CLASSES */
   int32_t max_sites[A2C_TUNE_MAX_CLASSES];   /* class c: sites <= max_sites[c] (increasing; the last
                                                 class also takes every larger lattice) */
-  int32_t lanes_mixed[A2C_TUNE_MAX_CLASSES]; /* K of the mixed-precision passes, 1, 2, 4 or 8 */
+  int32_t lanes_mixed[A2C_TUNE_MAX_CLASSES]; /* K of the mixed-precision passes, 1, 2, 4, 8 or 16 */
   int32_t lanes_fp64[A2C_TUNE_MAX_CLASSES];  /* K of the FP64 passes */
 } a2c_tune_table;
 
@@ -51,23 +51,24 @@ typedef struct a2c_autotune_opts {
 typedef struct a2c_autotune_report {
   int32_t sites[A2C_TUNE_MAX_CLASSES];          /* sites of each representative lattice */
   int32_t copies[A2C_TUNE_MAX_CLASSES];         /* lattices per fused launch */
-  double ns_per_site[A2C_TUNE_MAX_CLASSES][2][4]; /* [size][mixed, fp64][K = 1, 2, 4, 8]: wall time of
+  double ns_per_site[A2C_TUNE_MAX_CLASSES][2][5]; /* [size][mixed, fp64][K = 1, 2, 4, 8, 16]: wall time of
                                                      one fused evaluation per site */
-  double us_alone[A2C_TUNE_MAX_CLASSES][2][4];    /* the same, one lattice per launch: microseconds
+  double us_alone[A2C_TUNE_MAX_CLASSES][2][5];    /* the same, one lattice per launch: microseconds
                                                      per evaluation */
-  int32_t latency[A2C_TUNE_MAX_CLASSES][2];       /* 1: the class took the lone-lattice choice */
+  int32_t latency[A2C_TUNE_MAX_CLASSES][2];       /* 1: the lone-lattice time chose K */
 } a2c_autotune_report;
 
 void a2c_autotune_opts_default(a2c_autotune_opts* o);
 /* Time evaluations (objective and gradient, synthetic Pd-H lattices of the
  * batched workload's recipe) of every representative size under every K and
  * both precisions, in fused launches of many copies (throughput) and of one
- * lattice alone (latency).  Per size and precision the K of the best
- * throughput wins, unless one lattice under that K would give the device
- * fewer waves than it has SIMDs: such a lattice alone leaves SIMDs idle, and
- * the K that makes it finish soonest wins.  Class boundaries lie at the
- * geometric means of neighboring sizes.  out receives the table; report (may
- * be null) the measurements. */
+ * lattice alone (latency).  Per size and precision the K that makes one
+ * lattice finish soonest alone wins among those whose throughput is within
+ * 5% of the best -- within 25% if the lattice alone, under the best-
+ * throughput K, gives the device fewer waves than it has SIMDs (it then
+ * leaves SIMDs idle; more lanes cost per site when many lattices share a
+ * launch).  Class boundaries lie at the geometric means of neighboring
+ * sizes.  out receives the table; report (may be null) the measurements. */
 int a2c_autotune(const a2c_autotune_opts* o, a2c_tune_table* out, a2c_autotune_report* report);
 
 #ifdef __cplusplus

@@ -163,12 +163,12 @@ def test_tune_table_get_set_and_validation():
     """The lanes-per-site table is host state: no GPU needed to read, set or
     reject it."""
     t0 = A.tune_get()
-    assert t0["max_sites"][-1] == 2**31 - 1 and all(k in (1, 2, 4, 8) for k in t0["lanes_mixed"] + t0["lanes_fp64"])
+    assert t0["max_sites"][-1] == 2**31 - 1 and all(k in (1, 2, 4, 8, 16) for k in t0["lanes_mixed"] + t0["lanes_fp64"])
     try:
         A.tune_set({"max_sites": [1000, 2**31 - 1], "lanes_mixed": [4, 2], "lanes_fp64": [8, 1]})
         assert A.tune_get() == {"max_sites": [1000, 2**31 - 1], "lanes_mixed": [4, 2], "lanes_fp64": [8, 1]}
         for bad in ({"max_sites": [10], "lanes_mixed": [3], "lanes_fp64": [1]},             # K not a power of 2
-                    {"max_sites": [10], "lanes_mixed": [16], "lanes_fp64": [1]},            # K above 8
+                    {"max_sites": [10], "lanes_mixed": [32], "lanes_fp64": [1]},            # K above 16
                     {"max_sites": [100, 50], "lanes_mixed": [1, 1], "lanes_fp64": [1, 1]},  # not increasing
                     {"max_sites": [], "lanes_mixed": [], "lanes_fp64": []}):
             with pytest.raises(A.A2CError):
