@@ -122,8 +122,14 @@ class DmdReport(C.Structure):
     _fields_ = [
         ("f", C.c_double), ("lmvm_reason", C.c_int32), ("lmvm_iterations", C.c_int32), ("lmvm_evals", C.c_int32),
         ("total_h_before", C.c_double), ("total_h_after", C.c_double), ("xH", C.c_double),
-        ("xH_interior", C.c_double),
+        ("xH_interior", C.c_double), ("flux_sum", C.c_double), ("flux_gross", C.c_double),
+        ("clamp_lift", C.c_double), ("reservoir_supply", C.c_double),
     ]
+
+
+class DiffusionBalance(C.Structure):
+    _fields_ = [("flux_sum", C.c_double), ("flux_gross", C.c_double), ("clamp_lift", C.c_double),
+                ("reservoir_supply", C.c_double)]
 
 
 class LatticeDesc(C.Structure):
@@ -160,6 +166,7 @@ def _load():
         "a2c_tao_objgrad": (I, [VP, dp, dp, dp, VP]),
         "a2c_fe_chem_potential": (I, [VP, dp, dp, dp, D, dp, ip]),
         "a2c_fe_diffuse": (I, [VP, D, D, D, D, D, dp, ip, dp]),
+        "a2c_fe_diffusion_balance": (I, [VP, P(DiffusionBalance)]),
         "a2c_lmvm_opts_default": (None, [P(LmvmOpts)]),
         "a2c_fe_minimize": (I, [VP, dp, P(LmvmOpts), dp, P(LmvmReport)]),
         "a2c_dev_alloc": (I, [VP, C.c_size_t, P(VP)]),
@@ -396,6 +403,14 @@ class FreeEntropy:
         self._check(lib.a2c_fe_diffuse(self._ctx, dt, nu, Qm, xLb, xUb, _dptr(g), _iptr(fh), _dptr(x)))
         self._x = x
         return x
+
+    def diffusion_balance(self):
+        """Balance of the last integrateDiffusion: applied increments (equal
+        to the reservoirs' supply up to rounding), their scale (sum of |edge
+        terms|), the clamp lift and the reservoirs' supply."""
+        b = DiffusionBalance()
+        self._check(lib.a2c_fe_diffusion_balance(self._ctx, C.byref(b)))
+        return {k: getattr(b, k) for k, _ in DiffusionBalance._fields_}
 
     def minimizeFreeEntropy(self, X0, **opts):
         """Bounded LMVM relaxation; returns (X, report dict)."""

@@ -950,6 +950,7 @@ struct Slot {
   a2c_fe_ctx* fe = nullptr;
   bool owned = false;
   bool used = false;
+  size_t charged = 0;  // bytes of the HBM budget this slot holds (its footprint)
   // optimizer
   bool running = false, started = false, mixed = false;
   a2c_lmvm_opts opts{};
@@ -1786,6 +1787,12 @@ int lmvm_iteration(a2c_batch* b, std::vector<int>& act) {
         s.rho[s.head] = 1.0 / v[2 * k];
         s.head = (s.head + 1) % s.m;
         s.npairs = std::min(s.npairs + 1, s.m);
+      } else if (s.npairs == s.m) {
+        // k_seg_pair wrote the rejected s, y over S[head], Y[head], which with
+        // a full history held the oldest pair still in use: that pair is gone,
+        // so the recursion stops one short of it (the next accepted pair
+        // fills the slot again)
+        s.npairs = s.m - 1;
       }
     }
     std::vector<int> sw2;
@@ -1935,9 +1942,12 @@ size_t a2c_batch_footprint(int32_t n_pd, int32_t n_h, int32_t lmvm_m) {
   // state/gradient/staging 4N doubles x 4, per-site arrays ~5 doubles, ELL
   // lists ~100 int32 per site, the owned/received halves of two pair lists
   // per site (~28 wide: 2 int32 per owned slot, a 32-byte share per received
-  // slot), optimizer (7 + 2m) x 4N doubles
-  return N * (4 * 32 + 40 + 100 * 4 + 2 * 28 * (2 * 4 + 32) + (7 + 2 * (size_t)std::max(lmvm_m, 1)) * 32) +
-         ((size_t)1 << 20);
+  // slot), optimizer (7 + 2m) x 4N doubles; plus a quarter for the device
+  // cache's size classes (a block is rounded up by at most a quarter)
+  const size_t raw =
+      N * (4 * 32 + 40 + 100 * 4 + 2 * 28 * (2 * 4 + 32) + (7 + 2 * (size_t)std::max(lmvm_m, 1)) * 32) +
+      ((size_t)1 << 20);
+  return raw + raw / 4;
 }
 
 size_t a2c_batch_mem_used(const a2c_batch* b) { return b ? b->mem_used : 0; }
@@ -1979,6 +1989,7 @@ int a2c_batch_add(a2c_batch* b, const a2c_fe_desc* lat, const double* X, const d
   s.fe = fe;
   s.owned = true;
   s.used = true;
+  s.charged = need;
   b->mem_used += need;
   *slot = free_slot;
   return A2C_OK;
@@ -1989,7 +2000,7 @@ int a2c_batch_remove(a2c_batch* b, int32_t slot) {
   if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return berr(b, A2C_EINVAL, "bad slot");
   if (int rc = bsync(b)) return rc;
   Slot& s = b->slots[slot];
-  b->mem_used -= std::min(b->mem_used, a2c_batch_footprint(s.fe->desc.n_pd, s.fe->desc.n_h, 5));
+  b->mem_used -= std::min(b->mem_used, s.charged);
   lmvm_free(s);
   if (s.owned) a2c_fe_destroy(s.fe);
   s = Slot();
@@ -2035,6 +2046,14 @@ int a2c_batch_lmvm_start(a2c_batch* b, int32_t slot, const a2c_lmvm_opts* o) {
     return berr(b, A2C_EINVAL, "bad options");
   Slot& s = b->slots[slot];
   if (int rc = bsync(b)) return rc;  // queued launches may still use the buffers lmvm_alloc replaces
+  // admission charged an optimizer of 5 pairs; charge the one asked for
+  if (s.owned) {
+    const size_t want = a2c_batch_footprint(s.fe->desc.n_pd, s.fe->desc.n_h, o->m);
+    if (want > s.charged && b->mem_used - s.charged + want > b->desc.hbm_budget_bytes)
+      return berr(b, A2C_ENOMEM, "HBM budget exceeded by the optimizer's memory");
+    b->mem_used = b->mem_used - s.charged + want;
+    s.charged = want;
+  }
   s.opts = *o;
   s.m = o->m;
   if (int rc = lmvm_alloc(b, s)) return rc;
@@ -2084,6 +2103,9 @@ int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, doubl
   bd.deriv_mode = c->desc.deriv_mode;
   bd.fd_eps = c->desc.fd_eps;
   bd.max_lattices = 1;
+  // the context's stream may still run work that uses the buffers the
+  // optimizer shares with it (state, scratch of the evaluation)
+  A2C_HIP(c, hipStreamSynchronize(c->stream));
   a2c_batch* b = nullptr;
   if (int rc = a2c_batch_create(&bd, &b)) return set_err(c, rc, "batch creation failed");
   struct Guard {

@@ -218,6 +218,17 @@ struct a2c_fe_ctx {
   double* hX = nullptr;
   double* hG = nullptr;
 
+  // occupancy scratch of the chemical potential and the diffusion step
+  // (allocated at their first call, kept for the context's lifetime)
+  double* occ_g = nullptr;     // nH gamma
+  int32_t* occ_full = nullptr; // nH full_H
+  double* occ_corr = nullptr;  // N correction sums
+  double* occ_f = nullptr;     // nH f = gamma - ln(x/(1-x))
+  double* occ_xn = nullptr;    // nH new occupancy
+  double* occ_part = nullptr;  // 4 per block of the balance
+  double* occ_bal = nullptr;   // 4: the balance of the last step (a2c_diffusion_balance)
+  a2c_diffusion_balance last_balance{};
+
   a2c::DevSys sys() const {
     a2c::DevSys s;
     s.npd = desc.n_pd;
@@ -296,8 +307,16 @@ void host_tables64(std::vector<double>& t);
 int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, double* Xout, a2c_lmvm_report* rep);
 int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h,
                 double gamma_bd, double* dgam, int32_t* dfull, double* dcorr);  // occupancy.hip
+// Scratch of one diffusion step: f (n_h doubles), block partials of the
+// balance (4 per block of 256 H sites) and the balance itself (4 doubles).
+struct DiffuseScratch {
+  double* f;
+  double* part;
+  double* bal;
+};
+int diffuse_blocks(int nh);
 int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb,
-                   const double* dgam, const int32_t* dfull, double* df, double* dxnew);
+                   const double* dgam, const int32_t* dfull, const DiffuseScratch& w, double* dxnew);
 
 // Event timing of individual kernels on the context stream: mark() records
 // the next preallocated event; durations are differences of consecutive marks.

@@ -1,6 +1,7 @@
 // C-ABI of the occupancy kernels: chemical potential and diffusion step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -10,23 +11,21 @@ using namespace a2c;
 
 namespace {
 
-// Device scratch owned by one call; freed on every return path.
-struct Scratch {
-  std::vector<void*> p;
-  ~Scratch() {
-    for (void* q : p) (void)hipFree(q);
-  }
-  template <class T>
-  hipError_t alloc(T** out, size_t n) {
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, sizeof(T) * (n ? n : 1));
-    if (e == hipSuccess) {
-      p.push_back(q);
-      *out = static_cast<T*>(q);
-    }
-    return e;
-  }
-};
+// Context-owned scratch of the occupancy calls, from the library's device
+// cache at the first call and kept until the context is destroyed.
+int occ_scratch(a2c_fe_ctx* c) {
+  const size_t nh = std::max(c->desc.n_h, 1), N = std::max(c->N, 1);
+  if (c->occ_g) return A2C_OK;
+  const size_t nblk = (size_t)diffuse_blocks((int)nh);
+  A2C_HIP(c, dmalloc(&c->occ_g, 8 * nh));
+  A2C_HIP(c, dmalloc(&c->occ_full, 4 * nh));
+  A2C_HIP(c, dmalloc(&c->occ_corr, 8 * N));
+  A2C_HIP(c, dmalloc(&c->occ_f, 8 * nh));
+  A2C_HIP(c, dmalloc(&c->occ_xn, 8 * nh));
+  A2C_HIP(c, dmalloc(&c->occ_part, 8 * 4 * nblk));
+  A2C_HIP(c, dmalloc(&c->occ_bal, 8 * 4));
+  return A2C_OK;
+}
 
 }  // namespace
 
@@ -38,12 +37,9 @@ extern "C" int a2c_fe_chem_potential(a2c_fe_ctx* c, const double* X, const doubl
   if ((!X && c->N) || !coe_pd || !coe_h || (c->desc.n_h && (!gamma || !full_h)))
     return set_err(c, A2C_EINVAL, "null argument");
   const size_t nh = c->desc.n_h, N = c->N;
-  Scratch sc;
-  double *dg = nullptr, *dcorr = nullptr;
-  int32_t* dfull = nullptr;
-  A2C_HIP(c, sc.alloc(&dg, nh));
-  A2C_HIP(c, sc.alloc(&dfull, nh));
-  A2C_HIP(c, sc.alloc(&dcorr, N));
+  if (int rc = occ_scratch(c)) return rc;
+  double *dg = c->occ_g, *dcorr = c->occ_corr;
+  int32_t* dfull = c->occ_full;
   if (N) {
     std::copy(X, X + 4 * N, c->hX);
     A2C_HIP(c, hipMemcpyAsync(c->dX, c->hX, 32 * N, hipMemcpyHostToDevice, c->stream));
@@ -71,18 +67,24 @@ extern "C" int a2c_fe_diffuse(a2c_fe_ctx* c, double dt, double nu, double Qm, do
   if (!(xLb <= xUb)) return set_err(c, A2C_EINVAL, "xLb > xUb");
   if (nh == 0) return A2C_OK;
   const double pref = nu * std::exp(-Qm / (KB * c->desc.temperature));  // integrate.cpp:26
-  Scratch sc;
-  double *dg = nullptr, *df = nullptr, *dxn = nullptr;
-  int32_t* dfull = nullptr;
-  A2C_HIP(c, sc.alloc(&dg, nh));
-  A2C_HIP(c, sc.alloc(&df, nh));
-  A2C_HIP(c, sc.alloc(&dxn, nh));
-  A2C_HIP(c, sc.alloc(&dfull, nh));
+  if (int rc = occ_scratch(c)) return rc;
+  double *dg = c->occ_g, *dxn = c->occ_xn;
+  int32_t* dfull = c->occ_full;
   A2C_HIP(c, hipMemcpyAsync(dg, gamma, 8 * nh, hipMemcpyHostToDevice, c->stream));
   A2C_HIP(c, hipMemcpyAsync(dfull, full_h, 4 * nh, hipMemcpyHostToDevice, c->stream));
-  if (int rc = launch_diffuse(c, dt, pref, xLb, xUb, dg, dfull, df, dxn)) return rc;
+  const DiffuseScratch w{c->occ_f, c->occ_part, c->occ_bal};
+  if (int rc = launch_diffuse(c, dt, pref, xLb, xUb, dg, dfull, w, dxn)) return rc;
   A2C_HIP(c, hipMemcpyAsync(c->dx, dxn, 8 * nh, hipMemcpyDeviceToDevice, c->stream));
   A2C_HIP(c, hipMemcpyAsync(x, dxn, 8 * nh, hipMemcpyDeviceToHost, c->stream));
+  double bal[4];
+  A2C_HIP(c, hipMemcpyAsync(bal, c->occ_bal, sizeof(bal), hipMemcpyDeviceToHost, c->stream));
   A2C_HIP(c, hipStreamSynchronize(c->stream));
+  c->last_balance = a2c_diffusion_balance{bal[0], bal[1], bal[2], bal[3]};
+  return A2C_OK;
+}
+
+extern "C" int a2c_fe_diffusion_balance(const a2c_fe_ctx* c, a2c_diffusion_balance* out) {
+  if (!c || !out) return A2C_EINVAL;
+  *out = c->last_balance;
   return A2C_OK;
 }

@@ -158,28 +158,79 @@ __global__ void k_diff_f(int nh, const double* __restrict__ gamma, const double*
   f[a] = gamma[a] - log(xo[a] / (1.0 - xo[a]));
 }
 
-__global__ void k_diff_step(Ell hh1, const double* __restrict__ xo, const double* __restrict__ f,
-                            const int32_t* __restrict__ full, double pref, double dt, double xLb, double xUb,
-                            double* __restrict__ xn) {
+// One explicit step per H site, and per block the four balance sums of the
+// step (fixed-order block trees -> part[4 * block]): the increments applied
+// (non-reservoir sites), the absolute edge terms of every site (the scale of
+// the rounding), the clamp lifts (clamped minus unclamped occupancy) and the
+// H the reservoirs supplied (minus the increments a full_H site would have
+// received; it keeps x).  The edge terms are exactly antisymmetric, so the
+// applied increments equal the reservoirs' supply up to the rounding of the
+// site sums.
+__global__ __launch_bounds__(TB) void k_diff_step(Ell hh1, const double* __restrict__ xo,
+                                                  const double* __restrict__ f, const int32_t* __restrict__ full,
+                                                  double pref, double dt, double xLb, double xUb,
+                                                  double* __restrict__ xn, double* __restrict__ part) {
+  __shared__ double red[4][TB];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= hh1.rows) return;
-  const double xa = xo[a];
-  if (full[a] == 1) {
-    xn[a] = xa;
-    return;
+  double inc = 0.0, gross = 0.0, lift = 0.0, supply = 0.0;
+  if (a < hh1.rows) {
+    const double xa = xo[a];
+    const double fa = f[a];
+    double xd = 0.0, xg = 0.0;
+    const int n = hh1.cnt[a];
+    for (int t = 0; t < n; ++t) {
+      const int b = hh1.idx[(size_t)t * hh1.rows + a];
+      const double xb = xo[b], df = fa - f[b];
+      const double e = xb * (1.0 - xa) * exp(-df / 2.0) - xa * (1.0 - xb) * exp(df / 2.0);
+      xd += e;
+      xg += fabs(e);
+    }
+    gross = pref * xg * dt;
+    if (full[a] == 1) {  // reservoir: keeps x (integrate.cpp:50, :62, :77)
+      supply = -(pref * xd * dt);
+      xn[a] = xa;
+    } else {
+      inc = pref * xd * dt;
+      const double v0 = xa + inc;
+      double v = v0;
+      if (v < xLb) v = xLb;
+      else if (v > xUb) v = xUb;
+      lift = v - v0;
+      xn[a] = v;
+    }
   }
-  const double fa = f[a];
-  double xd = 0.0;
-  const int n = hh1.cnt[a];
-  for (int t = 0; t < n; ++t) {
-    const int b = hh1.idx[(size_t)t * hh1.rows + a];
-    const double xb = xo[b], df = fa - f[b];
-    xd += xb * (1.0 - xa) * exp(-df / 2.0) - xa * (1.0 - xb) * exp(df / 2.0);
+  red[0][threadIdx.x] = inc;
+  red[1][threadIdx.x] = gross;
+  red[2][threadIdx.x] = lift;
+  red[3][threadIdx.x] = supply;
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
   }
-  double v = xa + pref * xd * dt;
-  if (v < xLb) v = xLb;
-  else if (v > xUb) v = xUb;
-  xn[a] = v;
+  if (threadIdx.x < 4) part[4 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The block sums of k_diff_step in block order (one workgroup): bal[0..3].
+__global__ __launch_bounds__(TB) void k_diff_balance(const double* __restrict__ part, int nblk,
+                                                     double* __restrict__ bal) {
+  __shared__ double red[4][TB];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += TB)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] += part[4 * (size_t)b + k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) bal[threadIdx.x] = red[threadIdx.x][0];
 }
 
 }  // namespace
@@ -203,14 +254,19 @@ int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const dou
   return A2C_OK;
 }
 
+int diffuse_blocks(int nh) { return (nh + TB - 1) / TB; }
+
 int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb, const double* dgam,
-                   const int32_t* dfull, double* df, double* dxnew) {
+                   const int32_t* dfull, const DiffuseScratch& w, double* dxnew) {
   const int nh = c->desc.n_h;
   if (nh == 0) return A2C_OK;
-  hipLaunchKernelGGL(k_diff_f, dim3((nh + TB - 1) / TB), dim3(TB), 0, c->stream, nh, dgam, c->dx, df);
+  const int nblk = diffuse_blocks(nh);
+  hipLaunchKernelGGL(k_diff_f, dim3(nblk), dim3(TB), 0, c->stream, nh, dgam, c->dx, w.f);
   A2C_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_diff_step, dim3((nh + TB - 1) / TB), dim3(TB), 0, c->stream, c->ell[4], c->dx, df, dfull,
-                     prefactor, dt, xLb, xUb, dxnew);
+  hipLaunchKernelGGL(k_diff_step, dim3(nblk), dim3(TB), 0, c->stream, c->ell[4], c->dx, w.f, dfull, prefactor, dt,
+                     xLb, xUb, dxnew, w.part);
+  A2C_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_diff_balance, dim3(1), dim3(TB), 0, c->stream, w.part, nblk, w.bal);
   A2C_HIP(c, hipGetLastError());
   return A2C_OK;
 }

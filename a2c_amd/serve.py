@@ -20,7 +20,7 @@ per cell), T ~ U(300, 700) K, mean H fraction ~ U(0, 0.6) with iid noise
 +-0.05, displacements N(0, 0.02 A), sigma 0.10 A, periodic, seed 4 + request
 id.  The comparator relaxes the same requests one at a time in FP64.
 
-    python -m a2c_amd.serve --rates 1,5,20 --requests 24
+    python -m a2c_amd.serve --rates 5,10,20,40 --requests 200
 """
 import argparse
 import json
@@ -69,9 +69,11 @@ def iter_cost(sites):
     return ITER_FIXED_S + ITER_SITE_S * sites
 
 
-def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None, fair=True):
+def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None, fair=True,
+          keep_state=False):
     """fair: stride scheduling by iteration cost; else every running lattice
-    is stepped on every tick (lockstep)."""
+    is stepped on every tick (lockstep).  keep_state: each finished request
+    also carries its relaxed state ("X")."""
     opts = opts or {"gatol": 1e-8, "grtol": 0.0}
     vt = {}  # slot -> virtual time (seconds of GPU time received)
     pending = list(reqs)
@@ -87,7 +89,7 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
                 n_sites = 8 * req["n"] ** 3
                 if B.mem_used() + Batch.footprint(n_sites // 2, n_sites // 2) > budget and running:
                     break
-                X, x, box, npd, nh = materialize(req)
+                X, x, box, npd, nh = req["state"] if "state" in req else materialize(req)
                 slot = B.add(X, x, npd, nh, box, (1, 1, 1), T=req["T"])
                 B.start(slot, **opts)
                 vt[slot] = min(vt.values()) if vt else 0.0
@@ -113,6 +115,8 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
                     req["finish"] = time.perf_counter() - t0
                     req["latency"] = req["finish"] - req["arrival"]
                     req["report"] = rep
+                    if keep_state:
+                        req["X"] = B.get_state(slot)[0]
                     done.append(req)
                     B.remove(slot)
                     vt.pop(slot, None)
@@ -131,7 +135,7 @@ def serve_one_at_a_time(reqs, precision="fp64", opts=None):
         now = time.perf_counter() - t0
         if req["arrival"] > now:
             time.sleep(req["arrival"] - now)
-        X, x, box, npd, nh = materialize(req)
+        X, x, box, npd, nh = req["state"] if "state" in req else materialize(req)
         with FreeEntropy(npd, nh, box, (1, 1, 1), T=req["T"], precision=precision) as fe:
             fe.set_occupancy(x)
             fe.findNeighbors(X)
@@ -144,6 +148,74 @@ def serve_one_at_a_time(reqs, precision="fp64", opts=None):
     return done, time.perf_counter() - t0
 
 
+def service_times(reqs, precision="fp64", opts=None):
+    """Relax the requests one at a time, back to back (no arrivals): the
+    service time and report of each, in request order."""
+    opts = opts or {"gatol": 1e-8, "grtol": 0.0}
+    out = []
+    for req in reqs:
+        X, x, box, npd, nh = req["state"] if "state" in req else materialize(req)
+        t0 = time.perf_counter()
+        with FreeEntropy(npd, nh, box, (1, 1, 1), T=req["T"], precision=precision) as fe:
+            fe.set_occupancy(x)
+            fe.findNeighbors(X)
+            _, rep = fe.minimizeFreeEntropy(X, **opts)
+        out.append((time.perf_counter() - t0, rep))
+    return out
+
+
+def fifo_latencies(arrivals, service):
+    """Latencies of a single first-come-first-served server with the given
+    arrival times and service times (one at a time, no overlap)."""
+    t, lat = 0.0, []
+    for a, sv in zip(arrivals, service):
+        t = max(t, a) + sv
+        lat.append(t - a)
+    return np.array(lat)
+
+
+def sweep(rates, n_requests, fp64_sample=40, log=None, opts=None):
+    """Serve the same n_requests (sizes, temperatures, occupancies) at each
+    Poisson rate with continuous batching (mixed precision, FP64 endgame).
+    The comparator relaxes the first fp64_sample requests one at a time in
+    FP64, back to back: their mean service time gives its capacity, and a
+    first-come-first-served replay of those measured service times (request
+    i takes the time of sample request i mod fp64_sample) on each rate's
+    arrivals gives its p50/p99.  relaxed_f_rel_vs_fp64: the largest relative
+    difference of the served objectives from the FP64 ones over the sample."""
+    opts = opts or {"gatol": 1e-8, "grtol": 0.0}
+    base = make_requests(n_requests, 1.0)
+    states = [materialize(r) for r in base]  # the clients' payloads, built before the clock starts
+    k = min(fp64_sample, n_requests)
+    sv = service_times([dict(r, state=st) for r, st in zip(base[:k], states[:k])], opts=opts)
+    f64 = [rep["f"] for _, rep in sv]
+    svt = np.array([t for t, _ in sv])
+    rows = []
+    for rate in rates:
+        reqs = make_requests(n_requests, rate)
+        for r, st in zip(reqs, states):
+            r["state"] = st
+        done, wall = serve(reqs, opts=opts, log=log)
+        row = {"rate": rate, "batched_mixed": summarize(done, wall)}
+        byid = {d["id"]: d for d in done}
+        row["relaxed_f_rel_vs_fp64"] = max(abs(byid[i]["report"]["f"] - f64[i]) / abs(f64[i]) for i in range(k))
+        lat = fifo_latencies([r["arrival"] for r in reqs], [svt[i % k] for i in range(n_requests)])
+        row["one_at_a_time_fp64_replay"] = {"p50_s": float(np.percentile(lat, 50)),
+                                            "p99_s": float(np.percentile(lat, 99))}
+        rows.append(row)
+        if log:
+            log({"rate": rate, "summary": row["batched_mixed"]})
+    sites = [8 * r["n"] ** 3 for r in base]
+    return {
+        "requests_per_rate": n_requests, "sites_mean": float(np.mean(sites)), "sites_max": int(max(sites)),
+        "rates": rows,
+        "saturation_relaxations_per_s": max(r["batched_mixed"]["relaxations_per_s"] for r in rows),
+        "one_at_a_time_fp64": {"sample": k, "service_s_mean": float(svt.mean()),
+                               "relaxations_per_s": float(1.0 / svt.mean()),
+                               "converged": sum(1 for _, rep in sv if rep["reason"] > 0)},
+    }
+
+
 def summarize(done, wall):
     lat = np.array([d["latency"] for d in done])
     conv = sum(1 for d in done if d["report"]["reason"] > 0)
@@ -154,32 +226,15 @@ def summarize(done, wall):
 
 def main():
     p = argparse.ArgumentParser()
-    p.add_argument("--rates", default="1,5,20")
-    p.add_argument("--requests", type=int, default=24)
-    p.add_argument("--nmax", type=int, default=48)
-    p.add_argument("--no-baseline", action="store_true")
-    p.add_argument("--lockstep", action="store_true", help="also serve with every lattice stepped every tick")
+    p.add_argument("--rates", default="5,10,20,40")
+    p.add_argument("--requests", type=int, default=200)
+    p.add_argument("--fp64-sample", type=int, default=40)
     a = p.parse_args()
-    out = []
-    for rate in [float(r) for r in a.rates.split(",")]:
-        reqs = make_requests(a.requests, rate, nmax=a.nmax)
-        note = lambda r: print("rate %g: request %d (n=%d) done, latency %.3f s, %d iterations"
-                               % (rate, r["id"], r["n"], r["latency"], r["report"]["iterations"]),
-                               file=sys.stderr, flush=True)
-        done, wall = serve([dict(r) for r in reqs], log=note)
-        row = {"rate": rate, "batched_mixed": summarize(done, wall)}
-        if a.lockstep:
-            d2, w2 = serve([dict(r) for r in reqs], fair=False)
-            row["batched_mixed_lockstep"] = summarize(d2, w2)
-        if not a.no_baseline:
-            d1, w1 = serve_one_at_a_time([dict(r) for r in reqs])
-            row["one_at_a_time_fp64"] = summarize(d1, w1)
-            # relaxed objectives of the served (mixed) requests against the FP64 ones
-            f64 = {d["id"]: d["report"]["f"] for d in d1}
-            row["relaxed_f_rel_vs_fp64"] = max(abs(d["report"]["f"] - f64[d["id"]]) / abs(f64[d["id"]])
-                                               for d in done)
-        print(json.dumps(row), flush=True)
-        out.append(row)
+    note = lambda r: print(json.dumps(r) if "summary" in r else
+                           "request %d (n=%d) done, latency %.3f s, %d iterations"
+                           % (r["id"], r["n"], r["latency"], r["report"]["iterations"]), file=sys.stderr, flush=True)
+    out = sweep([float(r) for r in a.rates.split(",")], a.requests, a.fp64_sample, log=note)
+    print(json.dumps(out), flush=True)
     return out
 
 

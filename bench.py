@@ -10,8 +10,12 @@ N(0, 0.03 A), sigma U(0.08, 0.15) A, T = 500 K, rc = 5.35 A, 12-point rule
 Prints one JSON line:  value = sites * steps / seconds of the timed region
 (HIP events on the context stream, bracketed by device synchronisation),
 plus the roofline of the dominant kernel, the FP64 comparator, the error of
-the measured path against the FP64 kernel, and the CPU baseline (the FP64
-reference-faithful oracle in FD mode on the host cores, on a bounded sample).
+the measured path against the FP64 kernel, the relaxation half of the metric
+-- "batched" (64 uneven lattices relaxed in lockstep: relaxations/s, relaxed
+objectives against FP64-only relaxations) and "serving" (Poisson request
+streams over a sweep of rates: relaxations/s, p50/p99 latency, the FP64
+one-at-a-time comparator) -- and the CPU baseline (the FP64 reference-
+faithful oracle in FD mode on the host cores, on a bounded sample).
 """
 import argparse
 import json
@@ -54,6 +58,12 @@ def parse():
     p.add_argument("--dmd-steps", type=int, default=100)
     p.add_argument("--dmd-cells", type=int, default=64)
     p.add_argument("--fp64-sample", type=int, default=8, help="batched: lattices relaxed one at a time in FP64")
+    p.add_argument("--no-batched", action="store_true", help="single: skip the batched-relaxation block")
+    p.add_argument("--no-serving", action="store_true", help="single: skip the serving block")
+    p.add_argument("--serve-rates", default="5,10,20,40", help="serving: Poisson request rates (req/s)")
+    p.add_argument("--serve-requests", type=int, default=200, help="serving: requests per rate")
+    p.add_argument("--serve-fp64-sample", type=int, default=40,
+                   help="serving: requests relaxed one at a time in FP64 (the comparator)")
     p.add_argument("--lmvm-m", type=int, default=5, help="batched: stored pairs of the LMVM (TAO's default 5)")
     return p.parse_args()
 
@@ -81,26 +91,82 @@ def run_gpu(fe, X, steps, warmup):
     return tm, wall, f, G
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args, mk):
     """Reference-faithful FP64 oracle (FD derivatives recomputed as the
     reference does) on a smaller lattice of the same recipe: per-site work is
-    the same under periodic images, so sites/s carries over."""
+    the same under periodic images, so sites/s carries over.  Timed on every
+    core of this process's affinity mask up to 16 (the CPU share of one GPU
+    on the box), then on one core; plus the plain method's relaxation
+    (6^3 cells, GH3 rule) by scipy's L-BFGS-B (m = 5) on the oracle in FD
+    mode, to ||g|| <= 1e-8 or where its line search stalls."""
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, avail))
     X, x, box, npd, nh = mk(6)
-    orc = O.Oracle(npd, nh, box, (1, 1, 1), T=500.0, deriv="fd", threads=threads, quad=args.quad)
-    orc.build_neighbors(X)
-    orc.eval(X, x)  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        orc.eval(X, x)
-        n += 1
-        if time.perf_counter() - t0 >= args.cpu_seconds:
-            break
+
+    def rate(nthr, budget):
+        orc = O.Oracle(npd, nh, box, (1, 1, 1), T=500.0, deriv="fd", threads=nthr, quad=args.quad)
+        orc.build_neighbors(X)
+        orc.eval(X, x)  # warm-up
+        n, t0 = 0, time.perf_counter()
+        while True:
+            orc.eval(X, x)
+            n += 1
+            if time.perf_counter() - t0 >= budget:
+                break
+        dt = time.perf_counter() - t0
+        return (npd + nh) * n / dt, n, dt
+
+    v, n, dt = rate(threads, args.cpu_seconds)
+    v1, n1, dt1 = rate(1, max(2.0, args.cpu_seconds / 2))
+    out = {"value": v, "unit": "atom-evals/s", "cores": threads, "kind": "port",
+           "sample": "%d full FD-mode evaluations of the 6x6x6-cell lattice of the same recipe "
+                     "(1,728 sites), %.1f s, %d OpenMP threads (%d cores in the affinity mask, at most 16 used)"
+                     % (n, dt, threads, avail),
+           "cpu_model": cpu_model(), "cores_available": avail,
+           "one_core": {"value": v1, "evals": n1, "seconds": dt1}}
+    out["plain_relaxation"] = cpu_relaxation(threads)
+    return out
+
+
+def cpu_relaxation(threads):
+    """The plain method's case relaxed on the CPU: scipy L-BFGS-B (5 pairs)
+    on the oracle's FD-mode objective and gradient (the reference's
+    evaluation), from the same start as the GPU's plain workload."""
+    import oracle as O
+    from scipy.optimize import minimize
+    import a2c_amd as A
+    from a2c_amd import workloads
+    s = workloads.plain()
+    orc = O.Oracle(s["n_pd"], s["n_h"], s["box"], s["periodic"], T=s["T"], deriv="fd", threads=threads,
+                   quad=s["quad"])
+    orc.build_neighbors(s["X"])
+    nfev = [0]
+
+    def fg(v):
+        nfev[0] += 1
+        return orc.eval(v, s["x"])
+
+    bounds = [(None, None)] * 3 + [(1e-6, None)]
+    t0 = time.perf_counter()
+    r = minimize(fg, s["X"], jac=True, method="L-BFGS-B", bounds=bounds * (s["n_pd"] + s["n_h"]),
+                 options={"maxiter": 2000, "maxfun": 4000, "gtol": 1e-8 / np.sqrt(4 * (s["n_pd"] + s["n_h"])),
+                          "ftol": 0.0, "maxcor": 5})
     dt = time.perf_counter() - t0
-    return {"value": (npd + nh) * n / dt, "unit": "atom-evals/s", "cores": threads, "kind": "port",
-            "sample": "%d full FD-mode evaluations of the 6x6x6-cell lattice of the same recipe "
-                      "(1,728 sites), %.1f s, %d OpenMP threads" % (n, dt, threads)}
+    g = np.linalg.norm(fg(r.x)[1])
+    return {"seconds": dt, "evals": nfev[0] - 1, "iterations": int(r.nit), "f": float(r.fun), "gnorm": float(g),
+            "threads": threads, "solver": "scipy L-BFGS-B m=5 on the FD-mode oracle",
+            "stopped": str(r.message)}
 
 
 def batched(args):
@@ -110,15 +176,8 @@ def batched(args):
     evaluation round), mixed precision with the FP64 endgame, to ||Pg|| <= 1e-8.
     Comparator: the first --fp64-sample lattices relaxed one at a time in FP64."""
     import a2c_amd as A
-    rng = np.random.default_rng(3)
-    specs = []
-    for i in range(args.lattices):
-        n = int(rng.integers(8, 25))
-        T = float(rng.uniform(300, 700))
-        xb = float(rng.uniform(0.01, 0.6))
-        X, x, box, npd, nh = A.lattice(n, seed=3, lattice_id=i, disp_std=0.02, sigma=(0.10, 0.10),
-                                       x=(max(xb - 0.05, 1e-3), xb + 0.05))
-        specs.append(dict(X=X, x=x, box=box, periodic=(1, 1, 1), n_pd=npd, n_h=nh, T=T))
+    from a2c_amd import workloads
+    specs = workloads.batched(args.lattices)
     sites = sum(s["n_pd"] + s["n_h"] for s in specs)
     opts = dict(gatol=1e-8, grtol=0.0, m=args.lmvm_m)
     log("batched: %d lattices, %d sites" % (len(specs), sites))
@@ -256,6 +315,38 @@ def dmd(args):
     }
 
 
+def batched_block(args):
+    """The batched workload's relaxations/s, reported in the default line."""
+    r = batched(args)
+    return {"metric": "relaxations/s", "value": r["value"], "unit": "relaxations/s",
+            "lattices": r["config"]["lattices"], "sites": r["config"]["sites"], "converged": r["converged"],
+            "admission_s": r["admission_s"], "relax_s": r["relax_s"], "evals": r["evals"],
+            "evals_fp64": r["evals_fp64"], "iterations_max": max(r["iterations"]),
+            "relaxed_f_rel_vs_fp64": r["relaxed_f_rel_vs_fp64"],
+            "fp64_one_at_a_time": r["fp64_one_at_a_time"],
+            "config": "64 lattices, edge U{8..24} cells, T U(300,700) K, x U(0.01,0.6)+-0.05, disp N(0,0.02), "
+                      "sigma 0.10, periodic, 12-point rule; lockstep BLMVM (m=5) to ||Pg||<=1e-8, mixed precision "
+                      "with the FP64 endgame; admission (lists, optimizer memory) inside the timed wall"}
+
+
+def serving_block(args):
+    """Continuous-batching serving over a Poisson rate sweep (a2c_amd.serve)."""
+    from a2c_amd import serve
+    rates = [float(r) for r in args.serve_rates.split(",")]
+
+    def note(r):
+        if "summary" in r:
+            sm = r["summary"]
+            log("serving %g req/s: %.2f relaxations/s, p50 %.3f s, p99 %.3f s" % (
+                r["rate"], sm["relaxations_per_s"], sm["p50_s"], sm["p99_s"]))
+    out = serve.sweep(rates, args.serve_requests, args.serve_fp64_sample, log=note)
+    out["metric"] = "relaxations/s + p50/p99 latency"
+    out["config"] = ("Poisson arrivals, edge log-uniform [6,48] cells, T U(300,700) K, x U(0,0.6)+-0.05, "
+                     "disp N(0,0.02), sigma 0.10, periodic; continuous batching (admission by HBM footprint, "
+                     "stride scheduling), mixed precision with the FP64 endgame, ||Pg||<=1e-8")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -272,8 +363,11 @@ def main():
         return
     import a2c_amd as A
 
+    from a2c_amd import workloads
+
     def mk(n):
-        return A.lattice(n, a=3.89, seed=2, disp_std=0.03, sigma=(0.08, 0.15), x=(0.0, 0.6), x_clip=(0.0, 1.0))
+        s = workloads.single(n)
+        return s["X"], s["x"], s["box"], s["n_pd"], s["n_h"]
 
     X, x, box, npd, nh = mk(args.cells)
     log("workload: %d cells^3, %d sites" % (args.cells, npd + nh))
@@ -310,7 +404,14 @@ def main():
         result["gh3"] = {k: g[k] for k in ("value", "ms_per_step", "steps", "kernels_ms", "fp64_gpu",
                                            "error_vs_fp64")}
         result["gh3"]["roofline"] = g["roofline"]
+    if not args.no_batched:
+        log("batched block")
+        result["batched"] = batched_block(args)
+    if not args.no_serving:
+        log("serving block")
+        result["serving"] = serving_block(args)
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(args, mk)
     print(json.dumps(result), flush=True)
 
@@ -356,18 +457,22 @@ def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False):
     gms = kern.get(gkern, float("nan"))
     achieved = flop_grad / (gms * 1e-3) / 1e12
     traffic = None  # HBM bytes per dispatch of gkern from the committed PMC summary (FETCH/WRITE_SIZE, scaled)
+    traffic_src = None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
         if tr.get("kernel") == gkern and tr.get("quad", "r12") == quad:
             traffic = tr["bytes_per_dispatch"]
+            traffic_src = "copied from %s (rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by the copy calibration)" % (
+                tr.get("source", "profiles/pmc_traffic.json"))
     log("%s %s: %.3f ms/eval %s" % (quad, args.precision, tm["total_ms"] / steps, kern))
     return {
         "value": N * steps / (tm["total_ms"] * 1e-3),
         "steps": steps,
         "ms_per_step": tm["total_ms"] / steps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[args.precision], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[args.precision], "traffic": traffic, "kernel": gkern,
+                     "frac": achieved / PEAK[args.precision], "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": gkern,
                      "kernel_ms": gms, "algorithmic_flop": flop_grad},
         "kernels_ms": kern,
         "fp64_gpu": {"value": N / (ms64 * 1e-3), "ms_per_step": ms64, "kernels_ms": tm64["kernels_ms"]},

@@ -41,7 +41,10 @@ void a2c_batch_destroy(a2c_batch* b);
 const char* a2c_batch_last_error(const a2c_batch* b);
 
 /* Device bytes one lattice occupies (state, lists, work arrays and an LMVM
- * with m pairs); admission compares the sum against hbm_budget_bytes. */
+ * with m pairs, with the device cache's size-class rounding); admission
+ * charges a lattice with m = 5 against hbm_budget_bytes and
+ * a2c_batch_lmvm_start re-charges it for the m it asks for (A2C_ENOMEM when
+ * that exceeds the budget); a2c_batch_remove releases the charge. */
 size_t a2c_batch_footprint(int32_t n_pd, int32_t n_h, int32_t lmvm_m);
 size_t a2c_batch_mem_used(const a2c_batch* b);
 

@@ -59,9 +59,13 @@ typedef struct a2c_dmd_params {
 typedef struct a2c_dmd_report {
   double f;               /* free entropy after the minimization */
   int32_t lmvm_reason, lmvm_iterations, lmvm_evals;
-  double total_h_before;  /* sum of x before the diffusion update */
+  double total_h_before;  /* sum of x before the diffusion update (compensated sum) */
   double total_h_after;   /* and after */
   double xH, xH_interior;
+  double flux_sum;        /* the diffusion step's balance (a2c_fe_diffusion_balance): */
+  double flux_gross;      /*   applied increments, their scale, */
+  double clamp_lift;      /*   the H added by the clamp to [xLb, xUb] */
+  double reservoir_supply;  /* and the H the full_H reservoirs supplied */
 } a2c_dmd_report;
 
 void a2c_dmd_params_default(a2c_dmd_params* p);

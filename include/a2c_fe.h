@@ -133,6 +133,24 @@ int a2c_fe_chem_potential(a2c_fe_ctx* ctx, const double* X, const double* coe_pd
 int a2c_fe_diffuse(a2c_fe_ctx* ctx, double dt, double nu, double Qm, double xLb,
                    double xUb, const double* gamma, const int32_t* full_h, double* x);
 
+/* Balance of total H over one diffusion step (no counterpart in the
+ * reference, which sums x after the step: integrate.cpp:76-86).  The edge
+ * terms are exactly antisymmetric, so the increments applied to the sites
+ * that diffuse (flux_sum) equal the H the full_h reservoirs supply (minus the
+ * increments they would have received; they keep x) up to the rounding of the
+ * per-site sums; flux_gross (the sum of |edge term| x rate x dt over both
+ * ends of every edge) is the scale that rounding is relative to.  The clamp
+ * to [xLb, xUb] (integrate.cpp:79-81) adds clamp_lift.  Total H changes by
+ * flux_sum + clamp_lift (plus the rounding of x + increment). */
+typedef struct a2c_diffusion_balance {
+  double flux_sum;
+  double flux_gross;
+  double clamp_lift;
+  double reservoir_supply;
+} a2c_diffusion_balance;
+/* The balance of the context's last a2c_fe_diffuse call (zeros before one). */
+int a2c_fe_diffusion_balance(const a2c_fe_ctx* ctx, a2c_diffusion_balance* out);
+
 /* ---- bounded limited-memory variable metric (replaces TAOBLMVM, -------
  * ---- minimizer.cpp:535, TaoSolve :569, bounds :804-830) ---------------- */
 typedef struct a2c_lmvm_opts {

@@ -39,6 +39,14 @@ def test_dmd_steps_conserve_and_match_oracle():
         xo = orc.diffuse(1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16, g, fh, x_before)
         assert np.max(np.abs(x - xo)) <= 1e-13
         assert rep["total_h_before"] == pytest.approx(x_before.sum(), rel=1e-15)
+        # balance of the step: the net flux is zero to round-off of its scale,
+        # and the change of total H is the clamp lift (the oracle's step with
+        # the clamp opened gives the unclamped occupancy)
+        assert abs(rep["flux_sum"]) <= 1e-12 * rep["flux_gross"] and rep["flux_gross"] > 0
+        xu = orc.diffuse(1e-13, 1e13, 0.23, -1e300, 1e300, g, fh, x_before)
+        assert rep["clamp_lift"] == pytest.approx((xo - xu).sum(), rel=1e-9, abs=1e-14)
+        dH = rep["total_h_after"] - rep["total_h_before"]
+        assert abs(dH - rep["clamp_lift"]) <= 1e-12 * rep["total_h_before"]
         clamped = np.any(x <= 1e-3) and np.any(x_before > 1e-3)
         if not clamped:
             assert abs(rep["total_h_after"] - rep["total_h_before"]) <= 1e-12 * rep["total_h_before"]

@@ -78,3 +78,6 @@ def test_diffusion_conserves_total_h_at_size():
     x1 = fe.integrateDiffusion(1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16, gam, np.zeros(nh, np.int32))
     assert np.all(x1 > 1e-3) and np.any(x1 != x)
     assert abs(x1.sum() - x.sum()) <= 1e-12 * x.sum()
+    b = fe.diffusion_balance()
+    assert b["clamp_lift"] == 0.0 and b["flux_gross"] > 0
+    assert abs(b["flux_sum"]) <= 1e-12 * b["flux_gross"]

@@ -46,7 +46,7 @@ def test_autotune_returns_a_valid_table_and_leaves_the_default(restore_table):
     assert A.tune_get() != before or before["max_sites"] == [2**31 - 1]
 
 
-@pytest.mark.parametrize("k", [2, 4])
+@pytest.mark.parametrize("k", LANES)
 def test_other_table_meets_the_bars_and_batches_bitwise(k, restore_table):
     A.tune_set({"max_sites": [2**31 - 1], "lanes_mixed": [k], "lanes_fp64": [k]})
     X, x, box, per, npd, nh = specimen(4, seed=2, disp=0.05, sigma=(0.08, 0.15), x=(0.0, 0.6))
