@@ -167,6 +167,7 @@ def _load():
         "a2c_fe_chem_potential": (I, [VP, dp, dp, dp, D, dp, ip]),
         "a2c_fe_diffuse": (I, [VP, D, D, D, D, D, dp, ip, dp]),
         "a2c_fe_diffusion_balance": (I, [VP, P(DiffusionBalance)]),
+        "a2c_fe_diffuse_device_noalloc": (I, [VP, D, D, D, D, D, VP, VP, VP, VP]),
         "a2c_lmvm_opts_default": (None, [P(LmvmOpts)]),
         "a2c_fe_minimize": (I, [VP, dp, P(LmvmOpts), dp, P(LmvmReport)]),
         "a2c_dev_alloc": (I, [VP, C.c_size_t, P(VP)]),

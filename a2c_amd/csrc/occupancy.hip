@@ -24,6 +24,8 @@
 // exactly antisymmetric; the total changes only by the rounding of the site
 // sums (plus clamps and full_H reservoirs, as in the reference).
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "eam64.hpp"
 #include "fe_internal.hpp"
@@ -272,3 +274,71 @@ int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, doubl
 }
 
 }  // namespace a2c
+
+// ------------------------------------------------------------------ no-alloc
+// a2c_fe_diffuse_device_noalloc: the diffusion step on device arrays with its
+// scratch in this file-scope array -- f for up to A2C_DIFFUSE_STATIC_MAX_H H
+// sites, then 4 balance partials per block of 256 sites, then the balance.
+// Each GPU holds its own copy for the life of the process.  Every kernel
+// overwrites what it reads later (f per site, one partial triple per block,
+// the balance), so nothing needs clearing between calls; calls that overlap
+// on one GPU share the array and race.
+namespace a2c {
+namespace {
+constexpr size_t STATIC_BLK = (A2C_DIFFUSE_STATIC_MAX_H + TB - 1) / TB;
+constexpr size_t STATIC_DOUBLES = A2C_DIFFUSE_STATIC_MAX_H + 4 * STATIC_BLK + 4;
+static __device__ __align__(256) double scratch[STATIC_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of this device's copy of scratch (host code never uses the
+// array's own name: on the host it is only a stand-in for the symbol).
+int static_scratch(a2c_fe_ctx* c, double** out) {
+  const int dev = c->desc.device;
+  if (dev < 0 || dev >= MAX_DEVICES) return set_err(c, A2C_EINVAL, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    A2C_HIP(c, hipGetDevice(&cur));
+    if (cur != dev) A2C_HIP(c, hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return set_err(c, A2C_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return A2C_OK;
+}
+}  // namespace
+}  // namespace a2c
+
+extern "C" int a2c_fe_diffuse_device_noalloc(a2c_fe_ctx* c, double dt, double nu, double Qm, double xLb, double xUb,
+                                             const double* d_gamma, const int32_t* d_full_h, double* d_x,
+                                             double* d_balance) {
+  using namespace a2c;
+  if (!c) return A2C_EINVAL;
+  c->err.clear();
+  const int nh = c->desc.n_h;
+  if (nh > A2C_DIFFUSE_STATIC_MAX_H)
+    return set_err(c, A2C_EINVAL,
+                   "a2c_fe_diffuse_device_noalloc: n_h = " + std::to_string(nh) + " exceeds its static scratch (" +
+                       std::to_string(A2C_DIFFUSE_STATIC_MAX_H) + " H sites); use a2c_fe_diffuse");
+  if (!c->have_nbr || !c->have_x) return set_err(c, A2C_ESTATE, "neighbors/occupancy not set");
+  if (nh && (!d_gamma || !d_full_h || !d_x)) return set_err(c, A2C_EINVAL, "null argument");
+  if (d_x == c->dx) return set_err(c, A2C_EINVAL, "d_x must not be the context occupancy");
+  if (!(xLb <= xUb)) return set_err(c, A2C_EINVAL, "xLb > xUb");
+  if (nh == 0) return A2C_OK;
+  double* base = nullptr;
+  if (int rc = static_scratch(c, &base)) return rc;
+  const DiffuseScratch w{base, base + A2C_DIFFUSE_STATIC_MAX_H,
+                         d_balance ? d_balance : base + A2C_DIFFUSE_STATIC_MAX_H + 4 * STATIC_BLK};
+  const double pref = nu * std::exp(-Qm / (KB * c->desc.temperature));  // integrate.cpp:26
+  if (int rc = launch_diffuse(c, dt, pref, xLb, xUb, d_gamma, d_full_h, w, d_x)) return rc;
+  A2C_HIP(c, hipMemcpyAsync(c->dx, d_x, 8 * (size_t)nh, hipMemcpyDeviceToDevice, c->stream));
+  return A2C_OK;
+}
+

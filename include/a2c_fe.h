@@ -151,6 +151,24 @@ typedef struct a2c_diffusion_balance {
 /* The balance of the context's last a2c_fe_diffuse call (zeros before one). */
 int a2c_fe_diffusion_balance(const a2c_fe_ctx* ctx, a2c_diffusion_balance* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): a2c_fe_diffuse on device-resident arrays that calls no
+ * allocator.  Its scratch (f = gamma - ln(x/(1-x)) per H site and the balance
+ * partials, ~8.5 MB) is a library-static device array, one per GPU for the
+ * life of the process, so n_h is limited to A2C_DIFFUSE_STATIC_MAX_H
+ * (A2C_EINVAL beyond it; a2c_fe_diffuse has no limit).  d_gamma (n_h
+ * doubles) and d_full_h (n_h int32) are inputs; d_x (n_h doubles, not the
+ * context's own occupancy) receives the new occupancy, which also becomes the
+ * context occupancy; d_balance (4 doubles: the fields of
+ * a2c_diffusion_balance in order) may be null.  Asynchronous on the context stream; results are bitwise those
+ * of a2c_fe_diffuse.  Calls that overlap on one GPU share the scratch and
+ * race: serialise them (one stream, or synchronise between calls). */
+#define A2C_DIFFUSE_STATIC_MAX_H 1048576
+int a2c_fe_diffuse_device_noalloc(a2c_fe_ctx* ctx, double dt, double nu, double Qm, double xLb,
+                                  double xUb, const double* d_gamma, const int32_t* d_full_h,
+                                  double* d_x, double* d_balance);
+
 /* ---- bounded limited-memory variable metric (replaces TAOBLMVM, -------
  * ---- minimizer.cpp:535, TaoSolve :569, bounds :804-830) ---------------- */
 typedef struct a2c_lmvm_opts {
