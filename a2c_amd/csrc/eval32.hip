@@ -99,6 +99,44 @@ __device__ __forceinline__ float tab_v(const float4* __restrict__ tab, int w, fl
   if (!p.far) return cubic(tab[w * NT + p.k], p.u);
   return tail(w, r);
 }
+// The same without the branch: the row of the clamped interval, with far set
+// when r lies outside the table.  A per-point branch makes every point its own
+// basic block, so a wave waits out each point's LDS read before the next one
+// can issue; the pair passes evaluate all points of a pair this way and redo
+// the (rare) pair with a point outside [1, 6) A through the branching form.
+__device__ __forceinline__ float tab_vf(const float4* __restrict__ tab, int w, float r, bool& far) {
+  const Pos p = tab_pos(r);
+  far |= p.far;
+  return cubic(tab[w * NT + p.k], p.u);
+}
+
+// f_H(r) = 11.0025 exp(-1.30927 r) (potential.cpp:105-119; the exponential
+// tail past 5.35 A continues the same function to 1e-13), evaluated without a
+// table as 11.0025 * 2^(K r), K = -1.30927 log2(e) split into FP32 hi and lo
+// parts, with the rounding of K r carried into a first-order correction: the
+// value is within about an ulp of FP32 and costs one transcendental and five
+// FMA-class instructions instead of a 16-byte LDS lookup.  NaN below 1 A, as
+// the tables (see tab_pos).  Its slope is FH_K times the value.
+constexpr double FH_K = -1.30927;
+constexpr double LOG2E = 1.4426950408889634074;
+constexpr float FH_C = 11.0025f;
+constexpr float FH_KHI = (float)(FH_K * LOG2E);
+constexpr float FH_KLO = (float)(FH_K * LOG2E - (double)FH_KHI);
+constexpr float FH_KF = (float)FH_K;
+constexpr float LN2F = 0.69314718055994530942f;
+__device__ __forceinline__ float fH_exp(float r) {
+  const float e = FH_KHI * r;
+  const float el = fmaf(FH_KLO, r, fmaf(FH_KHI, r, -e));
+  const float v = FH_C * __builtin_amdgcn_exp2f(e) * fmaf(el, LN2F, 1.0f);
+  return r >= R0 ? v : __builtin_nanf("");
+}
+// density of a species at r: f_H directly, f_Pd from the value table in LDS
+// rows [0, NT) of the pair-owner density pass
+template <bool H, bool FAST>
+__device__ __forceinline__ float dens(const float4* __restrict__ tab, float r, bool& far) {
+  if (H) return fH_exp(r);
+  return FAST ? tab_vf(tab, T_FPD, r, far) : tab_v(tab, T_FPD, r);
+}
 
 struct Site {
   double q[3];
@@ -107,6 +145,19 @@ struct Site {
 __device__ __forceinline__ Site load_site(const double* __restrict__ X, int s) {
   const double4 v = *reinterpret_cast<const double4*>(X + 4 * (size_t)s);
   return {{v.x, v.y, v.z}, v.w};
+}
+__device__ __forceinline__ Site load_site(const A2C_GLOBAL double4* __restrict__ X, int s) {
+  const double4 v = X[s];
+  return {{v.x, v.y, v.z}, v.w};
+}
+// d = q_i - image(q_j) in FP64, rounded to FP32 (geometry held in registers)
+__device__ __forceinline__ void pair_vec32(const Image& g, const Site& a, const Site& b, float d[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double qj = b.q[c];
+    if (g.per[c]) qj = qj - g.box[c] * rint((qj - a.q[c]) * g.ibox[c]);
+    d[c] = (float)(a.q[c] - qj);
+  }
 }
 // d = q_i - image(q_j) in FP64, rounded to FP32
 __device__ __forceinline__ void pair_vec32(const LatDev& S, const Site& a, const Site& b, float d[3]) {
@@ -122,12 +173,17 @@ __device__ __forceinline__ void stage_tables(float4* lds, const float4* __restri
   for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
   __syncthreads();
 }
-// Gradient passes need f only through f': rows 0..2*NT-1 (f_Pd, f_H) are
-// staged as derivative rows {c1, 2 c2, 3 c3, 0}; the pair tables stay cubic.
+// Gradient passes need f_Pd only through f_Pd', staged as derivative rows
+// {c1, 2 c2, 3 c3, 0}; the pair tables stay cubic; f_H is evaluated directly.
+// LDS layout of the gradient passes: G_FPD (f_Pd'), G_PHPD, G_PHPDH, G_PHH,
+// NT rows each (32 KB).
+enum { G_FPD = 0, G_PHPD = 1, G_PHPDH = 2, G_PHH = 3, NGTAB = 4 };
 __device__ __forceinline__ void stage_grad_tables(float4* lds, const float4* __restrict__ g) {
-  for (int i = threadIdx.x; i < NTAB * NT; i += blockDim.x) {
-    const float4 c = g[i];
-    lds[i] = i < 2 * NT ? make_float4(c.y, 2.0f * c.z, 3.0f * c.w, 0.0f) : c;
+  for (int i = threadIdx.x; i < NGTAB * NT; i += blockDim.x) {
+    const int w = i / NT, k = i - w * NT;
+    const int src = w == G_FPD ? T_FPD : w + 1;  // T_PHPD, T_PHPDH, T_PHH follow T_FH in the global tables
+    const float4 c = g[src * NT + k];
+    lds[i] = w == G_FPD ? make_float4(c.y, 2.0f * c.z, 3.0f * c.w, 0.0f) : c;
   }
   __syncthreads();
 }
@@ -200,31 +256,39 @@ __global__ __launch_bounds__(TB) void k_density32(const LatDev* __restrict__ lat
 // of the objective, dE/dr = F'_i f_j'(r) + F'_j f_i'(r) + phi'(r) (f_k: the
 // density function of site k's species).  One table position serves all
 // three functions.
-template <int KIND>
-__device__ __forceinline__ void point_coef(const float4* tab, float r, float Fi, float Fj, float& phi, float& cu) {
-  constexpr int WP = KIND == 0 ? T_PHPD : (KIND == 3 ? T_PHH : T_PHPDH);
+template <int KIND, bool FAST = false>
+__device__ __forceinline__ void point_coef(const float4* tab, float r, float Fi, float Fj, float& phi, float& cu,
+                                           bool& far) {
+  constexpr int WP = KIND == 0 ? G_PHPD : (KIND == 3 ? G_PHH : G_PHPDH);    // LDS table of phi
+  constexpr int TP = KIND == 0 ? T_PHPD : (KIND == 3 ? T_PHH : T_PHPDH);    // its tail
   const Pos p = tab_pos(r);
   float dphi;
   cubic_vd(tab[WP * NT + p.k], p.u, phi, dphi);
-  if (KIND == 0 || KIND == 3) {
-    constexpr int WF = KIND == 0 ? T_FPD : T_FH;
-    cu = fmaf(Fi + Fj, quad(tab[WF * NT + p.k], p.u), dphi);
+  // h f_H'(r) (h = 1/INVH, the table step): f_H' = FH_K f_H
+  const float hdfh = KIND == 0 ? 0.0f : (FH_KF / INVH) * fH_exp(r);
+  if (KIND == 0) {
+    cu = fmaf(Fi + Fj, quad(tab[G_FPD * NT + p.k], p.u), dphi);
+  } else if (KIND == 3) {
+    cu = fmaf(Fi + Fj, hdfh, dphi);
   } else {
     const float Fpd = KIND == 1 ? Fi : Fj;  // F' of the Pd site multiplies f_H'
     const float Fh = KIND == 1 ? Fj : Fi;   // F' of the H site multiplies f_Pd'
-    cu = fmaf(Fpd, quad(tab[T_FH * NT + p.k], p.u), fmaf(Fh, quad(tab[T_FPD * NT + p.k], p.u), dphi));
+    cu = fmaf(Fpd, hdfh, fmaf(Fh, quad(tab[G_FPD * NT + p.k], p.u), dphi));
   }
-  if (p.far) {
-    phi = tail(WP, r);
-    const float dp = TAIL[WP][1] * phi;
+  if (FAST) {
+    far |= p.far;
+  } else if (p.far) {
+    phi = tail(TP, r);
+    const float dp = TAIL[TP][1] * phi;
     float c;
-    if (KIND == 0 || KIND == 3) {
-      constexpr int WF = KIND == 0 ? T_FPD : T_FH;
-      c = fmaf(Fi + Fj, TAIL[WF][1] * tail(WF, r), dp);
+    if (KIND == 0) {
+      c = fmaf(Fi + Fj, TAIL[T_FPD][1] * tail(T_FPD, r), dp);
+    } else if (KIND == 3) {
+      c = fmaf(Fi + Fj, hdfh * INVH, dp);
     } else {
       const float Fpd = KIND == 1 ? Fi : Fj;
       const float Fh = KIND == 1 ? Fj : Fi;
-      c = fmaf(Fpd, TAIL[T_FH][1] * tail(T_FH, r), fmaf(Fh, TAIL[T_FPD][1] * tail(T_FPD, r), dp));
+      c = fmaf(Fpd, hdfh * INVH, fmaf(Fh, TAIL[T_FPD][1] * tail(T_FPD, r), dp));
     }
     cu = c * (1.0f / INVH);
   }
@@ -253,7 +317,8 @@ __device__ __forceinline__ PairSums pair_grad(const float4* tab, const float d[3
       const float x = fmaxf(r2[q], 1e-12f);
       const float rinv = rsq(x);
       float phi, cc;
-      point_coef<KIND>(tab, x * rinv, Fi, Fj, phi, cc);
+      bool far = false;
+      point_coef<KIND>(tab, x * rinv, Fi, Fj, phi, cc, far);
       pe += phi;
       u[q] = cc * rinv;
     }
@@ -272,7 +337,7 @@ __device__ __forceinline__ PairSums pair_grad(const float4* tab, const float d[3
 
 __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l,
                                                const float4* __restrict__ gtab) {
-  __shared__ float4 tab[NTAB * NT];
+  __shared__ float4 tab[NGTAB * NT];
   stage_grad_tables(tab, gtab);
   const int blk = xcd_block();
   const LatDev& S = lats[b2l[blk]];
@@ -349,11 +414,13 @@ __global__ __launch_bounds__(TB) void k_grad32(const LatDev* __restrict__ lats, 
 // received slot; a gather kernel adds the received shares in FP64.  Each
 // buffer entry is written by exactly one lane, so results are deterministic.
 
-// 12-point sums of two density tables at the same points.  One branch per
-// lookup keeps the live range short (~63 VGPRs, 8 waves per SIMD), which
-// this latency-bound pass prefers over hoisting all 12 points' loads.
-__device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb, const float d[3], float ai,
-                                              float aj, float& sa, float& sb) {
+// 12-point sums of two densities at the same points: the density of the
+// neighbor's species (for this site) and of this site's species (for the
+// neighbor); one sum when the species are the same.  NB_H / ME_H: neighbor /
+// this site is an H site.  FAST: no per-point branch (see tab_vf).
+template <bool NB_H, bool ME_H, bool FAST>
+__device__ __forceinline__ void pair_density2(const float4* tab, const float d[3], float ai, float aj, float& sa,
+                                              float& sb, bool& far) {
   const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   const float bi = D2 + ai * ai, bj = D2 + aj * aj;
   sa = 0.0f;
@@ -366,18 +433,19 @@ __device__ __forceinline__ void pair_density2(const float4* tab, int wa, int wb,
     for (int q = 0; q < 4; ++q) {
       const float x = fmaxf(r2[q], 1e-12f);
       const float r = x * rsq(x);
-      sa += tab_v(tab, wa, r);
-      if (wb != wa) sb += tab_v(tab, wb, r);
+      sa += dens<NB_H, FAST>(tab, r, far);
+      if (NB_H != ME_H) sb += dens<ME_H, FAST>(tab, r, far);
     }
   }
-  if (wb == wa) sb = sa;
+  if (NB_H == ME_H) sb = sa;
 }
 
-// Weighted Gauss-Hermite sums of two density tables at the 27 nodes of a pair
+// Weighted Gauss-Hermite sums of the two densities at the 27 nodes of a pair
 // (s = sqrt(sigma_i^2 + sigma_j^2)): a rolled loop over the z offset a2, the
 // 9 nodes of each plane unrolled (a fully unrolled 27 spills at 64 VGPRs).
-__device__ __forceinline__ void pair_density2_gh(const float4* tab, int wa, int wb, const float d[3], float sd,
-                                                 float& sa, float& sb) {
+template <bool NB_H, bool ME_H, bool FAST>
+__device__ __forceinline__ void pair_density2_gh(const float4* tab, const float d[3], float sd, float& sa,
+                                                 float& sb, bool& far) {
   const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   const float s3 = 3.0f * sd * sd, tt = 2.0f * GHZF * sd;
   const float t0 = tt * d[0], t1 = tt * d[1], t2 = tt * d[2];
@@ -394,15 +462,44 @@ __device__ __forceinline__ void pair_density2_gh(const float4* tab, int wa, int 
         const float w = ghwf(a0) * ghwf(a1);
         const float x = fmaxf(base + s3 * (float)(a0 * a0 + a1 * a1) + (a0 * t0 + a1 * t1), 1e-12f);
         const float r = x * rsq(x);
-        pa += w * tab_v(tab, wa, r);
-        if (wb != wa) pb += w * tab_v(tab, wb, r);
+        pa += w * dens<NB_H, FAST>(tab, r, far);
+        if (NB_H != ME_H) pb += w * dens<ME_H, FAST>(tab, r, far);
       }
     }
     const float w2 = a2 ? 1.0f / 6.0f : 2.0f / 3.0f;
     sa += w2 * pa;
     sb += w2 * pb;
   }
-  if (wb == wa) sb = sa;
+  if (NB_H == ME_H) sb = sa;
+}
+
+template <int RULE, bool NB_H, bool ME_H, bool FAST>
+__device__ __forceinline__ void pair_dens_rule(const float4* tab, const float d[3], float as, float bs, float& mine,
+                                               float& theirs, bool& far) {
+  if (RULE == GH3) pair_density2_gh<NB_H, ME_H, FAST>(tab, d, sqrtf(as * as + bs * bs), mine, theirs, far);
+  else pair_density2<NB_H, ME_H, FAST>(tab, d, SQ6F * as, SQ6F * bs, mine, theirs, far);
+}
+// all points without a branch; the pair again with the tails if a point fell
+// outside the table
+template <int RULE, bool NB_H, bool ME_H>
+__device__ __attribute__((noinline)) float2 pair_dens_tails(const float4* tab, float d0, float d1, float d2, float as,
+                                                            float bs) {
+  const float d[3] = {d0, d1, d2};
+  bool far = false;
+  float mine, theirs;
+  pair_dens_rule<RULE, NB_H, ME_H, false>(tab, d, as, bs, mine, theirs, far);
+  return make_float2(mine, theirs);
+}
+template <int RULE, bool NB_H, bool ME_H>
+__device__ __forceinline__ void pair_dens(const float4* tab, const float d[3], float as, float bs, float& mine,
+                                          float& theirs) {
+  bool far = false;
+  pair_dens_rule<RULE, NB_H, ME_H, true>(tab, d, as, bs, mine, theirs, far);
+  if (far) {
+    const float2 v = pair_dens_tails<RULE, NB_H, ME_H>(tab, d[0], d[1], d[2], as, bs);
+    mine = v.x;
+    theirs = v.y;
+  }
 }
 
 // Partner list of each list: PdPd<->PdPd, PdH<->HPd, HH<->HH.
@@ -410,28 +507,29 @@ __device__ __forceinline__ const Ell& partner_of(const LatDev& S, int list) {
   return list == 0 ? S.pdpd : (list == 1 ? S.hpd : (list == 2 ? S.pdh : S.hh));
 }
 
-// 8 waves per SIMD: the lookups are latency-bound, 64 VGPRs hold the 12-point
-// pass without spills (the Gauss-Hermite planes take 69: 7 waves)
+// At least 6 waves per SIMD (at most 80 VGPRs): the 12-point pass takes 71
+// without spills (7 waves), the Gauss-Hermite planes 80
 template <int RULE>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(RULE == GH3 ? 6 : 8))) void k_dens_own(
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) void k_dens_own(
     const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l, const float4* __restrict__ gtab) {
-  __shared__ float4 tab[2 * NT];  // f_Pd, f_H
+  __shared__ float4 tab[NT];  // f_Pd (f_H is evaluated directly)
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
   if (S.rule != RULE) return;  // whole workgroup: the other rule's instance serves this lattice
-  stage_tables(tab, gtab, 2 * NT);
+  stage_tables(tab, gtab, NT);
   const int K = S.kl32;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_32) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
-  const int N = S.npd + S.nh;
+  const int npd = S.npd;
+  const int N = npd + S.nh;
   if (s >= N) return;  // whole lane groups
-  const double* __restrict__ X = S.X;
-  const double* __restrict__ xo = S.x;
-  const bool pd = s < S.npd;
-  const int row = pd ? s : s - S.npd;
+  // everything the pair loops use, out of the descriptor and into registers
+  const A2C_GLOBAL double4* __restrict__ X = gmem(reinterpret_cast<const double4*>(S.X));
+  const A2C_GLOBAL double* __restrict__ xo = gmem(S.x);
+  const Image img = image_of(S);
+  const bool pd = s < npd;
+  const int row = pd ? s : s - npd;
   const Site a = load_site(X, s);
-  const float ai = SQ6F * (float)a.s;
-  const int mytab = pd ? T_FPD : T_FH;
   const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
   double rho = 0.0;
 #pragma unroll
@@ -439,28 +537,31 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(RULE == GH3 
     const int list = pd ? side : 2 + side;
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const Ell& P = partner_of(S, list);
-    const int n = L.ocnt[row];
+    const A2C_GLOBAL int32_t* __restrict__ oj = gmem(L.oj);
+    const A2C_GLOBAL int32_t* __restrict__ orv = gmem(L.orv);
+    A2C_GLOBAL float* __restrict__ prb = gmem(reinterpret_cast<float*>(P.rbuf));
+    const size_t rows = L.rows, prows = P.rows;
+    const int n = gmem(L.ocnt)[row];
     for (int t = ph; t < n; t += K) {
-      const size_t e = (size_t)t * L.rows + row;
-      const int jl = L.oj[e];
-      const int j = side == 0 ? jl : S.npd + jl;
-      const Site b = load_site(X, j);
+      const size_t e = (size_t)t * rows + row;
+      const int jl = oj[e];
+      const int rv = orv[e];
+      const Site b = load_site(X, side == 0 ? jl : npd + jl);
+      const double wj = side == 0 ? 1.0 : xo[jl];
       float d[3];
-      pair_vec32(S, a, b, d);
+      pair_vec32(img, a, b, d);
       float mine, theirs;  // neighbor-type density (for me), my-type density (for the neighbor)
-      if (RULE == GH3) {
-        const float bs = (float)b.s, as = (float)a.s;
-        pair_density2_gh(tab, side == 0 ? T_FPD : T_FH, mytab, d, sqrtf(as * as + bs * bs), mine, theirs);
-      } else {
-        pair_density2(tab, side == 0 ? T_FPD : T_FH, mytab, d, ai, SQ6F * (float)b.s, mine, theirs);
-      }
-      rho += (side == 0 ? 1.0 : xo[jl]) * (double)mine;
-      const int rv = L.orv[e];
-      if (rv >= 0) reinterpret_cast<float*>(P.rbuf)[(size_t)rv * P.rows + jl] = (float)(wme * (double)theirs);
+      const float as = (float)a.s, bs = (float)b.s;
+      if (list == 0) pair_dens<RULE, false, false>(tab, d, as, bs, mine, theirs);
+      else if (list == 1) pair_dens<RULE, true, false>(tab, d, as, bs, mine, theirs);
+      else if (list == 2) pair_dens<RULE, false, true>(tab, d, as, bs, mine, theirs);
+      else pair_dens<RULE, true, true>(tab, d, as, bs, mine, theirs);
+      rho += wj * (double)mine;
+      if (rv >= 0) prb[(size_t)rv * prows + jl] = (float)(wme * (double)theirs);
     }
   }
   rho = site_sum(rho, K);
-  if (ph == 0) S.rho[s] = rho;
+  if (ph == 0) gmem(S.rho)[s] = rho;
 }
 
 __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
@@ -479,7 +580,7 @@ __global__ __launch_bounds__(TB) void k_dens_recv(const LatDev* __restrict__ lat
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const int n = L.rcnt[row];
     // shares arrive weighted by their senders
-    for_row_shares<RECV_RU_SCALAR>(reinterpret_cast<const float*>(L.rbuf), L.rows, row, n, [&](float v) { rho += (double)v; });
+    for_row_shares<RECV_RU_SCALAR>(gmem(reinterpret_cast<const float*>(L.rbuf)), L.rows, row, n, [&](float v) { rho += (double)v; });
   }
   rho *= S.rule == GH3 ? 1.0 : QW;
   S.rho[s] = rho;
@@ -496,9 +597,9 @@ struct PairSums2 {
   float gsj;   // partner sigma term
 };
 
-template <int KIND>
+template <int KIND, bool FAST>
 __device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d[3], float ai, float aj, float Fi,
-                                                float Fj) {
+                                                float Fj, bool& far) {
   const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   const float bi = D2 + ai * ai, bj = D2 + aj * aj;
   float su = 0.0f, ui = 0.0f, uj = 0.0f, pe = 0.0f, ti[3], tj[3];
@@ -512,7 +613,7 @@ __device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d
       const float x = fmaxf(r2[q], 1e-12f);
       const float rinv = rsq(x);
       float phi, cc;
-      point_coef<KIND>(tab, x * rinv, Fi, Fj, phi, cc);
+      point_coef<KIND, FAST>(tab, x * rinv, Fi, Fj, phi, cc, far);
       pe += phi;
       u[q] = cc * rinv;
     }
@@ -531,13 +632,75 @@ __device__ __forceinline__ PairSums2 pair_grad2(const float4* tab, const float d
   return p;
 }
 
+// pair_grad2 for the points inside the table, with the table rows of one
+// axis's four points read before any of them is used (their LDS latencies
+// overlap); far is set when a point lies outside [1, 6) A.
+template <int KIND>
+__device__ __forceinline__ PairSums2 pair_grad2_rows(const float4* tab, const float d[3], float ai, float aj,
+                                                     float Fi, float Fj, bool& far) {
+  constexpr int WP = KIND == 0 ? G_PHPD : (KIND == 3 ? G_PHH : G_PHPDH);
+  constexpr bool NEED_F = KIND != 3;  // f_Pd' row
+  const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float bi = D2 + ai * ai, bj = D2 + aj * aj;
+  float ui = 0.0f, uj = 0.0f, pe = 0.0f, ti[3], tj[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float t1 = 2.0f * ai * d[c], t2 = 2.0f * aj * d[c];
+    const float r2[4] = {bi + t1, bi - t1, bj - t2, bj + t2};
+    float rinv[4], r[4], u[4];
+    Pos p[4];
+    float4 rp[4], rf[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = fmaxf(r2[q], 1e-12f);
+      rinv[q] = rsq(x);
+      r[q] = x * rinv[q];
+      p[q] = tab_pos(r[q]);
+      far |= p[q].far;
+      rp[q] = tab[WP * NT + p[q].k];
+      if (NEED_F) rf[q] = tab[G_FPD * NT + p[q].k];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float phi, dphi, cu;
+      cubic_vd(rp[q], p[q].u, phi, dphi);
+      if (KIND == 0) {
+        cu = fmaf(Fi + Fj, quad(rf[q], p[q].u), dphi);
+      } else {
+        const float hdfh = (FH_KF / INVH) * fH_exp(r[q]);
+        if (KIND == 3) {
+          cu = fmaf(Fi + Fj, hdfh, dphi);
+        } else {
+          const float Fpd = KIND == 1 ? Fi : Fj;
+          const float Fh = KIND == 1 ? Fj : Fi;
+          cu = fmaf(Fpd, hdfh, fmaf(Fh, quad(rf[q], p[q].u), dphi));
+        }
+      }
+      pe += phi;
+      u[q] = cu * rinv[q];
+    }
+    ti[c] = u[0] - u[1];
+    tj[c] = u[2] - u[3];
+    ui += u[0] + u[1];
+    uj += u[2] + u[3];
+  }
+  const float su = ui + uj;
+  PairSums2 pr;
+  pr.pe = pe;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pr.g[c] = INVH * fmaf(d[c], su, ai * ti[c] - aj * tj[c]);
+  pr.gsi = INVH * fmaf(d[0], ti[0], fmaf(d[1], ti[1], fmaf(d[2], ti[2], ai * ui)));
+  pr.gsj = INVH * fmaf(aj, uj, -fmaf(d[0], tj[0], fmaf(d[1], tj[1], d[2] * tj[2])));
+  return pr;
+}
+
 // Gauss-Hermite pair sums.  With u_k = w_k (c/|r_k|) and the node offsets
 // a_k, n_k = |a_k|^2:  U = sum u, Va = sum u a, Wn = sum u n;
 //   q-gradient  sum u r = d U + sqrt(3) s Va,
 //   Sz = sum u (r . z) = sqrt(3) d . Va + 3 s Wn,  d/dsigma_k = Sz sigma_k / s.
-template <int KIND>
+template <int KIND, bool FAST>
 __device__ __forceinline__ PairSums2 pair_grad2_gh(const float4* tab, const float d[3], float si, float sj, float Fi,
-                                                   float Fj) {
+                                                   float Fj, bool& far) {
   const float sd = sqrtf(si * si + sj * sj);
   const float D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   const float s3 = 3.0f * sd * sd, tt = 2.0f * GHZF * sd;
@@ -556,7 +719,7 @@ __device__ __forceinline__ PairSums2 pair_grad2_gh(const float4* tab, const floa
         const float x = fmaxf(base + s3 * (float)n + (a0 * t0 + a1 * t1), 1e-12f);
         const float rinv = rsq(x);
         float phi, cc;
-        point_coef<KIND>(tab, x * rinv, Fi, Fj, phi, cc);
+        point_coef<KIND, FAST>(tab, x * rinv, Fi, Fj, phi, cc, far);
         p9 += w * phi;
         const float u = w * cc * rinv;
         u9 += u;
@@ -585,10 +748,30 @@ __device__ __forceinline__ PairSums2 pair_grad2_gh(const float4* tab, const floa
   return p;
 }
 
+// all points without a branch; the pair again with the tails if a point fell
+// outside the table
+template <int RULE, int KIND>
+__device__ __attribute__((noinline)) PairSums2 pair_grad_tails(const float4* tab, float d0, float d1, float d2,
+                                                               float as, float bs, float Fi, float Fj) {
+  const float d[3] = {d0, d1, d2};
+  bool far = false;
+  return RULE == GH3 ? pair_grad2_gh<KIND, false>(tab, d, as, bs, Fi, Fj, far)
+                     : pair_grad2<KIND, false>(tab, d, SQ6F * as, SQ6F * bs, Fi, Fj, far);
+}
+template <int RULE, int KIND>
+__device__ __forceinline__ PairSums2 pair_grad(const float4* tab, const float d[3], float as, float bs, float Fi,
+                                               float Fj) {
+  bool far = false;
+  PairSums2 p = RULE == GH3 ? pair_grad2_gh<KIND, true>(tab, d, as, bs, Fi, Fj, far)
+                            : pair_grad2_rows<KIND>(tab, d, SQ6F * as, SQ6F * bs, Fi, Fj, far);
+  if (far) p = pair_grad_tails<RULE, KIND>(tab, d[0], d[1], d[2], as, bs, Fi, Fj);
+  return p;
+}
+
 template <int RULE>
 __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats, const int32_t* __restrict__ ob2l,
                                                  const float4* __restrict__ gtab) {
-  __shared__ float4 tab[NTAB * NT];
+  __shared__ float4 tab[NGTAB * NT];
   const int w = xcd_block();
   const LatDev& S = lats[ob2l[w]];
   if (S.rule != RULE) return;  // whole workgroup: the other rule's instance serves this lattice
@@ -596,15 +779,17 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
   const int K = S.kl32;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_32) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
-  const int N = S.npd + S.nh;
+  const int npd = S.npd;
+  const int N = npd + S.nh;
   if (s >= N) return;  // whole lane groups
-  const double* __restrict__ X = S.X;
-  const double* __restrict__ xo = S.x;
-  const float2* __restrict__ aux = S.aux;
-  const bool pd = s < S.npd;
-  const int row = pd ? s : s - S.npd;
+  // everything the pair loops use, out of the descriptor and into registers
+  const A2C_GLOBAL double4* __restrict__ X = gmem(reinterpret_cast<const double4*>(S.X));
+  const A2C_GLOBAL double* __restrict__ xo = gmem(S.x);
+  const A2C_GLOBAL float2* __restrict__ aux = gmem(S.aux);
+  const Image img = image_of(S);
+  const bool pd = s < npd;
+  const int row = pd ? s : s - npd;
   const Site a = load_site(X, s);
-  const float ai = SQ6F * (float)a.s;
   const float Fi = aux[s].x;
   const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
   double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, gs = 0.0;
@@ -613,50 +798,51 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
     const int list = pd ? side : 2 + side;
     const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
     const Ell& P = partner_of(S, list);
-    const int n = L.ocnt[row];
-    int jl_n = 0;  // the next pair's neighbor, state and F' (loaded one pair ahead)
+    const A2C_GLOBAL int32_t* __restrict__ oj = gmem(L.oj);
+    const A2C_GLOBAL int32_t* __restrict__ orv = gmem(L.orv);
+    A2C_GLOBAL float4* __restrict__ prb = gmem(reinterpret_cast<float4*>(P.rbuf));
+    const size_t rows = L.rows, prows = P.rows;
+    const int n = gmem(L.ocnt)[row];
+    const int off = side == 0 ? 0 : npd;  // site index of neighbor 0 of this list
+    // the next pair's neighbor, slot, state, F' and weight (loaded one pair ahead)
+    int jl_n = 0, rv_n = -1;
     Site b_n{};
     float Fj_n = 0.0f;
+    double w_n = 1.0;
     if (ph < n) {
-      jl_n = L.oj[(size_t)ph * L.rows + row];
-      const int jn = side == 0 ? jl_n : S.npd + jl_n;
-      b_n = load_site(X, jn);
-      Fj_n = aux[jn].x;
+      const size_t e0 = (size_t)ph * rows + row;
+      jl_n = oj[e0];
+      rv_n = orv[e0];
+      b_n = load_site(X, off + jl_n);
+      Fj_n = aux[off + jl_n].x;
+      if (side) w_n = xo[jl_n];
     }
     for (int t = ph; t < n; t += K) {
-      const size_t ei = (size_t)t * L.rows + row;
-      const int jl = jl_n;
+      const int jl = jl_n, rv = rv_n;
       const Site b = b_n;
       const float Fj = Fj_n;
+      const double wj = w_n;
       if (t + K < n) {  // the next pair's gathers overlap this pair's points
-        jl_n = L.oj[(size_t)(t + K) * L.rows + row];
-        const int jn = side == 0 ? jl_n : S.npd + jl_n;
-        b_n = load_site(X, jn);
-        Fj_n = aux[jn].x;
+        const size_t en = (size_t)(t + K) * rows + row;
+        jl_n = oj[en];
+        rv_n = orv[en];
+        b_n = load_site(X, off + jl_n);
+        Fj_n = aux[off + jl_n].x;
+        if (side) w_n = xo[jl_n];
       }
       float d[3];
-      pair_vec32(S, a, b, d);
-      const float aj = SQ6F * (float)b.s;
+      pair_vec32(img, a, b, d);
+      const float as = (float)a.s, bs = (float)b.s;
       PairSums2 p;
-      if (RULE == GH3) {
-        const float as = (float)a.s, bs = (float)b.s;
-        if (list == 0) p = pair_grad2_gh<0>(tab, d, as, bs, Fi, Fj);
-        else if (list == 1) p = pair_grad2_gh<1>(tab, d, as, bs, Fi, Fj);
-        else if (list == 2) p = pair_grad2_gh<2>(tab, d, as, bs, Fi, Fj);
-        else p = pair_grad2_gh<3>(tab, d, as, bs, Fi, Fj);
-      } else {
-        if (list == 0) p = pair_grad2<0>(tab, d, ai, aj, Fi, Fj);
-        else if (list == 1) p = pair_grad2<1>(tab, d, ai, aj, Fi, Fj);
-        else if (list == 2) p = pair_grad2<2>(tab, d, ai, aj, Fi, Fj);
-        else p = pair_grad2<3>(tab, d, ai, aj, Fi, Fj);
-      }
-      const double w = side == 0 ? 1.0 : xo[jl];
-      e += w * p.pe;
-      g0 += w * p.g[0]; g1 += w * p.g[1]; g2 += w * p.g[2];
-      gs += w * p.gsi;
-      const int rv = L.orv[ei];
+      if (list == 0) p = pair_grad<RULE, 0>(tab, d, as, bs, Fi, Fj);
+      else if (list == 1) p = pair_grad<RULE, 1>(tab, d, as, bs, Fi, Fj);
+      else if (list == 2) p = pair_grad<RULE, 2>(tab, d, as, bs, Fi, Fj);
+      else p = pair_grad<RULE, 3>(tab, d, as, bs, Fi, Fj);
+      e += wj * p.pe;
+      g0 += wj * p.g[0]; g1 += wj * p.g[1]; g2 += wj * p.g[2];
+      gs += wj * p.gsi;
       if (rv >= 0)
-        reinterpret_cast<float4*>(P.rbuf)[(size_t)rv * P.rows + jl] =
+        prb[(size_t)rv * prows + jl] =
             make_float4((float)(-wme * p.g[0]), (float)(-wme * p.g[1]), (float)(-wme * p.g[2]), (float)(wme * p.gsj));
     }
   }
@@ -666,12 +852,8 @@ __global__ __launch_bounds__(TB) void k_grad_own(const LatDev* __restrict__ lats
   g2 = site_sum(g2, K);
   gs = site_sum(gs, K);
   if (ph != 0) return;
-  double* __restrict__ G = S.G;
-  G[4 * (size_t)s + 0] = g0;
-  G[4 * (size_t)s + 1] = g1;
-  G[4 * (size_t)s + 2] = g2;
-  G[4 * (size_t)s + 3] = gs;
-  S.epair[s] = e;
+  gmem(reinterpret_cast<double4*>(S.G))[s] = make_double4(g0, g1, g2, gs);
+  gmem(S.epair)[s] = e;
 }
 
 __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
@@ -694,7 +876,7 @@ __global__ __launch_bounds__(TB) void k_grad_recv(const LatDev* __restrict__ lat
       const Ell& L = list == 0 ? S.pdpd : (list == 1 ? S.pdh : (list == 2 ? S.hpd : S.hh));
       const int n = L.rcnt[row];
       // shares arrive weighted by their senders
-      for_row_shares<RECV_RU_WIDE>(reinterpret_cast<const float4*>(L.rbuf), L.rows, row, n, [&](const float4& v) {
+      for_row_shares<RECV_RU_WIDE>(gmem(reinterpret_cast<const float4*>(L.rbuf)), L.rows, row, n, [&](const float4& v) {
         g0 += v.x; g1 += v.y; g2 += v.z;
         gs += v.w;
       });

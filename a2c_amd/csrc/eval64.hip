@@ -77,6 +77,19 @@ __device__ __forceinline__ void pair_vec(const LatDev& S, const Site& a, const S
     d[c] = a.q[c] - qj;
   }
 }
+// the same with the geometry held in registers (pair-owner loops)
+__device__ __forceinline__ void pair_vec(const Image& g, const Site& a, const Site& b, double d[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double qj = b.q[c];
+    if (g.per[c]) qj = qj - g.box[c] * rint((qj - a.q[c]) * g.ibox[c]);
+    d[c] = a.q[c] - qj;
+  }
+}
+__device__ __forceinline__ Site load_site(const A2C_GLOBAL double4* __restrict__ X, int s) {
+  const double4 v = X[s];
+  return {{v.x, v.y, v.z}, v.w};
+}
 
 // ---------------------------------------------------------------- FP64 tables
 // Position of r in the tables: interval k, local coordinate u = (r - 1)/0.05 - k;
@@ -458,12 +471,15 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
-  const int N = S.npd + S.nh;
+  const int npd = S.npd;
+  const int N = npd + S.nh;
   if (s >= N) return;  // whole lane groups
-  const double* __restrict__ X = S.X;
-  const double* __restrict__ xo = S.x;
-  const bool pd = s < S.npd;
-  const int row = pd ? s : s - S.npd;
+  // everything the pair loops use, out of the descriptor and into registers
+  const A2C_GLOBAL double4* __restrict__ X = gmem(reinterpret_cast<const double4*>(S.X));
+  const A2C_GLOBAL double* __restrict__ xo = gmem(S.x);
+  const Image img = image_of(S);
+  const bool pd = s < npd;
+  const int row = pd ? s : s - npd;
   const Site a = load_site(X, s);
   const double ai = SQ6 * a.s;
   const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
@@ -473,14 +489,19 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
     const int list = pd ? side : 2 + side;
     const Ell& L = list_of(S, list);
     const Ell& P = partner_list(S, list);
-    double* __restrict__ share = reinterpret_cast<double*>(P.rbuf);
-    const int n = L.ocnt[row];
+    const A2C_GLOBAL int32_t* __restrict__ oj = gmem(L.oj);
+    const A2C_GLOBAL int32_t* __restrict__ orv = gmem(L.orv);
+    A2C_GLOBAL double* __restrict__ share = gmem(reinterpret_cast<double*>(P.rbuf));
+    const size_t rows = L.rows, prows = P.rows;
+    const int n = gmem(L.ocnt)[row];
     for (int t = ph; t < n; t += K) {
-      const size_t e = (size_t)t * L.rows + row;
-      const int jl = L.oj[e];
-      const Site b = load_site(X, side == 0 ? jl : S.npd + jl);
+      const size_t e = (size_t)t * rows + row;
+      const int jl = oj[e];
+      const int rv = orv[e];
+      const Site b = load_site(X, side == 0 ? jl : npd + jl);
+      const double wj = side == 0 ? 1.0 : xo[jl];
       double d[3];
-      pair_vec(S, a, b, d);
+      pair_vec(img, a, b, d);
       double mine = 0.0, theirs = 0.0;  // neighbor-species density at s, s-species density at the neighbor
       if (RULE == GH3) {
         const double sd = sqrt(a.s * a.s + b.s * b.s);
@@ -510,13 +531,12 @@ __global__ __launch_bounds__(TB) void k_dens_own64(const LatDev* __restrict__ la
           theirs += pd ? fpd : fh;
         }
       }
-      rho += (side == 0 ? 1.0 : xo[jl]) * mine;
-      const int rv = L.orv[e];
-      if (rv >= 0) share[(size_t)rv * P.rows + jl] = wme * theirs;
+      rho += wj * mine;
+      if (rv >= 0) share[(size_t)rv * prows + jl] = wme * theirs;
     }
   }
   rho = site_sum(rho, K);
-  if (ph == 0) S.rho[s] = rho;
+  if (ph == 0) gmem(S.rho)[s] = rho;
 }
 
 __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
@@ -534,7 +554,7 @@ __global__ __launch_bounds__(TB) void k_dens_recv64(const LatDev* __restrict__ l
     const Ell& L = list_of(S, pd ? side : 2 + side);
     const int n = L.rcnt[row];
     // shares arrive weighted by their senders
-    for_row_shares<RECV_RU_SCALAR>(reinterpret_cast<const double*>(L.rbuf), L.rows, row, n, [&](double v) { rho += v; });
+    for_row_shares<RECV_RU_SCALAR>(gmem(reinterpret_cast<const double*>(L.rbuf)), L.rows, row, n, [&](double v) { rho += v; });
   }
   rho *= S.rule == GH3 ? 1.0 : QW;
   S.rho[s] = rho;
@@ -662,13 +682,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int K = S.kl64;  // lanes per site: lane ph takes owned slots ph, ph + K, ...
   const int s = (w - S.owg0_64) * (TB / K) + (int)threadIdx.x / K;
   const int ph = (int)threadIdx.x % K;
-  const int N = S.npd + S.nh;
+  const int npd = S.npd;
+  const int N = npd + S.nh;
   if (s >= N) return;  // whole lane groups
-  const double* __restrict__ X = S.X;
-  const double* __restrict__ xo = S.x;
-  const double* __restrict__ Fp = S.Fp;
-  const bool pd = s < S.npd;
-  const int row = pd ? s : s - S.npd;
+  // everything the pair loops use, out of the descriptor and into registers
+  const A2C_GLOBAL double4* __restrict__ X = gmem(reinterpret_cast<const double4*>(S.X));
+  const A2C_GLOBAL double* __restrict__ xo = gmem(S.x);
+  const A2C_GLOBAL double* __restrict__ Fp = gmem(S.Fp);
+  const Image img = image_of(S);
+  const bool pd = s < npd;
+  const int row = pd ? s : s - npd;
   const Site a = load_site(X, s);
   const double Fi = Fp[s];
   const double wme = pd ? 1.0 : xo[row];  // shares leave weighted by the sender's occupancy
@@ -678,27 +701,32 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int list = pd ? side : 2 + side;
     const Ell& L = list_of(S, list);
     const Ell& P = partner_list(S, list);
-    double4* __restrict__ share = reinterpret_cast<double4*>(P.rbuf);
-    const int n = L.ocnt[row];
+    const A2C_GLOBAL int32_t* __restrict__ oj = gmem(L.oj);
+    const A2C_GLOBAL int32_t* __restrict__ orv = gmem(L.orv);
+    A2C_GLOBAL double4* __restrict__ share = gmem(reinterpret_cast<double4*>(P.rbuf));
+    const size_t rows = L.rows, prows = P.rows;
+    const int n = gmem(L.ocnt)[row];
+    const int off = side == 0 ? 0 : npd;  // site index of neighbor 0 of this list
     for (int t = ph; t < n; t += K) {
-      const size_t ei = (size_t)t * L.rows + row;
-      const int jl = L.oj[ei];
-      const int j = side == 0 ? jl : S.npd + jl;
-      const Site b = load_site(X, j);
+      // (no one-pair-ahead prefetch: an FP64 pair is long enough to cover its
+      // gathers, and the registers would push the kernel into spills)
+      const size_t ei = (size_t)t * rows + row;
+      const int jl = oj[ei];
+      const int rv = orv[ei];
+      const Site b = load_site(X, off + jl);
+      const double Fj = Fp[off + jl];
+      const double wj = side == 0 ? 1.0 : xo[jl];
       double d[3];
-      pair_vec(S, a, b, d);
-      const double Fj = Fp[j];
+      pair_vec(img, a, b, d);
       double pe, pg[3], psi, psj;
       if (list == 0) pair_both<0, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
       else if (list == 1) pair_both<1, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
       else if (list == 2) pair_both<2, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
       else pair_both<3, RULE>(d, a.s, b.s, Fi, Fj, t64, pe, pg, psi, psj);
-      const double w = side == 0 ? 1.0 : xo[jl];
-      e += w * pe;
-      g0 += w * pg[0]; g1 += w * pg[1]; g2 += w * pg[2];
-      gs += w * psi;
-      const int rv = L.orv[ei];
-      if (rv >= 0) share[(size_t)rv * P.rows + jl] = make_double4(-wme * pg[0], -wme * pg[1], -wme * pg[2], wme * psj);
+      e += wj * pe;
+      g0 += wj * pg[0]; g1 += wj * pg[1]; g2 += wj * pg[2];
+      gs += wj * psi;
+      if (rv >= 0) share[(size_t)rv * prows + jl] = make_double4(-wme * pg[0], -wme * pg[1], -wme * pg[2], wme * psj);
     }
   }
   e = site_sum(e, K);
@@ -707,12 +735,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_
   g2 = site_sum(g2, K);
   gs = site_sum(gs, K);
   if (ph != 0) return;
-  double* __restrict__ G = S.G;
-  G[4 * (size_t)s + 0] = g0;
-  G[4 * (size_t)s + 1] = g1;
-  G[4 * (size_t)s + 2] = g2;
-  G[4 * (size_t)s + 3] = gs;
-  S.epair[s] = e;
+  gmem(reinterpret_cast<double4*>(S.G))[s] = make_double4(g0, g1, g2, gs);
+  gmem(S.epair)[s] = e;
 }
 
 __global__ __launch_bounds__(TB) void k_grad_recv64(const LatDev* __restrict__ lats, const int32_t* __restrict__ b2l) {
@@ -734,7 +758,7 @@ __global__ __launch_bounds__(TB) void k_grad_recv64(const LatDev* __restrict__ l
       const Ell& L = list_of(S, pd ? side : 2 + side);
       const int n = L.rcnt[row];
       // shares arrive weighted by their senders
-      for_row_shares<RECV_RU_WIDE>(reinterpret_cast<const double4*>(L.rbuf), L.rows, row, n, [&](const double4& v) {
+      for_row_shares<RECV_RU_WIDE>(gmem(reinterpret_cast<const double4*>(L.rbuf)), L.rows, row, n, [&](const double4& v) {
         g0 += v.x; g1 += v.y; g2 += v.z;
         gs += v.w;
       });

@@ -89,13 +89,29 @@ using DevSys = LatDev;
 constexpr int SITE_BLOCK = 256;  // sites per block of every site-parallel kernel
 __host__ __device__ inline int site_blocks(int n) { return (n + SITE_BLOCK - 1) / SITE_BLOCK; }
 
+// Global-memory views.  Pointers read from a descriptor in memory are
+// generic to the compiler, which then issues flat_* instructions: those count
+// against the LDS counter as well as the vector-memory one, so every wait for
+// a table lookup also waits out outstanding gathers and stores.  Loads and
+// stores through these views are global_* instructions.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define A2C_GLOBAL __attribute__((address_space(1)))
+#else
+#define A2C_GLOBAL  // the host pass of a .hip file compiles device code too, without address spaces
+#endif
+template <class T>
+__device__ __forceinline__ A2C_GLOBAL T* gmem(T* p) {
+  return (A2C_GLOBAL T*)p;
+}
+
 // The n received shares of one row of a slot-major buffer (share t at
 // base[t * rows + row]) passed to add in slot order.  The loads of RU
 // consecutive shares are issued before any of them is added, so a row costs
 // one memory latency per RU shares rather than one per share; the order of
 // the additions, and so the result, is that of a plain loop.
 template <int RU, class T, class Add>
-__device__ __forceinline__ void for_row_shares(const T* __restrict__ base, size_t rows, int row, int n, Add add) {
+__device__ __forceinline__ void for_row_shares(const A2C_GLOBAL T* __restrict__ base, size_t rows, int row, int n,
+                                               Add add) {
   for (int t = 0; t < n; t += RU) {
     T v[RU];
 #pragma unroll
@@ -148,6 +164,25 @@ __device__ inline void gh_node(int k, double z[3], double& w) {
   z[1] = GH_Z * a1;
   z[2] = GH_Z * a2;
   w = gh_w(a0) * gh_w(a1) * gh_w(a2);
+}
+
+
+// Minimum-image geometry of one lattice, copied out of its descriptor into
+// registers before a pair loop (the loop's stores could otherwise alias the
+// descriptor, and its fields would be reloaded for every pair).
+struct Image {
+  double box[3], ibox[3];
+  bool per[3];
+};
+__device__ __forceinline__ Image image_of(const LatDev& S) {
+  Image g;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    g.box[c] = S.box[c];
+    g.ibox[c] = S.ibox[c];
+    g.per[c] = S.periodic[c] != 0;
+  }
+  return g;
 }
 
 // Logical site block of a workgroup.  Workgroups are dealt to the 8 XCDs
