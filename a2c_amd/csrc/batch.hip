@@ -129,7 +129,7 @@ struct SegBound {  // sigma-bound operations: mode 0 project x, 1 gp = P g, 2 ze
 };
 
 __global__ __launch_bounds__(TB) void k_seg_axpby(const SegAx* __restrict__ ops, const int32_t* __restrict__ b2l) {
-  const SegAx& o = ops[b2l[blockIdx.x]];
+  const SegAx o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   if (s >= o.nsite) return;
   const size_t i = 4 * (size_t)s;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(TB) void k_seg_axpby(const SegAx* __restrict__ ops,
 __global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, const int32_t* __restrict__ b2l,
                                                 double* __restrict__ part, int nblk) {
   __shared__ double r1[TB], r2[TB];
-  const SegDot& o = ops[b2l[blockIdx.x]];
+  const SegDot o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   double v1 = 0.0, v2 = 0.0;
   if (s < o.nsite) {
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(TB) void k_seg_dot(const SegDot* __restrict__ ops, 
 __global__ __launch_bounds__(1024) void k_seg_dot_final(const SegDot* __restrict__ ops, const double* __restrict__ part,
                                                         int nblk) {
   __shared__ double r1[1024], r2[1024];
-  const SegDot& o = ops[blockIdx.x];
+  const SegDot o = ops[blockIdx.x];
   const int n = site_blocks(o.nsite);
   double v1 = 0.0, v2 = 0.0;
   for (int i = threadIdx.x; i < n; i += 1024) {
@@ -272,7 +272,7 @@ constexpr int TL_PART = 1 + 2 * NGROUP;  // partial sums per block of pass 0
 __global__ __launch_bounds__(TB) void k_seg_tl0(const SegTL* __restrict__ ops, const int32_t* __restrict__ b2l,
                                                 double* __restrict__ part, int nblk) {
   __shared__ double red[TB / 64];
-  const SegTL& o = ops[b2l[blockIdx.x]];
+  const SegTL o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   double a = 0.0, gs[2 * NGROUP] = {};
   if (s < o.nsite) {
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(TB) void k_seg_tl0(const SegTL* __restrict__ ops, c
 __global__ __launch_bounds__(TB) void k_seg_tl(const SegTL* __restrict__ ops, const int32_t* __restrict__ b2l,
                                                double* __restrict__ part) {
   __shared__ double red[TB / 64];
-  const SegTL& o = ops[b2l[blockIdx.x]];
+  const SegTL o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   double acc = 0.0;
   if (s < o.nsite) {
@@ -362,7 +362,7 @@ struct SegPair {
 __global__ __launch_bounds__(TB) void k_seg_pair(const SegPair* __restrict__ ops, const int32_t* __restrict__ b2l,
                                                  double* __restrict__ part, int nblk) {
   __shared__ double red[TB / 64];
-  const SegPair& o = ops[b2l[blockIdx.x]];
+  const SegPair o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   double sy = 0.0, gg = 0.0;
   if (s < o.nsite) {
@@ -397,7 +397,7 @@ template <class Op, int NOUT>
 __global__ __launch_bounds__(256) void k_seg_final(const Op* __restrict__ ops, const double* __restrict__ part,
                                                    int nblk) {
   __shared__ double red[4];
-  const Op& o = ops[blockIdx.x];
+  const Op o = ops[blockIdx.x];
   const int n = site_blocks(o.nsite);
 #pragma unroll
   for (int k = 0; k < NOUT; ++k) {
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void k_seg_final(const Op* __restrict__ ops, c
 
 template <int MODE>
 __global__ __launch_bounds__(TB) void k_seg_bound(const SegBound* __restrict__ ops, const int32_t* __restrict__ b2l) {
-  const SegBound& o = ops[b2l[blockIdx.x]];
+  const SegBound o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   if (s >= o.nsite) return;
   const size_t i = 4 * (size_t)s;
@@ -795,7 +795,7 @@ struct SegVF {
 __global__ __launch_bounds__(TB) void k_seg_vf1(const SegVF* __restrict__ ops, const int32_t* __restrict__ b2l,
                                                 double* __restrict__ part, int nblk) {
   __shared__ double red[VF_SUMS * (TB / 64)];
-  const SegVF& o = ops[b2l[blockIdx.x]];
+  const SegVF o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   const int np = o.np;
   double acc[VF_SUMS] = {};
@@ -837,7 +837,7 @@ __global__ __launch_bounds__(TB) void k_seg_vf1(const SegVF* __restrict__ ops, c
 __global__ __launch_bounds__(TB) void k_seg_vf2(const SegVF* __restrict__ ops, const int32_t* __restrict__ b2l,
                                                 double* __restrict__ part, int nblk) {
   __shared__ double red[VF_PAIRS * (TB / 64)];
-  const SegVF& o = ops[b2l[blockIdx.x]];
+  const SegVF o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   const int np = o.np;
   double al[VF_PAIRS], gam[NGROUP];
@@ -874,7 +874,7 @@ __global__ __launch_bounds__(TB) void k_seg_vf2(const SegVF* __restrict__ ops, c
 __global__ __launch_bounds__(TB) void k_seg_vf3(const SegVF* __restrict__ ops, const int32_t* __restrict__ b2l,
                                                 double* __restrict__ part) {
   __shared__ double red[TB / 64];
-  const SegVF& o = ops[b2l[blockIdx.x]];
+  const SegVF o = ops[b2l[blockIdx.x]];
   const int s = (blockIdx.x - o.blk0) * TB + threadIdx.x;
   const int np = o.np;
   double al[VF_PAIRS], c[VF_PAIRS];
@@ -908,7 +908,7 @@ template <int K, int PASS>
 __global__ __launch_bounds__(256) void k_seg_vf_final(const SegVF* __restrict__ ops, const double* __restrict__ part,
                                                       int nblk) {
   __shared__ double red[4];
-  const SegVF& o = ops[blockIdx.x];
+  const SegVF o = ops[blockIdx.x];
   const int n = site_blocks(o.nsite);
   double* out = PASS == 1 ? o.sums : (PASS == 2 ? o.sums + VF_SUMS : o.out);
 #pragma unroll

@@ -268,50 +268,59 @@ def dmd(args):
     occupancy 0.6 in the first 8 cell layers along x, xLb = 1e-3 elsewhere,
     T = 400 K, nu = 1e13 /s, Qm = 0.23 eV, dt = 5e-13 s (12 rate dt < 0.1);
     each step: relaxation (mixed, FP64 endgame) -> chemical potential ->
-    diffusion (FP64 flux).  Reports s/step and the drift of total H."""
+    diffusion (FP64 flux).  Reports s/step and the balance of total H per
+    step: the net flux of the diffusion update (zero up to the rounding of the
+    per-site sums, relative to the summed |edge terms|), the H added by the
+    xLb clamp (integrate.cpp:79-81), and the change of total H, which equals
+    the clamp lift when the flux balances."""
     import a2c_amd as A
+    from a2c_amd import workloads
     n = args.dmd_cells
-    X, x, box, npd, nh = A.lattice(n, seed=5, disp_std=0.02, sigma=(0.10, 0.10), x=(1e-3, 1e-3),
-                                   slab_cells=8, slab_x=0.6)
-    fe = A.FreeEntropy(npd, nh, box, (1, 1, 1), T=400.0, precision="mixed")
+    s = workloads.dmd(n)
+    X, x, npd, nh = s["X"], s["x"], s["n_pd"], s["n_h"]
+    fe = A.FreeEntropy(npd, nh, s["box"], s["periodic"], T=s["T"], precision="mixed")
     fe.set_occupancy(x)
     fe.findNeighbors(X)
-    # Relaxation tolerance: the per-component RMS gradient that ||g|| <= 1e-8
-    # means for the 6^3-cell lattice (6,912 DOF), i.e. gatol scaled by
-    # sqrt(DOF / 6912); an absolute 1e-8 over 8.4 M DOF sits at the FP64
-    # evaluation's own noise floor.  The mixed -> FP64 switch is the library's
-    # automatic one (2e-9 sqrt(sites)).
-    # Widths are capped at 0.5 A (relaxed widths are 0.07-0.16 A): the
-    # functional decreases without bound as one width grows past a barrier
-    # near 1 A (-3 kB x ln sigma), and over hundreds of thousands of packets
-    # an uncapped line search eventually steps across it.
-    scale = np.sqrt(4.0 * (npd + nh) / 6912.0)
-    prm = A.dmd_params(dt=5e-13, nu=1e13, Qm=0.23, xLb=1e-3, xUb=1 - 1e-16, gamma_bd=1e300,
-                       lmvm={"gatol": 1e-8 * scale, "grtol": 0.0, "sigma_ub": 0.5})
+    lmvm = workloads.dmd_lmvm(npd + nh)
+    prm = A.dmd_params(**workloads.DMD_STEP, lmvm=lmvm)
     g = np.zeros(nh)
     fh = np.zeros(nh, np.int32)
-    h0 = x.sum()
+    h0 = float(np.sum(x))
     reps = []
     t0 = time.perf_counter()
     for st in range(args.dmd_steps):
         reps.append(A.dmd_step(fe, prm, X, x, g, fh))
+        r = reps[-1]
         sg = X.reshape(-1, 4)[:, 3]
-        log("dmd step %d: f=%.12g it=%d reason=%d dH=%.3e sigma %.4f..%.4f" % (
-            st, reps[-1]["f"], reps[-1]["lmvm_iterations"], reps[-1]["lmvm_reason"],
-            reps[-1]["total_h_after"] - reps[-1]["total_h_before"], sg.min(), sg.max()))
+        log("dmd step %d: f=%.12g it=%d reason=%d dH=%.6e lift=%.6e flux=%.2e/%.2e sigma %.4f..%.4f" % (
+            st, r["f"], r["lmvm_iterations"], r["lmvm_reason"], r["total_h_after"] - r["total_h_before"],
+            r["clamp_lift"], r["flux_sum"], r["flux_gross"], sg.min(), sg.max()))
     wall = time.perf_counter() - t0
     fe.close()
-    drift = max(abs(r["total_h_after"] - r["total_h_before"]) for r in reps)
+    dH = [r["total_h_after"] - r["total_h_before"] for r in reps]
+    flux_rel = [abs(r["flux_sum"] - r["reservoir_supply"]) / r["flux_gross"] for r in reps if r["flux_gross"] > 0]
+    resid = [abs(d - r["clamp_lift"] - r["reservoir_supply"]) for d, r in zip(dH, reps)]
     return {
         "metric": "s/step", "value": wall / args.dmd_steps, "unit": "s/step", "n_gpus": 1,
         "steps": args.dmd_steps, "warmup": 0, "ms_per_step": wall * 1e3 / args.dmd_steps,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32-bulk/f64-reduce",
         "data": "synthetic",
-        "config": {"workload": "coupled DMD, %d^3 cells, slab x=0.6 in 8 layers, T=400K" % n, "sites": npd + nh},
-        "max_step_abs_dH": drift, "total_H_start": h0, "total_H_end": float(x.sum()),
-        "dH_note": "total H changes only where the xLb clamp (integrate.cpp:79-81) lifts sites that lose H",
-        "gatol": 1e-8 * scale, "sigma_ub": 0.5,
+        "config": {"workload": "coupled DMD, %d^3 cells, slab x=0.6 in 8 layers, x=1e-3 elsewhere, T=400K, "
+                               "dt=5e-13 s, nu=1e13/s, Qm=0.23 eV" % n, "sites": npd + nh},
+        "total_H_start": h0, "total_H_end": float(np.sum(x)),
+        "balance": {
+            "flux_rel_max": max(flux_rel) if flux_rel else None,
+            "flux_note": "|sum of applied increments - reservoir supply| / sum of |edge terms| per step "
+                         "(exact antisymmetry of the edge terms: zero up to the rounding of the site sums)",
+            "clamp_lift_per_step": [r["clamp_lift"] for r in reps],
+            "dH_per_step": dH,
+            "dH_minus_lift_max": max(resid),
+            "dH_note": "total H (compensated sums) changes by the H the xLb clamp adds to sites that lose H; "
+                       "no reservoirs in this workload",
+        },
+        "gatol": lmvm["gatol"], "sigma_ub": lmvm["sigma_ub"],
         "lmvm_iterations": [r["lmvm_iterations"] for r in reps],
+        "lmvm_reasons": sorted(set(r["lmvm_reason"] for r in reps)),
     }
 
 
