@@ -158,6 +158,7 @@ def _load():
         "a2c_fe_set_precision": (I, [VP, C.c_int32]),
         "a2c_fe_set_occupancy": (I, [VP, dp]),
         "a2c_fe_build_neighbors": (I, [VP, dp, P(NbrStats)]),
+        "a2c_fe_build_neighbors_device": (I, [VP, VP, P(NbrStats)]),
         "a2c_fe_get_neighbors": (I, [VP, C.c_int32, ip, ip]),
         "a2c_fe_get_pdpdnum": (I, [VP, ip]),
         "a2c_fe_density": (I, [VP, dp, dp]),
@@ -356,6 +357,12 @@ class FreeEntropy:
         return {"max_neib": st.max_neib, "width": list(st.width), "total": list(st.total)}
 
     build_neighbors = findNeighbors
+
+    def findNeighbors_device(self, dX):
+        """Neighbor build from device-resident positions (a device pointer)."""
+        st = NbrStats()
+        self._check(lib.a2c_fe_build_neighbors_device(self._ctx, C.c_void_p(dX), C.byref(st)))
+        return {"max_neib": st.max_neib, "width": list(st.width), "total": list(st.total)}
 
     def neighbors(self, which):
         """CSR (offsets, indices) of one list: 'PdPd', 'PdH', 'HPd', 'HH' or 'HH1'."""

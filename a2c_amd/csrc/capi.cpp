@@ -173,15 +173,17 @@ int a2c_fe_build_neighbors(a2c_fe_ctx* c, const double* X, a2c_nbr_stats* stats)
   if (int rc = upload_state(c, X)) return rc;
   c->lat_dirty = true;
   if (int rc = nbr_build(c)) return rc;
-  // totals for the export
-  for (int w = 0; w < 5; ++w) {
-    std::vector<int32_t> cnt(c->ell[w].rows);
-    if (c->ell[w].rows)
-      A2C_HIP(c, hipMemcpy(cnt.data(), c->ell[w].cnt, 4 * cnt.size(), hipMemcpyDeviceToHost));
-    int64_t tot = 0;
-    for (int32_t v : cnt) tot += v;
-    c->stats.total[w] = tot;
-  }
+  if (stats) *stats = c->stats;
+  return A2C_OK;
+}
+
+int a2c_fe_build_neighbors_device(a2c_fe_ctx* c, const double* dX, a2c_nbr_stats* stats) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!dX && c->N > 0) return set_err(c, A2C_EINVAL, "dX is null");
+  if (c->N > 0 && dX != c->dX)
+    A2C_HIP(c, hipMemcpyAsync(c->dX, dX, 32 * (size_t)c->N, hipMemcpyDeviceToDevice, c->stream));
+  c->lat_dirty = true;
+  if (int rc = nbr_build(c)) return rc;
   if (stats) *stats = c->stats;
   return A2C_OK;
 }

@@ -229,11 +229,11 @@ struct a2c_fe_ctx {
   // neighbor lists
   a2c::Ell ell[5];           // PdPd, PdH, HPd, HH, HH1
   // neighbor-build scratch (grow-only)
-  int* nb_cid = nullptr;     // N
-  int* nb_order = nullptr;   // N
-  int* nb_cc = nullptr;      // cells + 1
-  int* nb_cursor = nullptr;  // cells + 1
-  int* nb_misc = nullptr;    // 8
+  int* nb_cid = nullptr;     // N: cell of each site
+  void* nb_order = nullptr;  // N double4: positions in cell order, site index in .w
+  int* nb_cc = nullptr;      // cells + 1: cell starts
+  int* nb_cursor = nullptr;  // cells + 1: histogram, then placement cursors
+  int* nb_misc = nullptr;    // 256 bytes: widths, overflow flag, list totals, bounding box
   int64_t nb_cap_cells = 0;
   int32_t* dpdpdnum = nullptr;  // nPd*3
   a2c_nbr_stats stats{};
@@ -294,7 +294,6 @@ int set_err(a2c_fe_ctx* c, int code, const std::string& msg);
 
 // kernels (launchers) ---------------------------------------------------------
 int nbr_build(a2c_fe_ctx* c);                           // nbr.hip
-int ensure_pair_buffers(a2c_fe_ctx* c);                 // owned/received split + contribution buffers
 void free_pair_buffers(Ell& L);                         // nbr.hip
 // Multi-lattice launches: lats/b2l on the device, nblk = total site blocks,
 // nlat = lattices.  The FP64 path fills rho, Fp, emb, G, part, f; the mixed

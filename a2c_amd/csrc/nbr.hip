@@ -8,16 +8,26 @@
 // 0.75a^2 < d2 < 1.25a^2, else (minimizer.cpp:935-947); HH1 = H neighbors with
 // d2 < 0.75a^2 (:999-1004).
 //
-// Kernels: k_bin (cell id + histogram), host scan, k_fill (scatter into cell
-// order), k_nbr<false> (row lengths), k_width (ELL widths), k_nbr<true>
-// (sorted ELL rows).  One 64-lane wave per site: lanes test consecutive members of a cell,
-// a ballot compacts hits into LDS, and each hit's slot is its rank among the
-// hits (an O(n) count per lane over LDS broadcasts), so rows come out sorted
-// whatever order the atomics binned the sites in.
+// Passes, all on the device (two host synchronisations: the list widths
+// after the count pass, the totals at the end):
+//   k_bin     cell of each site and the cell histogram
+//   k_scan    exclusive scan of the histogram (one workgroup)
+//   k_place   positions copied into cell order (with the site index), so a
+//             candidate is one load; three x-adjacent cells of a stencil row
+//             are one contiguous range
+//   k_site<false>  one lane per site: the candidates of its 27 cells (four
+//             loads in flight), row lengths, the owned/received split of each
+//             pair list, and their maxima
+//   k_site<true>   the same walk keeps each hit (neighbor index and shell
+//             bits) in the lane's LDS column; the wave sorts each row by
+//             ranking, then writes the rows slot by slot, 64 consecutive
+//             rows per store
+// Rows come out sorted whatever order the cell placement used.
 #pragma clang fp contract(off)
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "fe_internal.hpp"
@@ -48,200 +58,301 @@ __global__ void k_bin(const double* __restrict__ X, int N, Grid g, int* __restri
   atomicAdd(&ccount[c], 1);
 }
 
-__global__ void k_fill(int N, const int* __restrict__ cid, int* __restrict__ cursor, int* __restrict__ order) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+// start[i] = sum of cnt[0..i), start[n] = the total; cursor = start[0..n).
+// One workgroup: each thread scans a contiguous chunk, the chunk sums are
+// scanned across the workgroup.
+constexpr int SCAN_T = 1024;
+__global__ __launch_bounds__(SCAN_T) void k_scan(const int* cnt, int n, int* __restrict__ start, int* cursor) {
+  __shared__ int sm[SCAN_T];
+  const int per = (n + SCAN_T - 1) / SCAN_T;
+  const int b = min(n, (int)threadIdx.x * per), e = min(n, b + per);
+  int acc = 0;
+  for (int i = b; i < e; ++i) acc += cnt[i];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 1; off < SCAN_T; off <<= 1) {  // inclusive Hillis-Steele scan of the chunk sums
+    const int v = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
+    __syncthreads();
+    sm[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sm[threadIdx.x] - acc;
+  for (int i = b; i < e; ++i) {  // cnt and cursor may be one array: read before writing
+    const int ci = cnt[i];
+    start[i] = run;
+    cursor[i] = run;
+    run += ci;
+  }
+  if (threadIdx.x == SCAN_T - 1) start[n] = sm[SCAN_T - 1];
+}
+
+// positions in cell order, the site index in .w (exact in a double)
+__global__ void k_place(const double* __restrict__ X, int N, const int* __restrict__ cid, int* __restrict__ cursor,
+                        double4* __restrict__ qs) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  int pos = atomicAdd(&cursor[cid[s]], 1);
-  order[pos] = s;
+  const int pos = atomicAdd(&cursor[cid[s]], 1);
+  qs[pos] = make_double4(X[4 * s], X[4 * s + 1], X[4 * s + 2], (double)s);
 }
 
 struct NbrArgs {
   const double* X;
+  const double4* qs;   // cell-ordered positions
   int npd, N;
   Grid g;
+  double ibox[3];
   const int* cid;
   const int* cstart;
-  const int* order;
   double rc2, sh1, sh2;
   int cap;
-  int32_t* cnt[5];    // row lengths: PdPd, PdH (Pd rows); HPd, HH, HH1 (H rows)
-  int32_t* idx[5];    // ELL arrays (fill pass)
-  int32_t width[5];   // ELL widths (fill pass)
-  int32_t* pdpdnum;   // nPd*3 (fill pass)
-  int32_t* maxw;      // 6 ints: max row length per list, max total (count pass)
-  int32_t* overflow;  // 1 if a site exceeds cap
+  int width;           // fill pass: LDS slots per lane (the longest row)
+  int32_t* cnt[5];     // row lengths: PdPd, PdH (Pd rows); HPd, HH, HH1 (H rows)
+  int32_t* ocnt[4];    // owned entries of the four pair lists
+  int32_t* rcnt[4];    // received entries
+  int32_t* idx[5];     // ELL arrays (fill pass)
+  int32_t* pdpdnum;    // nPd*3 (fill pass)
+  int32_t* maxw;       // 14 ints: width of the 5 lists, owned widths of the 4 pair lists, received
+                       // widths, max total (count pass)
+  int32_t* overflow;   // 1 if a site exceeds cap
+  unsigned long long* totals;  // entries of the 5 lists (fill pass)
 };
 
-template <bool FILL>
-__global__ __launch_bounds__(64) void k_nbr(NbrArgs a) {
-  __shared__ int32_t sj[NBR_CAP];
-  __shared__ double sd[NBR_CAP];
-  const int s = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (s >= a.N) return;  // uniform over the block (one site per block)
-  const double qi0 = a.X[4 * s], qi1 = a.X[4 * s + 1], qi2 = a.X[4 * s + 2];
-  const int c = a.cid[s];
-  const int ci[3] = {c % a.g.nc[0], (c / a.g.nc[0]) % a.g.nc[1], c / (a.g.nc[0] * a.g.nc[1])};
-  int lst[3][3], cnt[3];
-  for (int ax = 0; ax < 3; ++ax) {  // distinct stencil cells per axis
-    cnt[ax] = 0;
-    for (int o = -1; o <= 1; ++o) {
-      int k = ci[ax] + o;
-      if (a.g.periodic[ax]) k = (k + a.g.nc[ax]) % a.g.nc[ax];
-      else if (k < 0 || k >= a.g.nc[ax]) continue;
-      bool dup = false;
-      for (int t = 0; t < cnt[ax]; ++t) dup |= (lst[ax][t] == k);
-      if (!dup) lst[ax][cnt[ax]++] = k;
-    }
+// Ownership of the unordered pair {s, j} (global site indices) in the
+// pair-once evaluation: (s < j) XOR ((s ^ j) & 1), which hands each site
+// about half of every list.
+__device__ __forceinline__ bool pair_owner(int s, int j) { return (s < j) != (((s ^ j) & 1) != 0); }
+
+// d - L rint(d / L) on a periodic axis.  The quotient is formed as d (1/L):
+// the two agree except within an ulp of a half-integer, where the division
+// itself decides (a candidate about L/2 away, so the chosen image and the set
+// stay exactly those of the division).
+__device__ __forceinline__ double wrap(double d, double L, double iL) {
+  const double q = d * iL;
+  const double f = fabs(q - trunc(q));
+  return d - L * rint(fabs(f - 0.5) < 1e-9 ? d / L : q);
+}
+
+// Candidates of one stencil row (dy, dz) of cell (cx, cy, cz): the x cells
+// cx-1..cx+1 are consecutive cell indices, one range [b0, e0), unless the
+// row wraps a periodic x boundary (a second range [b1, e1)).  Axes with one
+// or two cells contribute each cell once.
+__device__ __forceinline__ void row_ranges(const NbrArgs& a, int cx, int yy, int zz, int& b0, int& e0, int& b1,
+                                           int& e1) {
+  const int n0 = a.g.nc[0];
+  const int base = (zz * a.g.nc[1] + yy) * n0;
+  int x0, x1, y0 = 0, y1 = -1;  // [x0, x1], then [y0, y1] (inclusive cell ranges)
+  if (n0 < 3) {
+    x0 = 0; x1 = n0 - 1;
+  } else if (a.g.periodic[0] && cx == 0) {
+    x0 = 0; x1 = 1; y0 = n0 - 1; y1 = n0 - 1;
+  } else if (a.g.periodic[0] && cx == n0 - 1) {
+    x0 = n0 - 2; x1 = n0 - 1; y0 = 0; y1 = 0;
+  } else {
+    x0 = max(cx - 1, 0); x1 = min(cx + 1, n0 - 1);
   }
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int n = 0, npn = 0, nh1 = 0;
-  for (int zz = 0; zz < cnt[2]; ++zz)
-    for (int yy = 0; yy < cnt[1]; ++yy)
-      for (int xx = 0; xx < cnt[0]; ++xx) {
-        const int cc = (lst[2][zz] * a.g.nc[1] + lst[1][yy]) * a.g.nc[0] + lst[0][xx];
-        const int beg = a.cstart[cc], end = a.cstart[cc + 1];
-        for (int base = beg; base < end; base += 64) {
-          const int t = base + lane;
-          bool ok = false;
-          int j = -1;
-          double d2 = 0.0;
-          if (t < end) {
-            j = a.order[t];
-            double d0 = a.X[4 * j] - qi0, d1 = a.X[4 * j + 1] - qi1, dz = a.X[4 * j + 2] - qi2;
-            if (a.g.periodic[0]) d0 = d0 - a.g.box[0] * rint(d0 / a.g.box[0]);
-            if (a.g.periodic[1]) d1 = d1 - a.g.box[1] * rint(d1 / a.g.box[1]);
-            if (a.g.periodic[2]) dz = dz - a.g.box[2] * rint(dz / a.g.box[2]);
-            d2 = d0 * d0 + d1 * d1 + dz * dz;
-            ok = (j != s) && (d2 <= a.rc2);
-          }
-          const unsigned long long m = __ballot(ok);
-          if (FILL && ok) {
-            const int pos = n + __popcll(m & lt);
-            if (pos < NBR_CAP) { sj[pos] = j; sd[pos] = d2; }
-          }
-          n += __popcll(m);
-          npn += __popcll(__ballot(ok && j < a.npd));
-          nh1 += __popcll(__ballot(ok && j >= a.npd && d2 < a.sh1));
-        }
-      }
-  const bool pd = s < a.npd;
-  const int row = pd ? s : s - a.npd;
-  if (n > a.cap) {
-    if (lane == 0) atomicOr(a.overflow, 1);
-    return;
-  }
-  const int nhn = n - npn;
-  if (!FILL) {
-    if (lane == 0) {  // list widths: k_width (one atomic per block, not per site)
-      if (pd) {
-        a.cnt[0][row] = npn; a.cnt[1][row] = nhn;
-      } else {
-        a.cnt[2][row] = npn; a.cnt[3][row] = nhn; a.cnt[4][row] = nh1;
-      }
-    }
-    return;
-  }
-  __syncthreads();  // LDS hits visible to every lane of the (single-wave) block
-  const int rowsP = pd ? a.npd : a.N - a.npd;  // rows of this site's lists
-  int32_t* outP = pd ? a.idx[0] : a.idx[2];
-  int32_t* outH = pd ? a.idx[1] : a.idx[3];
-  int sh0 = 0, sh1n = 0, sh2n = 0;
-  for (int e0 = 0; e0 < n; e0 += 64) {
-    const int e = e0 + lane;
-    bool isp = false, s0 = false, s1 = false;
-    if (e < n) {
-      const int key = sj[e];
-      const double d2 = sd[e];
-      isp = key < a.npd;
-      int rank = 0, rank1 = 0;
-      for (int m = 0; m < n; ++m) {
-        const int o = sj[m];
-        const bool less = o < key;
-        rank += less;
-        rank1 += less && (o >= a.npd) && (sd[m] < a.sh1);
-      }
-      if (isp) {
-        outP[(size_t)rank * rowsP + row] = key;  // Pd neighbors occupy ranks 0..npn-1
-        s0 = d2 < a.sh1;
-        s1 = d2 > a.sh1 && d2 < a.sh2;
-      } else {
-        const int h = key - a.npd;
-        outH[(size_t)(rank - npn) * rowsP + row] = h;
-        if (!pd && d2 < a.sh1) a.idx[4][(size_t)rank1 * rowsP + row] = h;
-      }
-    }
-    sh0 += __popcll(__ballot(isp && s0));
-    sh1n += __popcll(__ballot(isp && s1));
-    sh2n += __popcll(__ballot(isp && !s0 && !s1));
-  }
-  if (pd && lane == 0) {
-    a.pdpdnum[3 * row] = sh0;
-    a.pdpdnum[3 * row + 1] = sh1n;
-    a.pdpdnum[3 * row + 2] = sh2n;
+  b0 = a.cstart[base + x0];
+  e0 = a.cstart[base + x1 + 1];
+  if (y1 >= y0) {
+    b1 = a.cstart[base + y0];
+    e1 = a.cstart[base + y1 + 1];
+  } else {
+    b1 = e1 = 0;
   }
 }
 
-// Reverse slots: for entry (row r, slot t) of list L with neighbor j, the slot
-// of r in row j of the partner list P (PdPd<->PdPd, PdH<->HPd, HH<->HH).
-// Rows are sorted, so a binary search finds it; lists are symmetric.
-// Widest row of each list and the largest total count over all sites:
-// block-local maxima, then one atomicMax per block and quantity.
-__global__ __launch_bounds__(256) void k_width(NbrArgs a) {
-  __shared__ int32_t red[6][256];
-  int m[6] = {0, 0, 0, 0, 0, 0};
-  for (int s = blockIdx.x * 256 + threadIdx.x; s < a.N; s += gridDim.x * 256) {
-    if (s < a.npd) {
-      const int p = a.cnt[0][s], h = a.cnt[1][s];
-      m[0] = max(m[0], p); m[1] = max(m[1], h); m[5] = max(m[5], p + h);
-    } else {
-      const int r = s - a.npd;
-      const int p = a.cnt[2][r], h = a.cnt[3][r], h1 = a.cnt[4][r];
-      m[2] = max(m[2], p); m[3] = max(m[3], h); m[4] = max(m[4], h1); m[5] = max(m[5], p + h);
+// stencil coordinate k in 0..2 of axis ax around c: the wrapped (periodic)
+// or clipped (-1: outside) cell, each distinct cell once
+__device__ __forceinline__ int stencil_axis(const NbrArgs& a, int ax, int c, int k) {
+  const int n = a.g.nc[ax];
+  if (a.g.periodic[ax]) {
+    if (n == 1) return k == 1 ? 0 : -1;  // periodic axes have 1 or >= 3 cells
+    return (c + k - 1 + n) % n;
+  }
+  const int v = c + k - 1;
+  return (v >= 0 && v < n) ? v : -1;
+}
+
+constexpr int NT = 64;  // sites (rows) per workgroup of the site passes: one lane each
+constexpr int CS = NT + 1;  // LDS: slot t of lane l at t * CS + l (a column and a slot row are both
+                            // conflict-free)
+constexpr int MAXK = NBR_CAP / NT;  // keys of one row per lane in the ranking sort
+
+// Sort each row of the wave (lane l's row in LDS column l) by key, one row at
+// a time: every lane ranks its (at most K) keys of the row against all of the
+// row's keys, read by broadcast, then writes them to their ranks (keys are
+// distinct, so the ranks are a permutation).
+template <int K>
+__device__ __forceinline__ void rank_rows(uint32_t* buf, int n, int lane) {
+  for (int rl = 0; rl < NT; ++rl) {
+    const int nr = __shfl(n, rl);
+    if (nr < 2) continue;
+    uint32_t mine[K];
+    int rk[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int t = lane + NT * q;
+      mine[q] = t < nr ? buf[t * CS + rl] : 0u;
+      rk[q] = 0;
     }
-  }
+    for (int t = 0; t < nr; ++t) {
+      const uint32_t o = buf[t * CS + rl];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = m[k];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
+      for (int q = 0; q < K; ++q) rk[q] += o < mine[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = max(red[k][threadIdx.x], red[k][threadIdx.x + w]);
-    __syncthreads();
+    for (int q = 0; q < K; ++q)
+      if (lane + NT * q < nr) buf[rk[q] * CS + rl] = mine[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if (threadIdx.x < 6) atomicMax(&a.maxw[threadIdx.x], red[threadIdx.x][0]);
+}
+
+// One lane per site (consecutive rows of one species per workgroup; Pd-row
+// workgroups first).  Each lane walks the candidates of its 27 cells from
+// the cell-ordered copy, four loads in flight.  Count pass (FILL = false):
+// row lengths, owned/received counts and their maxima.  Fill pass: each hit's
+// key (j << 2 | shell bits) goes into the lane's LDS column
+// (slot t of lane l at t * CS + l) in the order found, the wave ranks each
+// row's keys (the rows sorted by neighbor index), then the rows are written
+// slot by slot: 64 consecutive rows per store, one 256-byte line.
+template <bool FILL>
+__global__ __launch_bounds__(NT) void k_site(NbrArgs a, int nblk_pd) {
+  extern __shared__ uint32_t buf[];
+  const int lane = threadIdx.x;
+  const bool pdb = (int)blockIdx.x < nblk_pd;
+  const int nrows = pdb ? a.npd : a.N - a.npd;
+  const int row = (pdb ? (int)blockIdx.x : (int)blockIdx.x - nblk_pd) * NT + lane;
+  const bool live = row < nrows;
+  const int s = (pdb ? 0 : a.npd) + (live ? row : 0);
+  const A2C_GLOBAL double4* __restrict__ qs = gmem(a.qs);
+  int n = 0, npn = 0, nh1 = 0, op = 0, oh = 0, sh0 = 0, sh1n = 0;
+  if (live) {
+    const double qi0 = a.X[4 * (size_t)s], qi1 = a.X[4 * (size_t)s + 1], qi2 = a.X[4 * (size_t)s + 2];
+    const int c = a.cid[s];
+    const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
+    for (int kz = 0; kz < 3; ++kz) {
+      const int zz = stencil_axis(a, 2, cz, kz);
+      if (zz < 0) continue;
+      for (int ky = 0; ky < 3; ++ky) {
+        const int yy = stencil_axis(a, 1, cy, ky);
+        if (yy < 0) continue;
+        int b0, e0, b1, e1;
+        row_ranges(a, cx, yy, zz, b0, e0, b1, e1);
+        const int l0 = e0 - b0, tot = l0 + (e1 - b1);
+        for (int v0 = 0; v0 < tot; v0 += 4) {
+          double4 cq[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int v = v0 + u;
+            cq[u] = make_double4(0.0, 0.0, 0.0, -1.0);
+            if (v < tot) cq[u] = qs[v < l0 ? b0 + v : b1 + (v - l0)];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = (int)cq[u].w;
+            double d0 = cq[u].x - qi0, d1 = cq[u].y - qi1, dz = cq[u].z - qi2;
+            if (a.g.periodic[0]) d0 = wrap(d0, a.g.box[0], a.ibox[0]);
+            if (a.g.periodic[1]) d1 = wrap(d1, a.g.box[1], a.ibox[1]);
+            if (a.g.periodic[2]) dz = wrap(dz, a.g.box[2], a.ibox[2]);
+            const double d2 = d0 * d0 + d1 * d1 + dz * dz;
+            if (!(j >= 0 && j != s && d2 <= a.rc2)) continue;
+            const bool jpd = j < a.npd;
+            ++n;
+            if (jpd) ++npn;
+            if (!FILL) {
+              const bool own = pair_owner(s, j);
+              if (!jpd && d2 < a.sh1) ++nh1;
+              if (own) (jpd ? op : oh)++;
+            } else if (n <= a.width) {
+              if (jpd) {
+                if (d2 < a.sh1) ++sh0;
+                else if (d2 > a.sh1 && d2 < a.sh2) ++sh1n;
+              }
+              buf[(n - 1) * CS + lane] = ((uint32_t)j << 2) |
+                                         (jpd ? (d2 < a.sh1 ? 1u : ((d2 > a.sh1 && d2 < a.sh2) ? 2u : 0u))
+                                              : (d2 < a.sh1 ? 1u : 0u));
+            }
+          }
+        }
+      }
+    }
+    if (n > a.cap) atomicOr(a.overflow, 1);
+  }
+  const int nhn = n - npn;
+  const int lp = pdb ? 0 : 2, lh = pdb ? 1 : 3;
+  if (!FILL) {
+    if (live) {
+      a.cnt[lp][row] = npn;
+      a.cnt[lh][row] = nhn;
+      a.ocnt[lp][row] = op;
+      a.rcnt[lp][row] = npn - op;
+      a.ocnt[lh][row] = oh;
+      a.rcnt[lh][row] = nhn - oh;
+      if (!pdb) a.cnt[4][row] = nh1;
+    }
+    // maxima over the wave, then one atomic per quantity
+    int v[14] = {0};
+    v[lp] = npn; v[lh] = nhn; v[4] = pdb ? 0 : nh1;
+    v[5 + lp] = op; v[5 + lh] = oh; v[9 + lp] = npn - op; v[9 + lh] = nhn - oh; v[13] = n;
+#pragma unroll
+    for (int k = 0; k < 14; ++k) {
+      int m = v[k];
+      for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+      v[k] = m;
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 14; ++k)
+        if (v[k]) atomicMax(&a.maxw[k], v[k]);
+    return;
+  }
+  // each row of the wave sorted by neighbor index
+  if (a.width <= 2 * NT) rank_rows<2>(buf, n, lane);
+  else rank_rows<MAXK>(buf, n, lane);
+  if (live && pdb) {
+    a.pdpdnum[3 * (size_t)row] = sh0;
+    a.pdpdnum[3 * (size_t)row + 1] = sh1n;
+    a.pdpdnum[3 * (size_t)row + 2] = npn - sh0 - sh1n;
+  }
+  int mp = npn, mh = nhn;  // the wave's longest rows
+  for (int off = 32; off > 0; off >>= 1) {
+    mp = max(mp, __shfl_xor(mp, off));
+    mh = max(mh, __shfl_xor(mh, off));
+  }
+  int32_t* __restrict__ oP = a.idx[lp];
+  int32_t* __restrict__ oH = a.idx[lh];
+  for (int t = 0; t < mp; ++t)
+    if (t < npn) oP[(size_t)t * nrows + row] = (int)(buf[t * CS + lane] >> 2);
+  int k1 = 0;
+  for (int t = 0; t < mh; ++t)
+    if (t < nhn) {
+      const uint32_t key = buf[(npn + t) * CS + lane];
+      const int h = (int)(key >> 2) - a.npd;
+      oH[(size_t)t * nrows + row] = h;
+      if (!pdb && (key & 1u)) a.idx[4][(size_t)(k1++) * nrows + row] = h;  // HH1, in order
+    }
+  // totals of the five lists: wave sums, one atomic each
+  unsigned long long tp = npn, th = nhn, t1 = k1;
+  for (int off = 32; off > 0; off >>= 1) {
+    tp += __shfl_xor(tp, off);
+    th += __shfl_xor(th, off);
+    t1 += __shfl_xor(t1, off);
+  }
+  if (lane == 0) {
+    atomicAdd(&a.totals[lp], tp);
+    atomicAdd(&a.totals[lh], th);
+    if (!pdb) atomicAdd(&a.totals[4], t1);
+  }
 }
 
 // Split of each pair-list row into owned and received halves (pair-once
 // evaluation).  Ownership of {s, j} (global site indices):
 // (s < j) XOR ((s ^ j) & 1), which hands each site about half of every list.
 // Both halves keep the row's (sorted) order.
-__device__ __forceinline__ bool pair_owner(int s, int j) { return (s < j) != (((s ^ j) & 1) != 0); }
-
-__global__ void k_own_count(Ell L, int s_off, int j_off) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= L.rows) return;
-  const int n = L.cnt[r];
-  const int s = s_off + r;
-  int no = 0;
-  for (int t = 0; t < n; ++t) no += pair_owner(s, j_off + L.idx[(size_t)t * L.rows + r]);
-  L.ocnt[r] = no;
-  L.rcnt[r] = n - no;
-}
-
-// max over rows of a[r], one atomicMax per block
-__global__ __launch_bounds__(256) void k_rowmax(const int32_t* __restrict__ a, int n, int32_t* out) {
-  __shared__ int32_t red[256];
-  int m = 0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = max(m, a[i]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicMax(out, red[0]);
-}
 
 // Both halves keep the row's index order: consecutive lanes (sites) then
 // gather consecutive neighbor sites at the same slot, so the state and F'
@@ -292,23 +403,89 @@ __global__ void k_orv(Ell L, Ell P) {
   }
 }
 
+// Bounding box of the positions (non-periodic axes): per block, then one
+// atomic per block and quantity on the ordered-integer image of the double.
+__device__ __forceinline__ unsigned long long dkey(double x) {  // monotone map double -> uint64
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ X, int N, unsigned long long* out) {
+  __shared__ unsigned long long red[6][256];
+  unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < N; s += gridDim.x * 256)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const unsigned long long k = dkey(X[4 * (size_t)s + c]);
+      lo[c] = min(lo[c], k);
+      hi[c] = max(hi[c], k);
+    }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    red[c][threadIdx.x] = lo[c];
+    red[3 + c][threadIdx.x] = hi[c];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        red[c][threadIdx.x] = min(red[c][threadIdx.x], red[c][threadIdx.x + w]);
+        red[3 + c][threadIdx.x] = max(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) atomicMin(&out[threadIdx.x], red[threadIdx.x][0]);
+  else if (threadIdx.x < 6) atomicMax(&out[threadIdx.x], red[threadIdx.x][0]);
+}
+double dkey_inv(unsigned long long k) {
+  const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  double x;
+  std::memcpy(&x, &u, 8);
+  return x;
+}
+
 }  // namespace
 
-// Host driver: grid from the positions, bin, scan, count, allocate, fill.
+// Host driver: grid, bin, scan, place, count, allocate, fill, split.  The
+// positions are the context state c->dX (on the device).
 int nbr_build(a2c_fe_ctx* c) {
   const int N = c->N, npd = c->desc.n_pd, nh = c->desc.n_h;
   const double rc = c->desc.rc;
+  const int TB = 256;
+  // scratch (grow-only): misc = 16 ints (widths, overflow) + 5 totals + 6 bbox keys
+  if (!c->nb_cid) {
+    A2C_HIP(c, dmalloc(&c->nb_cid, sizeof(int) * std::max(N, 1)));
+    A2C_HIP(c, dmalloc(&c->nb_order, sizeof(double4) * std::max(N, 1)));  // cell-ordered positions
+    A2C_HIP(c, dmalloc(&c->nb_misc, 256));
+  }
+  int* misc = c->nb_misc;
+  unsigned long long* totals = reinterpret_cast<unsigned long long*>(misc + 16);
+  unsigned long long* bbox = totals + 5;
   Grid g;
+  bool need_bbox = false;
+  for (int ax = 0; ax < 3; ++ax) need_bbox |= !c->desc.periodic[ax];
+  double lo3[3] = {0, 0, 0}, hi3[3] = {1, 1, 1};
+  if (need_bbox && N > 0) {
+    unsigned long long init[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+    A2C_HIP(c, hipMemcpyAsync(bbox, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_bbox, dim3(std::min((N + TB - 1) / TB, 1024)), dim3(TB), 0, c->stream, c->dX, N, bbox);
+    A2C_HIP(c, hipGetLastError());
+    unsigned long long hb[6];
+    A2C_HIP(c, hipMemcpyAsync(hb, bbox, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+    A2C_HIP(c, hipStreamSynchronize(c->stream));
+    for (int ax = 0; ax < 3; ++ax) {
+      lo3[ax] = dkey_inv(hb[ax]);
+      hi3[ax] = dkey_inv(hb[3 + ax]);
+    }
+  }
   for (int ax = 0; ax < 3; ++ax) {
     g.periodic[ax] = c->desc.periodic[ax];
     g.box[ax] = c->desc.box[ax];
-    double lo, hi;
+    double lo = 0.0, hi = 1.0;
     if (g.periodic[ax]) {
       lo = 0.0; hi = g.box[ax];
-    } else {
-      lo = 1e300; hi = -1e300;
-      for (int s = 0; s < N; ++s) { lo = std::min(lo, c->hX[4 * s + ax]); hi = std::max(hi, c->hX[4 * s + ax]); }
-      if (N == 0) { lo = 0.0; hi = 1.0; }
+    } else if (N > 0) {
+      lo = lo3[ax]; hi = hi3[ax];
     }
     int nc = std::max(1, (int)std::floor((hi - lo) / rc));
     if (g.periodic[ax] && nc < 3) nc = 1;
@@ -319,12 +496,6 @@ int nbr_build(a2c_fe_ctx* c) {
   }
   const long long ncell = (long long)g.nc[0] * g.nc[1] * g.nc[2];
   if (ncell > (1LL << 28)) return set_err(c, A2C_EINVAL, "neighbor grid too large");
-  // scratch buffers persist across rebuilds (grow-only)
-  if (!c->nb_cid) {
-    A2C_HIP(c, dmalloc(&c->nb_cid, sizeof(int) * std::max(N, 1)));
-    A2C_HIP(c, dmalloc(&c->nb_order, sizeof(int) * std::max(N, 1)));
-    A2C_HIP(c, dmalloc(&c->nb_misc, sizeof(int) * 8));
-  }
   if (ncell + 1 > c->nb_cap_cells) {
     (void)dfree(c->nb_cc);
     (void)dfree(c->nb_cursor);
@@ -333,119 +504,70 @@ int nbr_build(a2c_fe_ctx* c) {
     A2C_HIP(c, dmalloc(&c->nb_cursor, sizeof(int) * (ncell + 1)));
     c->nb_cap_cells = ncell + 1;
   }
-  int *dcid = c->nb_cid, *dcc = c->nb_cc, *dorder = c->nb_order, *dmisc = c->nb_misc, *dcursor = c->nb_cursor;
-  A2C_HIP(c, hipMemsetAsync(dcc, 0, sizeof(int) * (ncell + 1), c->stream));
-  A2C_HIP(c, hipMemsetAsync(dmisc, 0, sizeof(int) * 8, c->stream));
-  const int TB = 256;
+  int *dcid = c->nb_cid, *dcc = c->nb_cc, *dcursor = c->nb_cursor;
+  double4* qs = reinterpret_cast<double4*>(c->nb_order);
+  A2C_HIP(c, hipMemsetAsync(dcursor, 0, sizeof(int) * (ncell + 1), c->stream));  // the histogram
+  A2C_HIP(c, hipMemsetAsync(misc, 0, 16 * sizeof(int) + 5 * sizeof(unsigned long long), c->stream));
   if (N > 0) {
-    hipLaunchKernelGGL(k_bin, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, c->dX, N, g, dcid, dcc);
+    hipLaunchKernelGGL(k_bin, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, c->dX, N, g, dcid, dcursor);
     A2C_HIP(c, hipGetLastError());
   }
-  std::vector<int> hc(ncell + 1);
-  A2C_HIP(c, hipMemcpyAsync(hc.data(), dcc, sizeof(int) * ncell, hipMemcpyDeviceToHost, c->stream));
-  A2C_HIP(c, hipStreamSynchronize(c->stream));
-  std::vector<int> start(ncell + 1, 0);
-  for (long long k = 0; k < ncell; ++k) start[k + 1] = start[k] + hc[k];
-  A2C_HIP(c, hipMemcpyAsync(dcc, start.data(), sizeof(int) * (ncell + 1), hipMemcpyHostToDevice, c->stream));
-  A2C_HIP(c, hipMemcpyAsync(dcursor, start.data(), sizeof(int) * (ncell + 1), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_T), 0, c->stream, dcursor, (int)ncell, dcc, dcursor);
+  A2C_HIP(c, hipGetLastError());
   if (N > 0) {
-    hipLaunchKernelGGL(k_fill, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, N, dcid, dcursor, dorder);
+    hipLaunchKernelGGL(k_place, dim3((N + TB - 1) / TB), dim3(TB), 0, c->stream, c->dX, N, dcid, dcursor, qs);
     A2C_HIP(c, hipGetLastError());
   }
-  // row-length arrays (rows are fixed for a context)
+  // per-row counts (rows are fixed for a context)
   for (int w = 0; w < 5; ++w) {
     Ell& L = c->ell[w];
     L.rows = (w < 2) ? npd : nh;
     if (!L.cnt) A2C_HIP(c, dmalloc(&L.cnt, sizeof(int32_t) * std::max(L.rows, 1)));
-    A2C_HIP(c, hipMemsetAsync(L.cnt, 0, sizeof(int32_t) * std::max(L.rows, 1), c->stream));
+    if (w < 4 && !L.ocnt) {
+      A2C_HIP(c, dmalloc(&L.ocnt, sizeof(int32_t) * std::max(L.rows, 1)));
+      A2C_HIP(c, dmalloc(&L.rcnt, sizeof(int32_t) * std::max(L.rows, 1)));
+    }
   }
   NbrArgs a{};
-  a.X = c->dX; a.npd = npd; a.N = N; a.g = g; a.cid = dcid; a.cstart = dcc; a.order = dorder;
+  a.X = c->dX; a.qs = qs; a.npd = npd; a.N = N; a.g = g; a.cid = dcid; a.cstart = dcc;
+  for (int ax = 0; ax < 3; ++ax) a.ibox[ax] = g.box[ax] > 0.0 ? 1.0 / g.box[ax] : 0.0;
   a.rc2 = rc * rc;
   a.sh1 = c->desc.a_pd * c->desc.a_pd * 0.75;
   a.sh2 = c->desc.a_pd * c->desc.a_pd * 1.25;
   a.cap = std::min(c->desc.max_neighbors > 0 ? c->desc.max_neighbors : NBR_CAP, NBR_CAP);
   for (int w = 0; w < 5; ++w) a.cnt[w] = c->ell[w].cnt;
-  a.maxw = dmisc;
-  a.overflow = dmisc + 6;
-  if (N > 0) {
-    hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, c->stream, a);
-    A2C_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_width, dim3(std::min((N + 255) / 256, 1024)), dim3(256), 0, c->stream, a);
+  for (int w = 0; w < 4; ++w) { a.ocnt[w] = c->ell[w].ocnt; a.rcnt[w] = c->ell[w].rcnt; }
+  a.maxw = misc;
+  a.overflow = misc + 14;
+  a.totals = totals;
+  const int nblk_pd = (npd + NT - 1) / NT, nblk_h = (nh + NT - 1) / NT;
+  if (nblk_pd + nblk_h > 0) {
+    hipLaunchKernelGGL(k_site<false>, dim3(nblk_pd + nblk_h), dim3(NT), 0, c->stream, a, nblk_pd);
     A2C_HIP(c, hipGetLastError());
   }
-  int hm[8];
-  A2C_HIP(c, hipMemcpyAsync(hm, dmisc, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+  int hm[16];
+  A2C_HIP(c, hipMemcpyAsync(hm, misc, sizeof(hm), hipMemcpyDeviceToHost, c->stream));
   A2C_HIP(c, hipStreamSynchronize(c->stream));
-  if (hm[6]) {
+  if (hm[14]) {
     c->have_nbr = false;
     return set_err(c, A2C_ENBR_OVERFLOW, "a site has more neighbors than max_neighbors");
   }
+  // storage: ELL lists, owned halves, received-share buffers (grow-only, with slack)
   for (int w = 0; w < 5; ++w) {
     Ell& L = c->ell[w];
+    const int64_t rows = std::max(L.rows, 1);
     L.width = hm[w];
-    const int64_t n = (int64_t)std::max(hm[w], 1) * std::max(L.rows, 1);
-    if (n > L.cap) {  // grow with slack so small changes of the widest row reuse the arrays
+    const int64_t n = (int64_t)std::max(hm[w], 1) * rows;
+    if (n > L.cap) {
       (void)dfree(L.idx);
       L.idx = nullptr;
-      L.cap = (int64_t)(std::max(hm[w], 1) + 4) * std::max(L.rows, 1);
+      L.cap = (int64_t)(std::max(hm[w], 1) + 4) * rows;
       A2C_HIP(c, dmalloc(&L.idx, sizeof(int32_t) * L.cap));
     }
-    A2C_HIP(c, hipMemsetAsync(L.idx, 0xff, sizeof(int32_t) * n, c->stream));
     a.idx[w] = L.idx;
-    a.width[w] = hm[w];
-  }
-  if (!c->dpdpdnum) A2C_HIP(c, dmalloc(&c->dpdpdnum, sizeof(int32_t) * 3 * std::max(npd, 1)));
-  a.pdpdnum = c->dpdpdnum;
-  if (N > 0) {
-    hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), 0, c->stream, a);
-    A2C_HIP(c, hipGetLastError());
-  }
-  A2C_HIP(c, hipStreamSynchronize(c->stream));
-  c->stats.max_neib = hm[5];
-  for (int w = 0; w < 5; ++w) c->stats.width[w] = hm[w];
-  c->have_nbr = true;
-  return ensure_pair_buffers(c);
-}
-
-// Owned/received split and the received-share buffers of the four pair
-// lists; recomputed after every build.  Arrays grow only.
-int ensure_pair_buffers(a2c_fe_ctx* c) {
-  if (!c->have_nbr) return A2C_OK;
-  static const int partner[4] = {0, 2, 1, 3};  // PdPd, PdH<->HPd, HH
-  static const int s_sp[4] = {0, 0, 1, 1}, j_sp[4] = {0, 1, 0, 1};  // species of rows / of neighbors
-  const int npd = c->desc.n_pd;
-  const int TB = 256;
-  // 1. per-row owned/received counts and their maxima
-  for (int w = 0; w < 4; ++w) {
-    Ell& L = c->ell[w];
-    if (!L.ocnt) {
-      A2C_HIP(c, dmalloc(&L.ocnt, sizeof(int32_t) * std::max(L.rows, 1)));
-      A2C_HIP(c, dmalloc(&L.rcnt, sizeof(int32_t) * std::max(L.rows, 1)));
-    }
-  }
-  A2C_HIP(c, hipMemsetAsync(c->nb_misc, 0, sizeof(int) * 8, c->stream));
-  for (int w = 0; w < 4; ++w) {
-    const Ell& L = c->ell[w];
-    if (L.rows == 0) continue;
-    const int g = (L.rows + TB - 1) / TB;
-    hipLaunchKernelGGL(k_own_count, dim3(g), dim3(TB), 0, c->stream, L, s_sp[w] * npd, j_sp[w] * npd);
-    A2C_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_rowmax, dim3(std::min(g, 1024)), dim3(TB), 0, c->stream, L.ocnt, L.rows, c->nb_misc + 2 * w);
-    A2C_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_rowmax, dim3(std::min(g, 1024)), dim3(TB), 0, c->stream, L.rcnt, L.rows,
-                       c->nb_misc + 2 * w + 1);
-    A2C_HIP(c, hipGetLastError());
-  }
-  int wm[8];
-  A2C_HIP(c, hipMemcpyAsync(wm, c->nb_misc, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
-  A2C_HIP(c, hipStreamSynchronize(c->stream));
-  // 2. storage
-  for (int w = 0; w < 4; ++w) {
-    Ell& L = c->ell[w];
-    const int64_t rows = std::max(L.rows, 1);
-    L.owidth = wm[2 * w];
-    L.rwidth = wm[2 * w + 1];
+    if (w == 4) continue;
+    L.owidth = hm[5 + w];
+    L.rwidth = hm[9 + w];
     const int64_t on = (int64_t)std::max(L.owidth, 1) * rows, rn = (int64_t)std::max(L.rwidth, 1) * rows;
     const int64_t bn = std::max<int64_t>(32 * rn, 4 * (int64_t)std::max(L.width, 1) * rows);
     if (on > L.ocap) {
@@ -462,8 +584,17 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
       A2C_HIP(c, dmalloc(&L.rbuf, L.bcap));
     }
   }
-  // 3. split, then the partner slots (needs every list's split)
-
+  if (!c->dpdpdnum) A2C_HIP(c, dmalloc(&c->dpdpdnum, sizeof(int32_t) * 3 * std::max(npd, 1)));
+  a.pdpdnum = c->dpdpdnum;
+  a.width = std::max(hm[13], 1);  // LDS slots per lane: the longest row
+  if (nblk_pd + nblk_h > 0) {
+    hipLaunchKernelGGL(k_site<true>, dim3(nblk_pd + nblk_h), dim3(NT), sizeof(uint32_t) * CS * a.width, c->stream, a,
+                       nblk_pd);
+    A2C_HIP(c, hipGetLastError());
+  }
+  // owned/received split of the four pair lists, then the partner slots
+  static const int partner[4] = {0, 2, 1, 3};  // PdPd, PdH<->HPd, HH
+  static const int s_sp[4] = {0, 0, 1, 1}, j_sp[4] = {0, 1, 0, 1};  // species of rows / of neighbors
   for (int w = 0; w < 4; ++w) {
     const Ell& L = c->ell[w];
     if (L.rows == 0) continue;
@@ -477,8 +608,16 @@ int ensure_pair_buffers(a2c_fe_ctx* c) {
     hipLaunchKernelGGL(k_orv, dim3((L.rows + TB - 1) / TB), dim3(TB), 0, c->stream, L, c->ell[partner[w]]);
     A2C_HIP(c, hipGetLastError());
   }
-  if (!c->depair) A2C_HIP(c, dmalloc(&c->depair, sizeof(double) * std::max(c->N, 1)));
+  if (!c->depair) A2C_HIP(c, dmalloc(&c->depair, sizeof(double) * std::max(N, 1)));
+  unsigned long long ht[5];
+  A2C_HIP(c, hipMemcpyAsync(ht, totals, sizeof(ht), hipMemcpyDeviceToHost, c->stream));
   A2C_HIP(c, hipStreamSynchronize(c->stream));
+  c->stats.max_neib = hm[13];
+  for (int w = 0; w < 5; ++w) {
+    c->stats.width[w] = hm[w];
+    c->stats.total[w] = (int64_t)ht[w];
+  }
+  c->have_nbr = true;
   c->lat_dirty = true;
   return A2C_OK;
 }

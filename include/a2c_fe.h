@@ -93,6 +93,9 @@ int a2c_fe_set_occupancy(a2c_fe_ctx* ctx, const double* x);
  * (sigma ignored), PdPdnum shell counts and the first H shell HH1.  Lists are
  * frozen until the next call. */
 int a2c_fe_build_neighbors(a2c_fe_ctx* ctx, const double* X, a2c_nbr_stats* stats);
+/* The same from device-resident positions (dX: 4N doubles in HBM, e.g. the
+ * state an optimizer left there); the context state becomes dX. */
+int a2c_fe_build_neighbors_device(a2c_fe_ctx* ctx, const double* dX, a2c_nbr_stats* stats);
 
 /* CSR export of one list (0 PdPd, 1 PdH, 2 HPd, 3 HH, 4 HH1), rows sorted by
  * neighbor index.  offsets: rows+1 entries; indices: total[which] entries.

@@ -1,7 +1,10 @@
 // a2c_minimizer.hpp -- C++ host mirror of the reference's legacy DMD
 // interface over the C-ABI of liba2c_fe.so (header-only).
 //
-// Same names, argument meaning and container types as the reference:
+// Same names, argument lists, argument meaning and container types as the
+// reference (a caller written against minimizer.h compiles against these
+// signatures with a2c:: types):
+//   a2c::Minimizer::Minimizer(argc, argv) minimizer.h:82
 //   a2c::Minimizer::initialize            minimizer.h:84-88   (minimizer.cpp:484-546)
 //   a2c::Minimizer::minimizeFreeEntropy   minimizer.h:89-92   (minimizer.cpp:548-586)
 //   a2c::Minimizer::formFunctionAndGradient                   (minimizer.cpp:83-369)
@@ -105,31 +108,62 @@ inline int findNeighbors(double a_Pd, double rc, std::vector<Vec3D>& q_Pd0, std:
   return rc_ ? rc_ : st.max_neib;
 }
 
+// The reference's 12-point rule as gaussianQuadrature(6, QPp) returns it
+// (quadrature.cpp:43-75): point (m, +-) displaces coordinate m of the 6-D pair
+// (xi1 = coordinates 0-2 of site i, xi2 = 3-5 of site j) by +-sqrt(3);
+// weight QWp = 1/12 (quadrature.cpp:39).  The device evaluates this rule in
+// its structured form; initialize() checks that the caller's QPp/QWp are it.
+inline bool is_reference_rule(const std::vector<std::vector<Vec3D>>& QPp, double QWp) {
+  if (QPp.size() != 12 || std::fabs(QWp - 1.0 / 12.0) > 1e-15) return false;
+  const double h = std::sqrt(3.0);
+  std::vector<int> seen(12, 0);
+  for (const auto& p : QPp) {
+    if (p.size() != 2) return false;
+    int nz = 0, m = -1, sg = 0;
+    for (int c = 0; c < 6; ++c) {
+      const double v = c < 3 ? p[0][c] : p[1][c - 3];
+      if (std::fabs(v) < 1e-12) continue;
+      if (std::fabs(std::fabs(v) - h) > 1e-12) return false;
+      ++nz;
+      m = c;
+      sg = v > 0 ? 0 : 1;
+    }
+    if (nz != 1) return false;
+    seen[2 * m + sg]++;
+  }
+  for (int k : seen)
+    if (k != 1) return false;
+  return true;
+}
+
 class Minimizer {
  public:
   Minimizer() = default;
+  // minimizer.h:82 (PETSc initialisation in the reference; nothing to set up here)
+  Minimizer(int* /*argc*/, char*** /*argv*/) {}
   Minimizer(const Minimizer&) = delete;
   Minimizer& operator=(const Minimizer&) = delete;
   ~Minimizer() { a2c_fe_destroy(ctx_); }
 
-  // minimizer.h:84-88: neighbor lists are built on the device from q at this
-  // call (the reference receives them prebuilt); QPp/QWp are the fixed
-  // 12-point rule of quadrature.cpp and are not passed.
+  // minimizer.h:84-88, the reference's argument list.  The neighbor lists
+  // PdPd..HH are rebuilt on the device from q at rc (the same sets the
+  // reference's findLocalNeighbors produces; the pointers may be null);
+  // QPp/QWp must be the 12-point rule of quadrature.cpp (A2C_EINVAL
+  // otherwise); coe_Pd/coe_H (4 values each, may be null: zeros) feed
+  // calculateChemicalPotential.  x0 is copied, not held.
+  int initialize(Input* input, Lists* /*PdPd*/, Lists* /*PdH*/, Lists* /*HPd*/, Lists* /*HH*/,
+                 std::vector<Vec3D>& q_Pd0, std::vector<Vec3D>& q_H0, std::vector<double>& sigma_Pd0,
+                 std::vector<double>& sigma_H0, std::vector<double>& x0, std::vector<std::vector<Vec3D>>* QPp,
+                 double& QWp, std::vector<double>* coe_Pd, std::vector<double>* coe_H) {
+    if (!input || (QPp && !is_reference_rule(*QPp, QWp))) return A2C_EINVAL;
+    return setup(*input, q_Pd0, q_H0, sigma_Pd0, sigma_H0, x0, coe_Pd, coe_H);
+  }
+  // the same without the lists and the rule (they are implied)
   int initialize(Input* input, std::vector<Vec3D>& q_Pd0, std::vector<Vec3D>& q_H0, std::vector<double>& sigma_Pd0,
                  std::vector<double>& sigma_H0, std::vector<double>& x0, std::vector<double>* coe_Pd = nullptr,
                  std::vector<double>* coe_H = nullptr) {
-    input_ = *input;
-    nPd_ = q_Pd0.size();
-    nH_ = q_H0.size();
-    coe_Pd_ = coe_Pd ? *coe_Pd : std::vector<double>(4, 0.0);
-    coe_H_ = coe_H ? *coe_H : std::vector<double>(4, 0.0);
-    a2c_fe_desc d = detail::describe(input_, nPd_, nH_);
-    a2c_fe_destroy(ctx_);
-    ctx_ = nullptr;
-    if (int rc = a2c_fe_create(&d, &ctx_)) return rc;
-    if (int rc = a2c_fe_set_occupancy(ctx_, x0.data())) return rc;
-    detail::pack(q_Pd0, q_H0, sigma_Pd0, sigma_H0, X_);
-    return a2c_fe_build_neighbors(ctx_, X_.data(), &stats_);
+    if (!input) return A2C_EINVAL;
+    return setup(*input, q_Pd0, q_H0, sigma_Pd0, sigma_H0, x0, coe_Pd, coe_H);
   }
 
   // The TAO callback (minimizer.cpp:83-369) on host arrays in TAO layout.
@@ -164,7 +198,15 @@ class Minimizer {
     return rc;
   }
 
-  // minimizer.h:94-98: lists at rc from the current state, plus PdPdnum and HH1
+  // minimizer.h:94-98, the reference's argument list: the refined lists at rc
+  // from the current state, PdPdnum and HH1.  The reference refines the
+  // candidate lists *_ext; the device finds the neighbors directly, so they
+  // are not read.
+  int findLocalNeighbors(Lists& /*PdPd_ext*/, Lists& /*PdH_ext*/, Lists& /*HPd_ext*/, Lists& /*HH_ext*/, Lists& PdPd,
+                         Lists& PdH, Lists& HPd, Lists& HH, Lists& PdPdnum, Lists& HH1, int& maxNeib) {
+    return findLocalNeighbors(PdPd, PdH, HPd, HH, PdPdnum, HH1, maxNeib);
+  }
+  // the same without the candidate lists
   int findLocalNeighbors(Lists& PdPd, Lists& PdH, Lists& HPd, Lists& HH, Lists& PdPdnum, Lists& HH1, int& maxNeib) {
     if (int rc = a2c_fe_build_neighbors(ctx_, X_.data(), &stats_)) return rc;
     detail::export_list(ctx_, 0, nPd_, PdPd);
@@ -184,6 +226,25 @@ class Minimizer {
   a2c_fe_ctx* context() { return ctx_; }
 
  private:
+  int setup(const Input& input, std::vector<Vec3D>& q_Pd0, std::vector<Vec3D>& q_H0, std::vector<double>& sigma_Pd0,
+            std::vector<double>& sigma_H0, std::vector<double>& x0, std::vector<double>* coe_Pd,
+            std::vector<double>* coe_H) {
+    input_ = input;
+    nPd_ = q_Pd0.size();
+    nH_ = q_H0.size();
+    if (sigma_Pd0.size() != nPd_ || sigma_H0.size() != nH_ || x0.size() != nH_) return A2C_EINVAL;
+    if ((coe_Pd && coe_Pd->size() < 4) || (coe_H && coe_H->size() < 4)) return A2C_EINVAL;
+    coe_Pd_ = coe_Pd ? *coe_Pd : std::vector<double>(4, 0.0);
+    coe_H_ = coe_H ? *coe_H : std::vector<double>(4, 0.0);
+    a2c_fe_desc d = detail::describe(input_, nPd_, nH_);
+    a2c_fe_destroy(ctx_);
+    ctx_ = nullptr;
+    if (int rc = a2c_fe_create(&d, &ctx_)) return rc;
+    if (int rc = a2c_fe_set_occupancy(ctx_, x0.data())) return rc;
+    detail::pack(q_Pd0, q_H0, sigma_Pd0, sigma_H0, X_);
+    return a2c_fe_build_neighbors(ctx_, X_.data(), &stats_);
+  }
+
   Input input_;
   a2c_fe_ctx* ctx_ = nullptr;
   size_t nPd_ = 0, nH_ = 0;
