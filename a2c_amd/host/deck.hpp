@@ -1,7 +1,7 @@
 // deck.hpp -- reader for the reference's input language (parser/Parser.y,
 // parser/Lexer.l): nested "under Name { ... }" groups, "Key = value;"
 // assignments, dotted and indexed names (A.B, A[3]), integer and double
-// expressions with + - * / and parentheses (integer operands give integer
+// expressions with + - * / ^ and parentheses (integer operands give integer
 // results, as in IntExpr), quoted strings, bare symbols (tokens such as
 // True/False), // and /* */ comments, and an optional "end".
 //
@@ -11,6 +11,7 @@
 // value, which the host program reads by name.
 #pragma once
 #include <cctype>
+#include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <map>
@@ -170,11 +171,39 @@ class Deck {
       }
       return r;
     }
-    DeckValue term() {
+    // a ^ b: the reference's lexer passes '^' and Parser.y:29 gives it the
+    // tightest binding, left-associative (%left '^'), but no rule of IntExpr
+    // or DblExpr reduces it; here it is the power, integer when both sides
+    // are integers and the exponent is not negative, double otherwise.
+    static DeckValue power_of(const DeckValue& a, const DeckValue& b) {
+      DeckValue r;
+      if (a.kind == DeckValue::INT && b.kind == DeckValue::INT && b.i >= 0) {
+        r.kind = DeckValue::INT;
+        long long acc = 1, base = a.i;
+        for (long long e = b.i; e; e >>= 1) {
+          if (e & 1) acc *= base;
+          base *= base;
+        }
+        r.i = acc;
+      } else {
+        r.kind = DeckValue::DBL;
+        r.d = std::pow(a.num(), b.num());
+      }
+      return r;
+    }
+    DeckValue power() {
       DeckValue v = factor();
+      while (peek('^')) {
+        ++p;
+        v = power_of(v, factor());
+      }
+      return v;
+    }
+    DeckValue term() {
+      DeckValue v = power();
       while (peek('*') || peek('/')) {
         char op = t[p++];
-        v = arith(v, factor(), op);
+        v = arith(v, power(), op);
       }
       return v;
     }

@@ -38,6 +38,10 @@ def test_deck_grammar(probe, tmp_path):
     Output.Prefix = "results/";
     Mixed = (1 + 2) * 0.5;
     IntDiv = 7 / 2;
+    Pow = 2^3^2;
+    PowMul = 3 * 2^2;
+    PowD = 2.0^0.5;
+    PowNeg = 2^(0-1);
     end
     Ignored = 1;
     """
@@ -51,6 +55,10 @@ def test_deck_grammar(probe, tmp_path):
     assert v["Output.Prefix"] == ("str", "results/")
     assert float(v["Mixed"][1]) == 1.5
     assert v["IntDiv"] == ("int", "3")  # IntExpr / IntExpr stays integer
+    assert v["Pow"] == ("int", "64")  # %left '^': (2^3)^2
+    assert v["PowMul"] == ("int", "12")  # '^' binds tighter than '*'
+    assert float(v["PowD"][1]) == 2.0 ** 0.5
+    assert float(v["PowNeg"][1]) == 0.5
     assert "Ignored" not in v
 
 
