@@ -94,7 +94,7 @@ int a2c_write_xyz(const char* path, int32_t n_pd, int32_t n_h, const double* X, 
   std::fprintf(f, "Lattice=\"%e %e %e %e %e %e %e %e %e\" ", box[0], 0.0, 0.0, 0.0, box[1], 0.0, 0.0, 0.0, box[2]);
   std::fprintf(f, "Origin=\"%e %e %e\" ", 0.0, 0.0, 0.0);
   std::fprintf(f, "pbc=\"%s %s %s\" ", periodic[0] ? "T" : "F", periodic[1] ? "T" : "F", periodic[2] ? "T" : "F");
-  std::fprintf(f, "Properties=\"species:S:1:pos:R:3:disp:R:3:type:I:1:sigma:R:1:tag:I:1:molar_frac:R:1:"
+  std::fprintf(f, "Properties=\"species:S:1:pos:R:3:disp:R:3:type:I:1:atomic_freq:R:1:tag:I:1:molar_frac:R:1:"
                   "chem_potent:R:1\" Time=%e\n", time);
   for (int32_t s = 0; s < n; ++s) {
     const double* q = X + 4 * (size_t)s;

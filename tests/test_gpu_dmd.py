@@ -79,3 +79,46 @@ def test_host_program_runs_a_deck(tmp_path):
     npd = nh = 4 * 8 * 3 * 3
     X, x, g, fh = A.readResults(str(tmp_path / "slab.2.dat"), npd, nh)
     assert X.size == 4 * (npd + nh) and np.all(x >= 1e-3)
+
+
+def test_host_program_runs_the_reference_objects(tmp_path):
+    """The same specimen from the reference's Lattice / Material / Species /
+    RegionalInitialCondition objects and from the generated block gives the
+    same DMD steps, bit for bit; the object path writes the reference's
+    VTP / PVD / XYZ snapshots (frame 0 before the loop)."""
+    import os
+    import subprocess
+    import xml.etree.ElementTree as ET
+    from test_specimen import fcc_deck
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "a2c_amd", "a2c_dmd")
+    common = """
+    under InteratomicPotential { CutOffDistance = 5.35; Tolerance = 1.0e-5; }
+    under Diffusion { Temperature = 400; AttemptFrequency = 1.0e13; ActivationEnergy = 0.23;
+                      SoluteChemicalPotential = 0.5; }
+    under Time { TimeStep = 1.0e-13; MaxIts = 2; }
+    under Solver { Precision = Mixed; GradientTolerance = 1.0e-8; }
+    """
+    (tmp_path / "out").mkdir()
+    objects = tmp_path / "objects.st"
+    objects.write_text(fcc_deck(4, fmt="VTP_and_XYZ", precision="Medium").replace('Prefix = "out/"',
+                       'Prefix = "%s/out/"' % tmp_path) + common)
+    block = tmp_path / "block.st"
+    block.write_text("""
+    under Specimen { Cells = 4; Displacement = 0.0; Sigma = 0.10; HFraction = 0.1; SlabCells = 2;
+                     SlabHFraction = 0.6; Periodic = True; }
+    Diffusion.SubsurfaceLayers = 2;
+    """ + common)
+    print("host program: 4^3 cells from objects and from the block", flush=True)
+    runs = []
+    for deck in (objects, block):
+        r = subprocess.run([exe, str(deck)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        runs.append([l for l in r.stdout.splitlines() if l.startswith(("a2c_dmd", "step"))])
+    assert runs[0] == runs[1] and len(runs[0]) == 3
+    assert "(128 reservoir)" in runs[0][0]
+    sets = ET.parse(tmp_path / "out" / "spec_L0.pvd").getroot().findall("./Collection/DataSet")
+    assert [float(s.get("timestep")) for s in sets] == [0.0, 1e-13, 2e-13]
+    piece = ET.parse(tmp_path / "out" / "spec_L0_0002.vtp").getroot().find("./PolyData/Piece")
+    assert int(piece.get("NumberOfPoints")) == 8 * 4 ** 3
+    assert (tmp_path / "out" / "spec_L0_0002.xyz").exists()

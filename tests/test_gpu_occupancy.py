@@ -50,6 +50,28 @@ def test_chemical_potential_cap():
     assert np.max(np.abs(gg - go)) <= 1e-10 * np.max(np.abs(go))
 
 
+def test_chemical_potential_at_empty_and_full_sites():
+    """x = 0 and x = 1 exactly: the reference skips ln(x / (1 - x)) and the
+    pair sum there, so gamma = 1.5 (minimizer.cpp:785-787, :792); below that
+    cap gamma_bd takes over and only the x = 1 sites are marked full
+    (:793-799)."""
+    fe, orc, X, x = setup()
+    x = x.copy()
+    x[0:7:2] = 0.0
+    x[1:8:2] = 1.0
+    fe.set_occupancy(x)
+    nh = fe.n_h
+    for gbd in (1e9, 1.0):
+        gg, fg = fe.calculateChemicalPotential(X, np.zeros(4), np.zeros(4), gbd, np.zeros(nh), np.zeros(nh, np.int32))
+        go, fo = orc.chem_potential(X, x, np.zeros(4), np.zeros(4), gbd, np.zeros(nh), np.zeros(nh, np.int32))
+        assert np.array_equal(fg, fo)
+        edge = np.zeros(nh, bool)
+        edge[:8] = True
+        assert np.all(go[edge] == min(1.5, gbd)) and np.array_equal(gg[edge], go[edge])
+        assert np.all(fg[1:8:2] == (1 if gbd < 1.5 else 0)) and np.all(fg[0:7:2] == 0)
+        assert np.max(np.abs(gg - go) / np.maximum(np.abs(go), 1.0)) <= 1e-10
+
+
 def test_diffusion_parity_and_context_update():
     fe, orc, X, x = setup()
     nh = fe.n_h
