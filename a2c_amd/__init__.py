@@ -56,6 +56,19 @@ class FeDesc(C.Structure):
     ]
 
 
+class EamTabulated(C.Structure):
+    _fields_ = [
+        ("n_rho", C.c_int32), ("d_rho", C.c_double), ("n_r", C.c_int32), ("d_r", C.c_double),
+        ("F_pd", C.POINTER(C.c_double)), ("F_h", C.POINTER(C.c_double)),
+        ("f_pd", C.POINTER(C.c_double)), ("f_h", C.POINTER(C.c_double)),
+        ("rphi_pdpd", C.POINTER(C.c_double)), ("rphi_pdh", C.POINTER(C.c_double)),
+        ("rphi_hh", C.POINTER(C.c_double)),
+    ]
+
+
+TABLE_KEYS = ("F_pd", "F_h", "f_pd", "f_h", "rphi_pdpd", "rphi_pdh", "rphi_hh")
+
+
 class NbrStats(C.Structure):
     _fields_ = [("max_neib", C.c_int32), ("width", C.c_int32 * 5), ("total", C.c_int64 * 5)]
 
@@ -157,6 +170,11 @@ def _load():
         "a2c_fe_last_error": (C.c_char_p, [VP]),
         "a2c_fe_set_precision": (I, [VP, C.c_int32]),
         "a2c_fe_set_occupancy": (I, [VP, dp]),
+        "a2c_fe_set_potential_tables": (I, [VP, P(EamTabulated)]),
+        "a2c_fe_set_potential_file": (I, [VP, C.c_char_p, C.c_char_p, C.c_char_p]),
+        "a2c_fe_clear_potential": (I, [VP]),
+        "a2c_fe_set_quadrature": (I, [VP, C.c_int32, dp, dp]),
+        "a2c_eam_setfl_read": (I, [C.c_char_p, C.c_char_p, C.c_char_p, ip, dp, ip, dp, dp] + [dp] * 7),
         "a2c_fe_build_neighbors": (I, [VP, dp, P(NbrStats)]),
         "a2c_fe_build_neighbors_device": (I, [VP, VP, P(NbrStats)]),
         "a2c_fe_get_neighbors": (I, [VP, C.c_int32, ip, ip]),
@@ -239,6 +257,24 @@ def _f64(a, n=None):
 # --------------------------------------------------------------------------
 # specimens
 # --------------------------------------------------------------------------
+def eam_setfl_read(path, host="Pd", solute="H"):
+    """The tables a DYNAMO setfl file gives the engine (a2c_eam_setfl_read):
+    dict with n_rho, d_rho, n_r, d_r, cutoff and the seven arrays."""
+    n_rho, n_r = C.c_int32(), C.c_int32()
+    d_rho, d_r, cut = C.c_double(), C.c_double(), C.c_double()
+    args = [path.encode(), host.encode(), solute.encode(), C.byref(n_rho), C.byref(d_rho), C.byref(n_r),
+            C.byref(d_r), C.byref(cut)]
+    rc = lib.a2c_eam_setfl_read(*args, *([None] * 7))
+    if rc != 0:
+        raise A2CError(rc, "cannot read setfl file %s" % path)
+    out = {k: np.zeros(n_rho.value if k.startswith("F_") else n_r.value) for k in TABLE_KEYS}
+    rc = lib.a2c_eam_setfl_read(*args, *[_dptr(out[k]) for k in TABLE_KEYS])
+    if rc != 0:
+        raise A2CError(rc, "cannot read setfl file %s" % path)
+    out.update(n_rho=n_rho.value, d_rho=d_rho.value, n_r=n_r.value, d_r=d_r.value, cutoff=cut.value)
+    return out
+
+
 def lattice(n, a=3.89, seed=1, lattice_id=0, disp_std=0.02, sigma=(0.10, 0.10),
             x=(0.10, 0.10), x_clip=(1e-3, 1.0 - 1e-16), slab_cells=0, slab_x=0.0):
     """Seeded FCC Pd + octahedral-H specimen (include/a2c_lattice.h).
@@ -345,6 +381,40 @@ class FreeEntropy:
         self._check(lib.a2c_fe_set_precision(self._ctx, PREC[precision]))
 
     # -- reference-named API --
+    def set_potential_tables(self, tables):
+        """Tabulated potential (include/a2c_fe.h a2c_eam_tabulated): dict with
+        n_rho, d_rho, n_r, d_r and the arrays F_pd, F_h, f_pd, f_h, rphi_pdpd,
+        rphi_pdh, rphi_hh (pair functions as r*phi)."""
+        arrs = {k: np.ascontiguousarray(tables[k], dtype=np.float64) for k in TABLE_KEYS}
+        t = EamTabulated()
+        t.n_rho, t.d_rho = int(tables["n_rho"]), float(tables["d_rho"])
+        t.n_r, t.d_r = int(tables["n_r"]), float(tables["d_r"])
+        for k in TABLE_KEYS:
+            n = t.n_rho if k.startswith("F_") else t.n_r
+            if arrs[k].size != n:
+                raise ValueError("%s: %d values, the grid has %d" % (k, arrs[k].size, n))
+            setattr(t, k, _dptr(arrs[k]))
+        self._check(lib.a2c_fe_set_potential_tables(self._ctx, C.byref(t)))
+
+    def set_potential_file(self, path, host="Pd", solute="H"):
+        """Tabulated potential from a DYNAMO setfl file."""
+        self._check(lib.a2c_fe_set_potential_file(self._ctx, path.encode(), host.encode(), solute.encode()))
+
+    def clear_potential(self):
+        self._check(lib.a2c_fe_clear_potential(self._ctx))
+
+    def set_quadrature(self, xi=None, w=None):
+        """Node table (K x 6 xi, K weights; swap-symmetric); None restores the
+        descriptor's rule."""
+        if xi is None:
+            self._check(lib.a2c_fe_set_quadrature(self._ctx, 0, None, None))
+            return
+        xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, 6)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if len(w) != len(xi):
+            raise ValueError("one weight per node")
+        self._check(lib.a2c_fe_set_quadrature(self._ctx, len(w), _dptr(xi), _dptr(w)))
+
     def set_occupancy(self, x):
         x = _f64(x, self.n_h)
         self._x = x

@@ -1357,9 +1357,50 @@ int seg_bound(a2c_batch* b, const std::vector<int>& ids, Fill fill) {
 // Fused objective + gradient of the listed slots at X[slot] -> G[slot].
 // state_of(s) selects the state/gradient buffers.  Slots are evaluated in
 // FP64 or mixed precision as `fp64` says.  f values land in hF after sync.
+// A slot whose context has a general model (a tabulated potential or a node
+// table: a2c_fe_set_potential_*, a2c_fe_set_quadrature) is evaluated by the
+// general kernels, on its own; the fused launch takes the others.
 template <class StateOf>
-int fused_eval(a2c_batch* b, const std::vector<int>& ids, bool fp64, StateOf state_of, bool sync = true) {
-  if (ids.empty()) return A2C_OK;
+int gen_eval(a2c_batch* b, int id, bool fp64, StateOf state_of) {
+  Slot& s = b->slots[id];
+  LatDev L = s.fe->sys();
+  const double* X;
+  double* G;
+  state_of(s, &X, &G);
+  L.X = X;
+  L.G = G;
+  L.f = b->dF + id;
+  if (b->arena.used + 256 + sizeof(LatDev) > b->arena.cap)
+    if (int rc = bsync(b)) return rc;
+  LatDev* dl = b->arena.push(&L, 1);
+  if (!dl) return berr(b, A2C_ENOMEM, "staging arena too small");
+  const GenDev g = s.fe->gen;
+  const int N = s.N(), dm = b->desc.deriv_mode;
+  const double eps = b->desc.fd_eps;
+  enqueue(b, {8, (uint64_t)fp64, u64(dl), (uint64_t)N, u64(g.nodes), u64(g.tab), u64(g.tab32), (uint64_t)g.K,
+              (uint64_t)g.pot, (uint64_t)dm, u64(eps)},
+          [=](hipStream_t st) {
+            const int rc = run_eval_gen(dl, N, g, dm, eps, !fp64, st, nullptr);
+            if (rc && !b->qrc) b->qrc = rc;
+          });
+  return A2C_OK;
+}
+
+template <class StateOf>
+int fused_eval(a2c_batch* b, const std::vector<int>& ids_in, bool fp64, StateOf state_of, bool sync = true) {
+  std::vector<int> ids;
+  for (int i : ids_in) {
+    if (!b->slots[i].fe->gen_on) {
+      ids.push_back(i);
+    } else if (int rc = gen_eval(b, i, fp64, state_of)) {
+      return rc;
+    }
+  }
+  if (ids.empty()) {
+    if (!sync || ids_in.empty()) return A2C_OK;
+    enqueue_d2h(b, b->hF, b->dF, sizeof(double) * b->slots.size());
+    return bsync(b);
+  }
   std::vector<LatDev> lats(ids.size());
   std::vector<int32_t> map, omap;  // site block -> lattice; pair-owner workgroup -> lattice
   int cum = 0, ocum = 0, rules = 0;

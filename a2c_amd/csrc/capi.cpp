@@ -42,6 +42,7 @@ bool aligned32(const void* p) { return ((uintptr_t)p & 31u) == 0; }
 int eval_dispatch(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm) {
   if (!c->have_nbr) return set_err(c, A2C_ESTATE, "build_neighbors must be called before eval");
   if (!c->have_x) return set_err(c, A2C_ESTATE, "set_occupancy must be called before eval");
+  if (c->gen_on) return launch_eval_gen(c, dX, dG, dF, timed, tm);
   if (c->desc.precision == A2C_PREC_MIXED) return launch_eval32(c, dX, dG, dF, timed, tm);
   return launch_eval64(c, dX, dG, dF, timed, tm);
 }
@@ -128,6 +129,7 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   dfree(c->dX); dfree(c->dG); dfree(c->dx); dfree(c->drho); dfree(c->dFp);
   dfree(c->demb); dfree(c->dpart); dfree(c->dF); dfree(c->dpdpdnum); dfree(c->dtab); dfree(c->daux);
   dfree(c->dtab64);
+  gen_free(c);
   for (int w = 0; w < 5; ++w) {
     dfree(c->ell[w].idx); dfree(c->ell[w].cnt);
     a2c::free_pair_buffers(c->ell[w]);

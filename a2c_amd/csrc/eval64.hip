@@ -946,6 +946,7 @@ static Launch single_launch(a2c_fe_ctx* c, bool fp64) {
 }
 
 int launch_density64(a2c_fe_ctx* c, const double* dX) {
+  if (c->gen_on) return launch_density_gen(c, dX);
   if (int rc = sync_lattice(c, dX, c->dG, c->dF)) return rc;
   if (run_density64(single_launch(c, true))) return set_err(c, A2C_EHIP, "density64 launch failed");
   return A2C_OK;

@@ -167,6 +167,31 @@ __device__ inline void gh_node(int k, double z[3], double& w) {
 }
 
 
+// General model of a context (a2c_fe_set_potential_*, a2c_fe_set_quadrature):
+// the functions of a tabulated potential as cubic pieces on uniform grids,
+// and the rule as a node table.  Function order: F_Pd, F_H (embedding),
+// f_Pd, f_H (density), r phi_PdPd, r phi_PdH, r phi_HH (pair).
+enum { GEN_F_PD = 0, GEN_F_H, GEN_RHO_PD, GEN_RHO_H, GEN_RPHI_PD, GEN_RPHI_PDH, GEN_RPHI_H, GEN_NFN };
+struct GenFn {
+  int32_t n = 0;        // nodes (n - 1 pieces), node k at k * h
+  int32_t off = 0;      // first piece in the coefficient arrays
+  double ih = 0.0;      // 1 / h
+  double yend = 0.0;    // value and slope at the last node (embedding
+  double dend = 0.0;    // functions continue along it)
+};
+struct GenDev {
+  int32_t pot = 0;      // 0: compiled-in Pd-H closed forms; 1: tabulated
+  int32_t K = 0;        // nodes of the rule
+  int32_t rel = 0;      // 1: nodes z on the relative coordinate, r = d + s z,
+                        //    s^2 = sigma_i^2 + sigma_j^2 (the Gauss-Hermite rule);
+                        // 0: 6-D nodes, r = d + sqrt(2) (sigma_i xi1 - sigma_j xi2)
+  const double* nodes = nullptr;  // K x 8: xi1[3] (or z), xi2[3], w, 0
+  const double4* tab = nullptr;   // FP64 piece coefficients (c0..c3 in u = x/h - k)
+  const float4* tab32 = nullptr;  // FP32 pieces of the five r-functions (mixed)
+  GenFn fn[GEN_NFN];
+  GenFn fn32[GEN_NFN];  // the FP32 pieces' grids (f_Pd .. r phi_HH)
+};
+
 // Minimum-image geometry of one lattice, copied out of its descriptor into
 // registers before a pair loop (the loop's stores could otherwise alias the
 // descriptor, and its fields would be reloaded for every pair).
@@ -264,6 +289,15 @@ struct a2c_fe_ctx {
   double* occ_bal = nullptr;   // 4: the balance of the last step (a2c_diffusion_balance)
   a2c_diffusion_balance last_balance{};
 
+  // general model (null gen_nodes: the descriptor's rule and the closed forms)
+  bool gen_on = false;          // evaluate with the general kernels
+  bool gen_table_rule = false;  // a node table was set (else the descriptor's rule)
+  a2c::GenDev gen{};
+  double* gen_nodes = nullptr;  // device: K x 8
+  double4* gen_tab = nullptr;   // device: FP64 pieces
+  float4* gen_tab32 = nullptr;  // device: FP32 pieces
+  size_t gen_tab_n = 0, gen_tab32_n = 0;
+
   a2c::DevSys sys() const {
     a2c::DevSys s;
     s.npd = desc.n_pd;
@@ -338,6 +372,15 @@ enum { T64Q_FPD = 0, T64Q_PHPD = 1 };
 constexpr int T64_PART1 = T64_NF * T64_N * T64_P;                // doubles
 constexpr int T64_SIZE = T64_PART1 + T64Q_NF * T64_N * T64Q_P;  // doubles
 void host_tables64(std::vector<double>& t);
+// General model (evalgen.hip, general.cpp)
+int run_density_gen(const LatDev* dlat, int N, const GenDev& g, int deriv_mode, double eps, hipStream_t st);
+int run_eval_gen(const LatDev* dlat, int N, const GenDev& g, int deriv_mode, double eps, bool fp32, hipStream_t st,
+                 Timing* tm);
+int launch_density_gen(a2c_fe_ctx* c, const double* dX);  // fills drho, dFp, demb
+int launch_eval_gen(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool timed, Timing* tm);
+int launch_chem_gen(a2c_fe_ctx* c, const double* dX, const double* coe_h, double gamma_bd, double* dgam,
+                    int32_t* dfull, const double* dcorr);
+void gen_free(a2c_fe_ctx* c);
 int lmvm_minimize(a2c_fe_ctx* c, const double* X0, const a2c_lmvm_opts* o, double* Xout, a2c_lmvm_report* rep);
 int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const double* coe_h,
                 double gamma_bd, double* dgam, int32_t* dfull, double* dcorr);  // occupancy.hip

@@ -247,6 +247,7 @@ int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const dou
                        coe_pd[2], coe_pd[3], coe_h[0], coe_h[2], coe_h[3], dcorr);
     A2C_HIP(c, hipGetLastError());
   }
+  if (S.nh > 0 && c->gen_on) return launch_chem_gen(c, dX, coe_h, gamma_bd, dgam, dfull, dcorr);
   if (S.nh > 0) {
     hipLaunchKernelGGL(S.rule == A2C_QUAD_GH3 ? k_chem<A2C_QUAD_GH3> : k_chem<A2C_QUAD_R12>,
                        dim3((S.nh + TB - 1) / TB), dim3(TB), 0, c->stream, S, dX, c->dx, c->drho, c->dFp, dcorr,

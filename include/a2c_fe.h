@@ -129,6 +129,62 @@ int a2c_fe_chem_potential(a2c_fe_ctx* ctx, const double* X, const double* coe_pd
                           const double* coe_h, double gamma_bd, double* gamma,
                           int32_t* full_h);
 
+/* ---- general potentials and quadrature rules -------------------------
+ * The reference parses InteratomicPotential.File (IoData.cpp:523-534,
+ * IoData.h:289-300) but evaluates only its compiled-in Pd-H functions
+ * (potential.cpp), and uses only its 12-point rule (quadrature.cpp:43-75).
+ * These calls give a context a tabulated potential and/or a node table; the
+ * context then evaluates (a2c_fe_eval, _density, _chem_potential, the
+ * optimizer) with per-site kernels over the full lists, FP64 throughout
+ * (A2C_PREC_FP64) or FP32 pair terms with FP64 site sums (A2C_PREC_MIXED),
+ * analytic or central-FD derivatives.  Batches do not take such contexts.
+ *
+ * Tabulated functions are uniform grids from 0 (value k at k*d), interpolated
+ * by cubic Hermite pieces whose node slopes are fourth-order central
+ * differences (second-order and one-sided at the ends) -- the interpolation
+ * the DYNAMO setfl format is written for.  Pair functions are given as
+ * r*phi(r).  Past the last node a density or pair function is 0; an
+ * embedding function continues along its end slope. */
+typedef struct a2c_eam_tabulated {
+  int32_t n_rho;      /* nodes of F_pd and F_h, on rho_k = k * d_rho */
+  double d_rho;
+  int32_t n_r;        /* nodes of the density and pair functions, on r_k = k * d_r */
+  double d_r;
+  const double* F_pd;      /* embedding energy of a Pd site, n_rho values */
+  const double* F_h;       /* embedding energy of an H site */
+  const double* f_pd;      /* density contributed by a Pd neighbor, n_r values */
+  const double* f_h;       /* density contributed by an H neighbor */
+  const double* rphi_pdpd; /* r * phi(r) of Pd-Pd, Pd-H and H-H pairs, n_r values */
+  const double* rphi_pdh;
+  const double* rphi_hh;
+} a2c_eam_tabulated;
+int a2c_fe_set_potential_tables(a2c_fe_ctx* ctx, const a2c_eam_tabulated* t);
+/* The same from a DYNAMO setfl file (the eam/alloy layout: 3 comment lines;
+ * "n el_1 .. el_n"; "n_rho d_rho n_r d_r cutoff"; per element a header line,
+ * F(rho) and f(r); then r*phi for every pair i >= j), taking the elements
+ * named host (the Pd sites) and solute (the H sites). */
+int a2c_fe_set_potential_file(a2c_fe_ctx* ctx, const char* path, const char* host, const char* solute);
+/* Host-only reader behind it: the grid (any output may be null) and the seven
+ * tables a2c_fe_set_potential_file installs, copied into caller arrays of
+ * n_rho or n_r doubles (null arrays skipped: a first call with null arrays
+ * returns the sizes). */
+int a2c_eam_setfl_read(const char* path, const char* host, const char* solute, int32_t* n_rho, double* d_rho,
+                       int32_t* n_r, double* d_r, double* cutoff, double* F_pd, double* F_h, double* f_pd,
+                       double* f_h, double* rphi_pdpd, double* rphi_pdh, double* rphi_hh);
+/* Back to the compiled-in Pd-H functions. */
+int a2c_fe_clear_potential(a2c_fe_ctx* ctx);
+
+/* A node table for the phase-packet average: node k displaces site i by
+ * sqrt(2) sigma_i xi1_k and site j by sqrt(2) sigma_j xi2_k, so the pair
+ * vector is r_k = q_i - q_j + sqrt(2) (sigma_i xi1_k - sigma_j xi2_k), with
+ * weight w_k (the reference's rule is xi = +-sqrt(3) e_m, w = 1/12).
+ * xi: K x 6 doubles (xi1 then xi2 per node), w: K doubles.  The table must be
+ * swap-symmetric -- (xi2_k, xi1_k, w_k) is also a node -- so a pair's average
+ * is the same from either end, which the reference's gradient assumes.  K in
+ * 1..A2C_QUAD_MAX_NODES; K = 0 restores the descriptor's rule. */
+#define A2C_QUAD_MAX_NODES 512
+int a2c_fe_set_quadrature(a2c_fe_ctx* ctx, int32_t K, const double* xi, const double* w);
+
 /* Replaces integrateDiffusion (integrate.cpp:9-87): one explicit step of the
  * mean-field master equation on HH1 with rate nu*exp(-Qm/(kB T)); full_h sites
  * keep x; result clamped to [xLb, xUb].  x0 is the context occupancy; the new

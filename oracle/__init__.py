@@ -22,8 +22,8 @@ REF_QUAD = os.path.join(HERE, "_ref", "quadrature_ref")
 F_PD, F_H, RHO_PD, RHO_H, PHI_PD, PHI_PDH, PHI_H = range(7)
 FUNC_NAMES = ["F_Pd", "F_H", "f_Pd", "f_H", "phi_Pd", "phi_PdH", "phi_H"]
 DERIV = {"analytic": 0, "fd": 1}
-POT = {"eam": 0, "test": 1, "test4": 2}
-QUAD = {"r12": 0, "gh3": 1}
+POT = {"eam": 0, "test": 1, "test4": 2, "tab": 3}
+QUAD = {"r12": 0, "gh3": 1, "table": 2}
 
 
 def build(reference=None):
@@ -62,6 +62,9 @@ def _load():
         "or_eval": (I, [P(Params), VP, dp, dp, dp, dp]),
         "or_chem_potential": (I, [P(Params), VP, dp, dp, dp, dp, D, dp, ip]),
         "or_diffuse": (I, [P(Params), VP, D, D, D, D, D, dp, ip, dp, dp]),
+        "or_set_tabulated": (I, [I, D, I, D, P(dp)]),
+        "or_set_quad_table": (I, [I, dp, dp]),
+        "or_tab_eval": (I, [I, D, dp, I, D, dp, dp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -99,6 +102,34 @@ def quadrature():
     w = np.zeros(12)
     lib.or_quadrature(_d(xi), _d(w))
     return xi.reshape(12, 6), w
+
+
+_keep = []  # the arrays behind the process-wide tables stay alive
+
+
+def set_tabulated(tables):
+    """Tables of the "tab" potential (process-wide): dict with n_rho, d_rho,
+    n_r, d_r and F_pd, F_h, f_pd, f_h, rphi_pdpd, rphi_pdh, rphi_hh."""
+    arrs = [np.ascontiguousarray(tables[k], dtype=np.float64)
+            for k in ("F_pd", "F_h", "f_pd", "f_h", "rphi_pdpd", "rphi_pdh", "rphi_hh")]
+    ptrs = (C.POINTER(C.c_double) * 7)(*[_d(a) for a in arrs])
+    assert lib.or_set_tabulated(int(tables["n_rho"]), float(tables["d_rho"]), int(tables["n_r"]),
+                                float(tables["d_r"]), ptrs) == 0
+
+
+def set_quad_table(xi, w):
+    """Nodes of the "table" rule (process-wide): xi K x 6, w K."""
+    xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, 6)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    assert lib.or_set_quad_table(len(w), _d(xi), _d(w)) == 0
+
+
+def tab_eval(y, h, x, emb=False):
+    """One table on its own: (value, slope) at x."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    v, d = C.c_double(), C.c_double()
+    assert lib.or_tab_eval(len(y), h, _d(y), int(emb), float(x), C.byref(v), C.byref(d)) == 0
+    return v.value, d.value
 
 
 class Oracle:

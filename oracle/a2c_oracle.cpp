@@ -227,9 +227,67 @@ NOIPA double phi_H(double r) { return morse(K_PHH, R_PHH, r); }
 /* OR_POT_TEST: g(r) = r^2 for every pair/density function, F(rho) = rho. */
 int g_potential = OR_POT_EAM;
 
+/* OR_POT_TAB: the seven functions as uniform-grid node values (pair
+ * functions as r*phi), interpolated piecewise by cubic Hermite polynomials
+ * in the Hermite basis h00 = 2u^3 - 3u^2 + 1, h10 = u^3 - 2u^2 + u,
+ * h01 = 3u^2 - 2u^3, h11 = u^3 - u^2 with node slopes (per grid step) from
+ * central differences: (y[m-2] - 8y[m-1] + 8y[m+1] - y[m+2])/12 inside,
+ * (y[m+1] - y[m-1])/2 at the second and second-to-last node, one-sided at
+ * the ends -- the interpolation DYNAMO setfl tables are written for.  Past
+ * the last node r-functions are 0 and embedding functions continue along the
+ * end slope.  Function order as OR_F_PD .. OR_PHI_H. */
+struct Tab {
+  std::vector<double> y, s;
+  double h = 0.0;
+};
+Tab g_tab[OR_NFUNC];
+
+void tab_slopes(Tab& t) {
+  const std::vector<double>& y = t.y;
+  const int n = (int)y.size();
+  t.s.assign(n, 0.0);
+  t.s[0] = y[1] - y[0];
+  t.s[n - 1] = y[n - 1] - y[n - 2];
+  t.s[1] = (y[2] - y[0]) / 2.0;
+  t.s[n - 2] = (y[n - 1] - y[n - 3]) / 2.0;
+  for (int m = 2; m <= n - 3; ++m) t.s[m] = (y[m - 2] - 8.0 * y[m - 1] + 8.0 * y[m + 1] - y[m + 2]) / 12.0;
+}
+
+/* value (d = 0) or slope (d = 1) of table t at x */
+double tab_eval(const Tab& t, double x, bool emb, int d) {
+  const int n = (int)t.y.size();
+  const double p = x / t.h;
+  int m = (int)std::floor(p);
+  if (m < 0) m = 0;
+  if (m >= n - 1) {
+    if (!emb) return 0.0;
+    const double slope = t.s[n - 1] / t.h;
+    return d ? slope : t.y[n - 1] + slope * (x - (n - 1) * t.h);
+  }
+  const double u = p - m, u2 = u * u, u3 = u2 * u;
+  if (!d)
+    return (2 * u3 - 3 * u2 + 1) * t.y[m] + (u3 - 2 * u2 + u) * t.s[m] + (3 * u2 - 2 * u3) * t.y[m + 1] +
+           (u3 - u2) * t.s[m + 1];
+  return ((6 * u2 - 6 * u) * t.y[m] + (3 * u2 - 4 * u + 1) * t.s[m] + (6 * u - 6 * u2) * t.y[m + 1] +
+          (3 * u2 - 2 * u) * t.s[m + 1]) / t.h;
+}
+
+double tab_val(int fn, double x) {
+  const bool emb = fn == OR_F_PD || fn == OR_F_H;
+  const double v = tab_eval(g_tab[fn], x, emb, 0);
+  return fn >= OR_PHI_PD ? v / x : v;
+}
+double tab_der(int fn, double x) {
+  const bool emb = fn == OR_F_PD || fn == OR_F_H;
+  const double d = tab_eval(g_tab[fn], x, emb, 1);
+  if (fn < OR_PHI_PD) return d;
+  return (d * x - tab_eval(g_tab[fn], x, emb, 0)) / (x * x);  /* (z/x)' */
+}
+
 double eam_val(int fn, double x) {
   if (g_potential == OR_POT_TEST) return (fn == OR_F_PD || fn == OR_F_H) ? x : x * x;
   if (g_potential == OR_POT_TEST4) return (fn == OR_F_PD || fn == OR_F_H) ? x : x * x * x * x;
+  if (g_potential == OR_POT_TAB) return tab_val(fn, x);
   switch (fn) {
     case OR_F_PD: return F_Pd(x);
     case OR_F_H: return F_H(x);
@@ -248,6 +306,7 @@ double eam_der(int fn, double x, int mode, double eps) {
   if (mode == OR_DERIV_FD) return (eam_val(fn, x + eps) - eam_val(fn, x - eps)) / (2.0 * eps);
   if (g_potential == OR_POT_TEST) return (fn == OR_F_PD || fn == OR_F_H) ? 1.0 : 2.0 * x;
   if (g_potential == OR_POT_TEST4) return (fn == OR_F_PD || fn == OR_F_H) ? 1.0 : 4.0 * x * x * x;
+  if (g_potential == OR_POT_TAB) return tab_der(fn, x);
   switch (fn) {
     case OR_F_PD: return F_Pd_a(x).d;
     case OR_F_H: return F_H_a(x).d;
@@ -296,15 +355,28 @@ const Quad QUAD = make_quad();
  * dsi_k, the derivative of r_k w.r.t. sigma_i up to the factor sigfac (the
  * 12-point rule keeps the reference's sqrt(2) as a final factor on the
  * summed sigma gradient, minimizer.cpp:221,283). */
+const int MAXK = 512;
 struct Nodes {
   int K;
-  V3 r[27];
-  double w[27];
-  V3 dsi[27];
+  V3 r[MAXK];
+  double w[MAXK];
+  V3 dsi[MAXK];
 };
+/* OR_QUAD_TABLE: a node table (xi1, xi2, w) used like the 12-point rule. */
+struct QTable { std::vector<V3> xi1, xi2; std::vector<double> w; };
+QTable g_qtab;
 const double SQ3 = 1.7320508075688772935;
 inline void pair_nodes(int rule, const V3& qi, double si, const V3& qj, double sj, Nodes& n) {
   const double sq2 = std::sqrt(2.0);
+  if (rule == OR_QUAD_TABLE) {
+    n.K = (int)g_qtab.w.size();
+    for (int k = 0; k < n.K; ++k) {
+      n.r[k] = vsub(vsub(vadd(qi, vscale(sq2 * si, g_qtab.xi1[k])), qj), vscale(sq2 * sj, g_qtab.xi2[k]));
+      n.w[k] = g_qtab.w[k];
+      n.dsi[k] = g_qtab.xi1[k];
+    }
+    return;
+  }
   if (rule != OR_QUAD_GH3) {
     n.K = QK;
     for (int k = 0; k < QK; ++k) {
@@ -525,6 +597,41 @@ extern "C" int or_quadrature(double* xi, double* w) {
     w[k] = QUAD.w;
   }
   return QK;
+}
+
+extern "C" int or_set_tabulated(int n_rho, double d_rho, int n_r, double d_r, const double* const* tables) {
+  if (n_rho < 5 || n_r < 5 || !(d_rho > 0) || !(d_r > 0) || !tables) return -1;
+  for (int f = 0; f < OR_NFUNC; ++f) {
+    const bool emb = f == OR_F_PD || f == OR_F_H;
+    const int n = emb ? n_rho : n_r;
+    g_tab[f].y.assign(tables[f], tables[f] + n);
+    g_tab[f].h = emb ? d_rho : d_r;
+    tab_slopes(g_tab[f]);
+  }
+  return 0;
+}
+
+extern "C" int or_set_quad_table(int K, const double* xi, const double* w) {
+  if (K < 1 || K > MAXK || !xi || !w) return -1;
+  g_qtab = QTable();
+  for (int k = 0; k < K; ++k) {
+    g_qtab.xi1.push_back({{xi[6 * k], xi[6 * k + 1], xi[6 * k + 2]}});
+    g_qtab.xi2.push_back({{xi[6 * k + 3], xi[6 * k + 4], xi[6 * k + 5]}});
+    g_qtab.w.push_back(w[k]);
+  }
+  return 0;
+}
+
+/* One table evaluated on its own (test access): value and slope at x. */
+extern "C" int or_tab_eval(int n, double h, const double* y, int emb, double x, double* v, double* d) {
+  if (n < 5 || !(h > 0) || !y) return -1;
+  Tab t;
+  t.y.assign(y, y + n);
+  t.h = h;
+  tab_slopes(t);
+  if (v) *v = tab_eval(t, x, emb != 0, 0);
+  if (d) *d = tab_eval(t, x, emb != 0, 1);
+  return 0;
 }
 
 /* ===========================================================================

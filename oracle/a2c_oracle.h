@@ -51,13 +51,13 @@ enum { OR_DERIV_ANALYTIC = 0, OR_DERIV_FD = 1 };
 /* OR_POT_TEST replaces every EAM function by g(r)=r^2 (pairs and density) and
  * F(rho)=rho, so the averaged energy has a closed-form Gaussian moment;
  * OR_POT_TEST4 uses g(r)=r^4 (exact for GH3, not for the 12-point rule). */
-enum { OR_POT_EAM = 0, OR_POT_TEST = 1, OR_POT_TEST4 = 2 };
+enum { OR_POT_EAM = 0, OR_POT_TEST = 1, OR_POT_TEST4 = 2, OR_POT_TAB = 3 };
 /* Phase-packet rules: the reference's 12-point rule (quadrature.cpp:43-75), or
  * a 3-point-per-axis Gauss-Hermite product rule on the relative coordinate
  * (27 nodes; builder addition, not in the reference): every pair function
  * depends only on r = d + sqrt(2)(sigma_i xi_1 - sigma_j xi_2) ~ N(d, s^2 I)
  * with s^2 = sigma_i^2 + sigma_j^2, so E g(|r|) = sum_k w_k g(|d + s z_k|). */
-enum { OR_QUAD_R12 = 0, OR_QUAD_GH3 = 1 };
+enum { OR_QUAD_R12 = 0, OR_QUAD_GH3 = 1, OR_QUAD_TABLE = 2 };
 
 typedef struct or_params {
   int32_t n_pd, n_h;
@@ -80,6 +80,14 @@ int or_eam_constants(int fn, double* out, int cap);
 
 /* Reference rule: K=12 nodes xi[k][0..5] (site i: 0..2, site j: 3..5), w[k]. */
 int or_quadrature(double* xi, double* w);
+
+/* OR_POT_TAB tables (process-wide): tables[f] for f in OR_F_PD..OR_PHI_H,
+ * node k at k*d_rho (F) or k*d_r (the others; pair functions as r*phi). */
+int or_set_tabulated(int n_rho, double d_rho, int n_r, double d_r, const double* const* tables);
+/* OR_QUAD_TABLE nodes (process-wide): xi K x 6, w K. */
+int or_set_quad_table(int K, const double* xi, const double* w);
+/* One table on its own: value and slope at x (emb: continue past the end). */
+int or_tab_eval(int n, double h, const double* y, int emb, double x, double* v, double* d);
 
 typedef struct or_nbr or_nbr;
 or_nbr* or_nbr_build(const or_params* p, const double* X);
