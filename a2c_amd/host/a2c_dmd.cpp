@@ -21,7 +21,8 @@
 //    Diffusion.SubsurfaceLayers layers of cells along x; XYZ snapshots.
 //
 // Keys (reference names where the reference has them, IoData.cpp):
-//   InteratomicPotential.Tolerance, .CutOffDistance          (:523-534)
+//   InteratomicPotential.File (a DYNAMO setfl file, elements Pd and H),
+//   .Tolerance, .CutOffDistance                               (:523-534)
 //   Diffusion.Temperature, .AttemptFrequency, .ActivationEnergy,
 //   .SoluteChemicalPotential (the reservoir potential)        (:639-651)
 //   Time.TimeStep, Time.MaxIts                                 (:696-699)
@@ -170,6 +171,13 @@ int main(int argc, char** argv) {
   a2c_fe_ctx* ctx = nullptr;
   if (int rc = a2c_fe_create(&d, &ctx)) {
     std::fprintf(stderr, "ERROR: cannot create the engine (%d)\n", rc);
+    return 1;
+  }
+  // InteratomicPotential.File: a tabulated potential (setfl) for the Pd and H sites
+  const std::string pot_file = deck.str("InteratomicPotential.File", "");
+  if (!pot_file.empty() && a2c_fe_set_potential_file(ctx, pot_file.c_str(), "Pd", "H")) {
+    std::fprintf(stderr, "ERROR: %s\n", a2c_fe_last_error(ctx));
+    a2c_fe_destroy(ctx);
     return 1;
   }
   a2c_dmd_params p;

@@ -187,3 +187,30 @@ def test_bad_tables_are_refused(setfl):
         bad.update({k: t[k][:3] for k in ("f_pd", "f_h", "rphi_pdpd", "rphi_pdh", "rphi_hh")})
         with pytest.raises(A.A2CError):
             fe.set_potential_tables(bad)
+
+
+def test_host_program_takes_the_potential_file(setfl, tmp_path):
+    """InteratomicPotential.File in a deck: the host program relaxes with the
+    tabulated potential; with the Pd-H functions sampled finely its step
+    objectives are the compiled-in ones to the interpolation error."""
+    import os
+    import re
+    import subprocess
+    path, t = setfl
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    base = """
+    under Specimen { Cells = 4; Seed = 5; Displacement = 0.02; HFraction = 0.1; }
+    under Diffusion { Temperature = 400; AttemptFrequency = 1.0e13; ActivationEnergy = 0.23; }
+    under Time { TimeStep = 1.0e-13; MaxIts = 1; }
+    under Solver { Precision = FP64; GradientTolerance = 1.0e-8; }
+    """
+    out = []
+    print("host program: compiled-in potential, then the setfl file", flush=True)
+    for extra in ("", 'InteratomicPotential.File = "%s";' % path):
+        deck = tmp_path / "d.st"
+        deck.write_text(base + extra)
+        r = subprocess.run([os.path.join(root, "a2c_amd", "a2c_dmd"), str(deck)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        out.append(float(re.search(r"f = (\S+)", r.stdout).group(1)))
+    assert abs(out[1] - out[0]) <= 1e-7 * abs(out[0])
