@@ -70,7 +70,8 @@ TABLE_KEYS = ("F_pd", "F_h", "f_pd", "f_h", "rphi_pdpd", "rphi_pdh", "rphi_hh")
 
 
 class NbrStats(C.Structure):
-    _fields_ = [("max_neib", C.c_int32), ("width", C.c_int32 * 5), ("total", C.c_int64 * 5)]
+    _fields_ = [("max_neib", C.c_int32), ("width", C.c_int32 * 5), ("total", C.c_int64 * 5),
+                ("refreshed", C.c_int32), ("max_displacement", C.c_double)]
 
 
 class LmvmOpts(C.Structure):
@@ -332,6 +333,11 @@ def philox4x32_10(ctr, key):
 # --------------------------------------------------------------------------
 # engine
 # --------------------------------------------------------------------------
+def _nbr_stats(st):
+    return {"max_neib": st.max_neib, "width": list(st.width), "total": list(st.total),
+            "refreshed": bool(st.refreshed), "max_displacement": st.max_displacement}
+
+
 class FreeEntropy:
     """One lattice on one GPU (a2c_fe_ctx)."""
 
@@ -424,7 +430,7 @@ class FreeEntropy:
         X = _f64(X, 4 * self.N)
         st = NbrStats()
         self._check(lib.a2c_fe_build_neighbors(self._ctx, _dptr(X), C.byref(st)))
-        return {"max_neib": st.max_neib, "width": list(st.width), "total": list(st.total)}
+        return _nbr_stats(st)
 
     build_neighbors = findNeighbors
 
@@ -432,7 +438,7 @@ class FreeEntropy:
         """Neighbor build from device-resident positions (a device pointer)."""
         st = NbrStats()
         self._check(lib.a2c_fe_build_neighbors_device(self._ctx, C.c_void_p(dX), C.byref(st)))
-        return {"max_neib": st.max_neib, "width": list(st.width), "total": list(st.total)}
+        return _nbr_stats(st)
 
     def neighbors(self, which):
         """CSR (offsets, indices) of one list: 'PdPd', 'PdH', 'HPd', 'HH' or 'HH1'."""

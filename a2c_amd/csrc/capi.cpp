@@ -133,7 +133,10 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   for (int w = 0; w < 5; ++w) {
     dfree(c->ell[w].idx); dfree(c->ell[w].cnt);
     a2c::free_pair_buffers(c->ell[w]);
+    dfree(c->nb_sup[w].idx); dfree(c->nb_sup[w].cnt);
+    a2c::free_pair_buffers(c->nb_sup[w]);
   }
+  dfree(c->nb_sup_q);
   dfree(c->depair);
   dfree(c->occ_g); dfree(c->occ_full); dfree(c->occ_corr); dfree(c->occ_f); dfree(c->occ_xn);
   dfree(c->occ_part); dfree(c->occ_bal);

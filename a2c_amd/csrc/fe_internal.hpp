@@ -258,8 +258,16 @@ struct a2c_fe_ctx {
   void* nb_order = nullptr;  // N double4: positions in cell order, site index in .w
   int* nb_cc = nullptr;      // cells + 1: cell starts
   int* nb_cursor = nullptr;  // cells + 1: histogram, then placement cursors
-  int* nb_misc = nullptr;    // 256 bytes: widths, overflow flag, list totals, bounding box
+  int* nb_misc = nullptr;    // build counters (list widths, totals; copies per workgroup group), displacement, bounding box
   int64_t nb_cap_cells = 0;
+  // skin lists: the pair lists at rc + skin and the positions they were built
+  // at; while every site is within skin/2 of those positions the lists at rc
+  // are filtered from them (nbr.hip)
+  a2c::Ell nb_sup[5];
+  double* nb_sup_q = nullptr;  // 4N: positions at the skin build
+  bool nb_sup_ok = false;      // skin lists valid for the current positions' neighbors
+  bool nb_sup_off = false;     // skin lists refused for this context (a row over the cap)
+  int32_t nb_builds = 0;       // builds by walking the cells
   int32_t* dpdpdnum = nullptr;  // nPd*3
   a2c_nbr_stats stats{};
 

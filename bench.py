@@ -405,6 +405,7 @@ def main():
         "error_vs_fp64": m["error_vs_fp64"],
         "neighbor_build_ms": m["neighbor_build_ms"],
         "neighbor_rebuild_device_ms": m["neighbor_rebuild_device_ms"],
+        "neighbor_skin_rebuild_device_ms": m["neighbor_skin_rebuild_device_ms"],
         "density_pass_flop": m["density_pass_flop"],
         "wall_s_timed": m["wall_s_timed"],
     }
@@ -440,19 +441,40 @@ def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False):
     K = POINTS[quad]
     flop_grad = K * sum(c * u for c, u in zip(FLOP_GRAD, U))
     flop_dens = K * sum(c * u for c, u in zip(FLOP_DENS, U))
-    # rebuilds from device-resident positions (the relaxation loop's case):
-    # kernels plus the build's two host synchronisations, wall clock
+    # rebuilds from device-resident positions (the DMD loop's case), wall
+    # clock including the build's host synchronisations: after moves like a
+    # DMD step's (every coordinate by N(0, 0.005 A) per rebuild, so the sites
+    # stay within skin/2 of the skin lists: the lists are filtered from them),
+    # and after a move beyond that (N(0, 0.2 A): the cells are walked at
+    # rc + skin first).  Positions are uploaded outside the timed region.
     dXn = fe.dev_alloc(X.nbytes)
-    fe.upload(dXn, X)
-    reb = []
+    rng = np.random.default_rng(11)
+    Xm = X.copy().reshape(-1, 4)
+    fe.upload(dXn, Xm.reshape(-1))
+    fe.findNeighbors_device(dXn)  # the skin lists (second build)
+    reb, skin_reb = [], []
     for _ in range(5):
+        Xm[:, :3] += rng.normal(0.0, 0.005, size=(N, 3))
+        fe.upload(dXn, Xm.reshape(-1))
+        t0 = time.perf_counter()
+        sd = fe.findNeighbors_device(dXn)
+        reb.append((time.perf_counter() - t0) * 1e3)
+        assert sd["refreshed"]
+    for _ in range(3):
+        Xj = Xm.copy()
+        Xj[:, :3] += rng.normal(0.0, 0.2, size=(N, 3))
+        fe.upload(dXn, Xj.reshape(-1))
         t0 = time.perf_counter()
         fe.findNeighbors_device(dXn)
-        reb.append((time.perf_counter() - t0) * 1e3)
+        skin_reb.append((time.perf_counter() - t0) * 1e3)
+        fe.upload(dXn, X)
+        fe.findNeighbors_device(dXn)  # back to the specimen for the timed evaluations
     fe.dev_free(dXn)
     nbr_dev_ms = float(np.median(reb))
-    log("%s neighbors: %.1f ms from host arrays, %.2f ms from device (median of 5), max %d, totals %s" % (
-        quad, nbr_ms, nbr_dev_ms, st["max_neib"], st["total"]))
+    nbr_skin_ms = float(np.median(skin_reb))
+    log("%s neighbors: %.1f ms first build from host arrays; rebuilds from device: %.2f ms filtered from the "
+        "skin lists (median of 5), %.2f ms with new skin lists (median of 3); max %d, totals %s" % (
+            quad, nbr_ms, nbr_dev_ms, nbr_skin_ms, st["max_neib"], st["total"]))
     if calib:
         nb = 256 << 20
         a, b = fe.dev_alloc(nb), fe.dev_alloc(nb)
@@ -502,6 +524,7 @@ def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False):
                           "gradient_rel_l2": float(np.linalg.norm(G - G64) / np.linalg.norm(G64))},
         "neighbor_build_ms": nbr_ms,
         "neighbor_rebuild_device_ms": nbr_dev_ms,
+        "neighbor_skin_rebuild_device_ms": nbr_skin_ms,
         "density_pass_flop": flop_dens,
         "wall_s_timed": wall,
     }

@@ -71,6 +71,8 @@ typedef struct a2c_nbr_stats {
   int32_t max_neib;       /* max neighbors of any site (findNeighbors' return) */
   int32_t width[5];       /* max row length of PdPd, PdH, HPd, HH, HH1 */
   int64_t total[5];       /* entries in PdPd, PdH, HPd, HH, HH1 */
+  int32_t refreshed;      /* 1: lists filtered from the context's skin lists (below) */
+  double max_displacement;  /* largest site displacement since the skin lists were built (A) */
 } a2c_nbr_stats;
 
 /* Fill a descriptor with the reference defaults (rc 5.35, eps 1e-5, R12,
@@ -91,7 +93,10 @@ int a2c_fe_set_occupancy(a2c_fe_ctx* ctx, const double* x);
 /* Replaces findNeighbors (neighbors.cpp:8-113) + Minimizer::findLocalNeighbors
  * (minimizer.cpp:833-1144): neighbor sets at rc from the positions in X
  * (sigma ignored), PdPdnum shell counts and the first H shell HH1.  Lists are
- * frozen until the next call. */
+ * frozen until the next call.  From the second call on a context also keeps
+ * lists at rc + skin (skin 0.2 A; environment A2C_NBR_SKIN, 0 to disable)
+ * and, while no site has moved by skin/2 since they were built, filters them
+ * instead of walking the cells: the same sets, rows and shell counts. */
 int a2c_fe_build_neighbors(a2c_fe_ctx* ctx, const double* X, a2c_nbr_stats* stats);
 /* The same from device-resident positions (dX: 4N doubles in HBM, e.g. the
  * state an optimizer left there); the context state becomes dX. */
