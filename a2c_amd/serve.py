@@ -3,17 +3,30 @@
 Requests (a lattice to relax) arrive as a Poisson stream.  The loop keeps
 every admitted lattice resident in HBM in one a2c_batch and advances running
 relaxations together, one LMVM iteration per tick (one fused launch per
-evaluation round for all lattices stepped).  Which lattices a tick steps is
-decided by stride scheduling: each lattice carries a virtual time that
-advances by the cost of its iteration (ITER_FIXED_S + ITER_SITE_S x sites,
-measured on one MI355X) whenever it is stepped, and a tick steps every
-lattice within one smallest-iteration cost of the least virtual time.  Each
-request thus receives about equal GPU time, so a small request does not wait
-for the iterations of much larger ones and no request starves (a lattice's
-trajectory is the same whichever lattices share its steps).  Between ticks
-the loop admits the requests that have arrived, as long as their device
-footprint fits the HBM budget, and retires the converged ones.  Latency =
-finish - arrival.
+evaluation round for all lattices stepped).  Between ticks the loop admits
+the requests that have arrived, as long as their device footprint fits the
+HBM budget, and retires the converged ones.  Latency = finish - arrival.
+
+Which lattices a tick steps is the policy's choice (due_set):
+
+* "deadline" (default): earliest deadline first, the deadline of a request
+  being its arrival plus DEADLINE_SLACK times an estimate of its relaxation
+  alone (EST_ITERS iterations at its iteration cost).  A tick steps the most
+  urgent lattice and, behind it in deadline order, every lattice that fits a
+  tick budget of (1 + RIDE_ALONG) times the head's iteration cost (at least
+  TICK_MIN_S); the fixed part of an iteration (launches, host round trips)
+  is paid once per tick, so a lattice that rides along costs only its
+  per-site part.  Small requests finish early, and a large one is not
+  overtaken by later arrivals once its deadline is near.
+* "fcfs": the same with arrival order (deadline slack 0).
+* "fair": stride scheduling: each lattice's virtual time advances by its
+  iteration cost whenever it is stepped, and a tick steps every lattice
+  within one smallest-iteration cost of the least virtual time (each request
+  receives about equal GPU time).
+* "lockstep": every running lattice on every tick.
+
+A lattice's trajectory is the same whichever lattices share its ticks, so a
+policy changes only when results arrive, not what they are.
 
 Workload (defaults): edge n log-uniform in [6, 48] cells (n^3 cells, 8 sites
 per cell), T ~ U(300, 700) K, mean H fraction ~ U(0, 0.6) with iid noise
@@ -69,13 +82,51 @@ def iter_cost(sites):
     return ITER_FIXED_S + ITER_SITE_S * sites
 
 
-def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None, fair=True,
+POLICIES = ("deadline", "fcfs", "fair", "lockstep")
+DEADLINE_SLACK = 3.0  # deadline = arrival + slack x estimated relaxation alone
+EST_ITERS = 150  # LMVM iterations of a typical request of this workload
+RIDE_ALONG = 0.5  # tick budget: (1 + RIDE_ALONG) x the head's iteration cost ...
+TICK_MIN_S = 5e-3  # ... and at least this
+
+
+def deadline(req, slack=DEADLINE_SLACK):
+    return req["arrival"] + slack * EST_ITERS * iter_cost(8 * req["n"] ** 3)
+
+
+def due_set(policy, running, vt):
+    """Slots one tick steps.  running: slot -> request (with "arrival" and
+    "n"); vt: slot -> virtual time, advanced here for "fair"."""
+    cost = {sl: iter_cost(8 * r["n"] ** 3) for sl, r in running.items()}
+    if policy == "lockstep":
+        return list(running)
+    if policy == "fair":
+        horizon = min(vt[sl] for sl in running) + min(cost.values())
+        due = [sl for sl in running if vt[sl] <= horizon]
+        for sl in due:
+            vt[sl] += cost[sl]
+        return due
+    if policy not in ("deadline", "fcfs"):
+        raise ValueError("unknown policy %r (one of %s)" % (policy, ", ".join(POLICIES)))
+    slack = DEADLINE_SLACK if policy == "deadline" else 0.0
+    order = sorted(running, key=lambda sl: (deadline(running[sl], slack), sl))
+    cap = max((1.0 + RIDE_ALONG) * cost[order[0]], TICK_MIN_S)
+    due, acc = [], ITER_FIXED_S
+    for sl in order:
+        c = cost[sl] - ITER_FIXED_S  # the fixed part is the tick's, paid once
+        if not due or acc + c <= cap:
+            due.append(sl)
+            acc += c
+    return due
+
+
+def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None, policy="deadline",
           keep_state=False):
-    """fair: stride scheduling by iteration cost; else every running lattice
-    is stepped on every tick (lockstep).  keep_state: each finished request
-    also carries its relaxed state ("X")."""
+    """Serve reqs (sorted by arrival) under one of POLICIES.  keep_state:
+    each finished request also carries its relaxed state ("X")."""
+    if policy not in POLICIES:
+        raise ValueError("unknown policy %r (one of %s)" % (policy, ", ".join(POLICIES)))
     opts = opts or {"gatol": 1e-8, "grtol": 0.0}
-    vt = {}  # slot -> virtual time (seconds of GPU time received)
+    vt = {}  # slot -> virtual time (seconds of GPU time received; "fair")
     pending = list(reqs)
     running = {}  # slot -> request
     done = []
@@ -99,15 +150,7 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
             if not running:
                 time.sleep(max(0.0, pending[0]["arrival"] - (time.perf_counter() - t0)))
                 continue
-            if fair:
-                cost = {sl: iter_cost(8 * r["n"] ** 3) for sl, r in running.items()}
-                horizon = min(vt[sl] for sl in running) + min(cost.values())
-                due = [sl for sl in running if vt[sl] <= horizon]
-                B.step_slots(due, 1)
-                for sl in due:
-                    vt[sl] += cost[sl]
-            else:
-                B.step(1)
+            B.step_slots(due_set(policy, running, vt), 1)
             for slot in list(running):
                 rep = B.report(slot)
                 if rep["reason"] != 0:
@@ -174,19 +217,19 @@ def fifo_latencies(arrivals, service):
     return np.array(lat)
 
 
-def sweep(rates, n_requests, fp64_sample=40, log=None, opts=None):
+def sweep(rates, n_requests, fp64_sample=None, log=None, opts=None, policy="deadline"):
     """Serve the same n_requests (sizes, temperatures, occupancies) at each
     Poisson rate with continuous batching (mixed precision, FP64 endgame).
-    The comparator relaxes the first fp64_sample requests one at a time in
-    FP64, back to back: their mean service time gives its capacity, and a
-    first-come-first-served replay of those measured service times (request
-    i takes the time of sample request i mod fp64_sample) on each rate's
+    The comparator relaxes the requests one at a time in FP64, back to back
+    (all of them, or the first fp64_sample): the mean service time gives its
+    capacity, and, when every request was relaxed, a first-come-first-served
+    replay of each request's own measured service time on each rate's
     arrivals gives its p50/p99.  relaxed_f_rel_vs_fp64: the largest relative
-    difference of the served objectives from the FP64 ones over the sample."""
+    difference of the served objectives from the FP64 ones."""
     opts = opts or {"gatol": 1e-8, "grtol": 0.0}
     base = make_requests(n_requests, 1.0)
     states = [materialize(r) for r in base]  # the clients' payloads, built before the clock starts
-    k = min(fp64_sample, n_requests)
+    k = n_requests if fp64_sample is None else min(fp64_sample, n_requests)
     sv = service_times([dict(r, state=st) for r, st in zip(base[:k], states[:k])], opts=opts)
     f64 = [rep["f"] for _, rep in sv]
     svt = np.array([t for t, _ in sv])
@@ -195,22 +238,22 @@ def sweep(rates, n_requests, fp64_sample=40, log=None, opts=None):
         reqs = make_requests(n_requests, rate)
         for r, st in zip(reqs, states):
             r["state"] = st
-        done, wall = serve(reqs, opts=opts, log=log)
+        done, wall = serve(reqs, opts=opts, log=log, policy=policy)
         row = {"rate": rate, "batched_mixed": summarize(done, wall)}
         byid = {d["id"]: d for d in done}
         row["relaxed_f_rel_vs_fp64"] = max(abs(byid[i]["report"]["f"] - f64[i]) / abs(f64[i]) for i in range(k))
-        lat = fifo_latencies([r["arrival"] for r in reqs], [svt[i % k] for i in range(n_requests)])
-        row["one_at_a_time_fp64_replay"] = {"p50_s": float(np.percentile(lat, 50)),
-                                            "p99_s": float(np.percentile(lat, 99))}
+        if k == n_requests:
+            lat = fifo_latencies([r["arrival"] for r in reqs], svt)
+            row["one_at_a_time_fp64"] = {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99))}
         rows.append(row)
         if log:
             log({"rate": rate, "summary": row["batched_mixed"]})
     sites = [8 * r["n"] ** 3 for r in base]
     return {
         "requests_per_rate": n_requests, "sites_mean": float(np.mean(sites)), "sites_max": int(max(sites)),
-        "rates": rows,
+        "policy": policy, "rates": rows,
         "saturation_relaxations_per_s": max(r["batched_mixed"]["relaxations_per_s"] for r in rows),
-        "one_at_a_time_fp64": {"sample": k, "service_s_mean": float(svt.mean()),
+        "one_at_a_time_fp64": {"relaxed": k, "service_s_mean": float(svt.mean()),
                                "relaxations_per_s": float(1.0 / svt.mean()),
                                "converged": sum(1 for _, rep in sv if rep["reason"] > 0)},
     }
@@ -228,12 +271,13 @@ def main():
     p = argparse.ArgumentParser()
     p.add_argument("--rates", default="5,10,20,40")
     p.add_argument("--requests", type=int, default=200)
-    p.add_argument("--fp64-sample", type=int, default=40)
+    p.add_argument("--fp64-sample", type=int, default=None, help="requests relaxed in FP64 (default: all)")
+    p.add_argument("--policy", default="deadline", choices=POLICIES)
     a = p.parse_args()
     note = lambda r: print(json.dumps(r) if "summary" in r else
                            "request %d (n=%d) done, latency %.3f s, %d iterations"
                            % (r["id"], r["n"], r["latency"], r["report"]["iterations"]), file=sys.stderr, flush=True)
-    out = sweep([float(r) for r in a.rates.split(",")], a.requests, a.fp64_sample, log=note)
+    out = sweep([float(r) for r in a.rates.split(",")], a.requests, a.fp64_sample, log=note, policy=a.policy)
     print(json.dumps(out), flush=True)
     return out
 

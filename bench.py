@@ -62,8 +62,10 @@ def parse():
     p.add_argument("--no-serving", action="store_true", help="single: skip the serving block")
     p.add_argument("--serve-rates", default="5,10,20,40", help="serving: Poisson request rates (req/s)")
     p.add_argument("--serve-requests", type=int, default=200, help="serving: requests per rate")
-    p.add_argument("--serve-fp64-sample", type=int, default=40,
-                   help="serving: requests relaxed one at a time in FP64 (the comparator)")
+    p.add_argument("--serve-fp64-sample", type=int, default=None,
+                   help="serving: requests relaxed one at a time in FP64 (the comparator; default all)")
+    p.add_argument("--serve-policy", default="deadline", choices=("deadline", "fcfs", "fair", "lockstep"),
+                   help="serving: which running lattices a tick steps (a2c_amd.serve)")
     p.add_argument("--lmvm-m", type=int, default=5, help="batched: stored pairs of the LMVM (TAO's default 5)")
     return p.parse_args()
 
@@ -348,11 +350,13 @@ def serving_block(args):
             sm = r["summary"]
             log("serving %g req/s: %.2f relaxations/s, p50 %.3f s, p99 %.3f s" % (
                 r["rate"], sm["relaxations_per_s"], sm["p50_s"], sm["p99_s"]))
-    out = serve.sweep(rates, args.serve_requests, args.serve_fp64_sample, log=note)
+    out = serve.sweep(rates, args.serve_requests, args.serve_fp64_sample, log=note, policy=args.serve_policy)
     out["metric"] = "relaxations/s + p50/p99 latency"
     out["config"] = ("Poisson arrivals, edge log-uniform [6,48] cells, T U(300,700) K, x U(0,0.6)+-0.05, "
                      "disp N(0,0.02), sigma 0.10, periodic; continuous batching (admission by HBM footprint, "
-                     "stride scheduling), mixed precision with the FP64 endgame, ||Pg||<=1e-8")
+                     "'%s' scheduling), mixed precision with the FP64 endgame, ||Pg||<=1e-8; comparator: the same "
+                     "requests relaxed one at a time in FP64, replayed first-come-first-served on each rate's "
+                     "arrivals" % args.serve_policy)
     return out
 
 

@@ -10,11 +10,12 @@ from a2c_amd import serve
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("fair", [True, False])
-def test_served_results_equal_standalone(fair):
+@pytest.mark.parametrize("policy", serve.POLICIES)
+def test_served_results_equal_standalone(policy):
     reqs = serve.make_requests(6, rate=50.0, nmin=3, nmax=6)
     opts = {"gatol": 1e-8, "grtol": 0.0}
-    done, wall = serve.serve([dict(r) for r in reqs], precision="mixed", opts=opts, fair=fair)
+    print("serving 6 requests, policy", policy, flush=True)
+    done, wall = serve.serve([dict(r) for r in reqs], precision="mixed", opts=opts, policy=policy)
     assert len(done) == len(reqs)
     overlap = sum(1 for d in done if any(o["admitted"] < d["admitted"] < o["finish"] for o in done if o is not d))
     assert overlap > 0  # some request was admitted while another was running
