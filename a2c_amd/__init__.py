@@ -197,6 +197,7 @@ def _load():
         "a2c_fe_sync": (I, [VP]),
         "a2c_dev_copy": (I, [VP, VP, VP, C.c_size_t]),
         "a2c_fe_time_eval": (I, [VP, VP, VP, VP, C.c_int32, P(Timing)]),
+        "a2c_fe_time_diffuse": (I, [VP, D, D, D, D, D, VP, VP, VP, C.c_int32, P(Timing)]),
         "a2c_lattice_desc_default": (None, [P(LatticeDesc)]),
         "a2c_lattice_fcc_pdh": (I, [P(LatticeDesc), dp, dp, dp]),
         "a2c_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
@@ -538,6 +539,15 @@ class FreeEntropy:
         t = Timing()
         self._check(lib.a2c_fe_time_eval(self._ctx, C.c_void_p(dX), C.c_void_p(dG), C.c_void_p(df), iters,
                                          C.byref(t)))
+        kern = {t.names[k].value.decode(): t.avg_ms[k] for k in range(t.n_kernels)}
+        return {"kernels_ms": kern, "total_ms": t.total_ms}
+
+    def time_diffuse(self, dt, nu, Qm, xLb, xUb, d_gamma, d_full_h, d_x, iters):
+        """iters diffusion steps on device arrays, each from the context
+        occupancy (unchanged), timed per kernel with HIP events."""
+        t = Timing()
+        self._check(lib.a2c_fe_time_diffuse(self._ctx, dt, nu, Qm, xLb, xUb, C.c_void_p(d_gamma),
+                                            C.c_void_p(d_full_h), C.c_void_p(d_x), iters, C.byref(t)))
         kern = {t.names[k].value.decode(): t.avg_ms[k] for k in range(t.n_kernels)}
         return {"kernels_ms": kern, "total_ms": t.total_ms}
 

@@ -401,7 +401,8 @@ struct DiffuseScratch {
 };
 int diffuse_blocks(int nh);
 int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb,
-                   const double* dgam, const int32_t* dfull, const DiffuseScratch& w, double* dxnew);
+                   const double* dgam, const int32_t* dfull, const DiffuseScratch& w, double* dxnew,
+                   Timing* tm = nullptr);
 
 // Event timing of individual kernels on the context stream: mark() records
 // the next preallocated event; durations are differences of consecutive marks.

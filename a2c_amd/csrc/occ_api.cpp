@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "fe_internal.hpp"
@@ -86,5 +87,66 @@ extern "C" int a2c_fe_diffuse(a2c_fe_ctx* c, double dt, double nu, double Qm, do
 extern "C" int a2c_fe_diffusion_balance(const a2c_fe_ctx* c, a2c_diffusion_balance* out) {
   if (!c || !out) return A2C_EINVAL;
   *out = c->last_balance;
+  return A2C_OK;
+}
+
+extern "C" int a2c_fe_time_diffuse(a2c_fe_ctx* c, double dt, double nu, double Qm, double xLb, double xUb,
+                                   const double* d_gamma, const int32_t* d_full_h, double* d_x, int32_t iters,
+                                   a2c_timing* out) {
+  if (!c) return A2C_EINVAL;
+  c->err.clear();
+  if (!c->have_nbr || !c->have_x) return set_err(c, A2C_ESTATE, "neighbors/occupancy not set");
+  const int nh = c->desc.n_h;
+  if (!out || iters <= 0 || iters > 100000 || (nh && (!d_gamma || !d_full_h || !d_x)))
+    return set_err(c, A2C_EINVAL, "bad timing args");
+  if (d_x == c->dx) return set_err(c, A2C_EINVAL, "d_x must not be the context occupancy");
+  if (!(xLb <= xUb)) return set_err(c, A2C_EINVAL, "xLb > xUb");
+  std::memset(out, 0, sizeof(*out));
+  if (nh == 0) return A2C_OK;
+  if (int rc = occ_scratch(c)) return rc;
+  const double pref = nu * std::exp(-Qm / (KB * c->desc.temperature));  // integrate.cpp:26
+  const DiffuseScratch w{c->occ_f, c->occ_part, c->occ_bal};
+  constexpr int per = 4;  // marks per step: three kernels and the end
+  std::vector<hipEvent_t> ev((size_t)per * iters + 2);
+  for (auto& e : ev) A2C_HIP(c, hipEventCreate(&e));
+  struct Destroy {
+    std::vector<hipEvent_t>& v;
+    ~Destroy() {
+      for (auto e : v) hipEventDestroy(e);
+    }
+  } ds{ev};
+  Timing tm;
+  const char* names[8] = {};
+  A2C_HIP(c, hipEventRecord(ev[0], c->stream));
+  for (int it = 0; it < iters; ++it) {
+    tm.ev = ev.data() + 1 + (size_t)per * it;
+    tm.cap = per;
+    tm.n = 0;
+    tm.on = true;
+    if (int rc = launch_diffuse(c, dt, pref, xLb, xUb, d_gamma, d_full_h, w, d_x, &tm)) return rc;
+    if (tm.n != per) return set_err(c, A2C_EHIP, "diffusion timing marks");
+    if (it == 0)
+      for (int k = 0; k < per - 1; ++k) names[k] = tm.name[k];
+  }
+  hipEvent_t tend = ev[1 + (size_t)per * iters];
+  A2C_HIP(c, hipEventRecord(tend, c->stream));
+  A2C_HIP(c, hipEventSynchronize(tend));
+  float tot = 0.f;
+  A2C_HIP(c, hipEventElapsedTime(&tot, ev[0], tend));
+  double acc[per - 1] = {0};
+  for (int it = 0; it < iters; ++it) {
+    hipEvent_t* e = ev.data() + 1 + (size_t)per * it;
+    for (int k = 0; k < per - 1; ++k) {
+      float ms = 0.f;
+      A2C_HIP(c, hipEventElapsedTime(&ms, e[k], e[k + 1]));
+      acc[k] += ms;
+    }
+  }
+  out->n_kernels = per - 1;
+  for (int k = 0; k < per - 1; ++k) {
+    std::strncpy(out->names[k], names[k] ? names[k] : "?", 31);
+    out->avg_ms[k] = acc[k] / iters;
+  }
+  out->total_ms = tot;
   return A2C_OK;
 }

@@ -260,17 +260,21 @@ int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const dou
 int diffuse_blocks(int nh) { return (nh + TB - 1) / TB; }
 
 int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, double xUb, const double* dgam,
-                   const int32_t* dfull, const DiffuseScratch& w, double* dxnew) {
+                   const int32_t* dfull, const DiffuseScratch& w, double* dxnew, Timing* tm) {
   const int nh = c->desc.n_h;
   if (nh == 0) return A2C_OK;
   const int nblk = diffuse_blocks(nh);
+  if (tm) tm->mark(c->stream, "diff_f");
   hipLaunchKernelGGL(k_diff_f, dim3(nblk), dim3(TB), 0, c->stream, nh, dgam, c->dx, w.f);
   A2C_HIP(c, hipGetLastError());
+  if (tm) tm->mark(c->stream, "diff_step");
   hipLaunchKernelGGL(k_diff_step, dim3(nblk), dim3(TB), 0, c->stream, c->ell[4], c->dx, w.f, dfull, prefactor, dt,
                      xLb, xUb, dxnew, w.part);
   A2C_HIP(c, hipGetLastError());
+  if (tm) tm->mark(c->stream, "diff_balance");
   hipLaunchKernelGGL(k_diff_balance, dim3(1), dim3(TB), 0, c->stream, w.part, nblk, w.bal);
   A2C_HIP(c, hipGetLastError());
+  if (tm) tm->mark(c->stream, nullptr);
   return A2C_OK;
 }
 

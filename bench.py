@@ -298,6 +298,9 @@ def dmd(args):
             st, r["f"], r["lmvm_iterations"], r["lmvm_reason"], r["total_h_after"] - r["total_h_before"],
             r["clamp_lift"], r["flux_sum"], r["flux_gross"], sg.min(), sg.max()))
     wall = time.perf_counter() - t0
+    # the diffusion kernels alone at this size and the last step's occupancy
+    diff_kernels = diffusion_timing(fe, x, nh)
+    diff_ms = sum(diff_kernels.values())
     fe.close()
     dH = [r["total_h_after"] - r["total_h_before"] for r in reps]
     flux_rel = [abs(r["flux_sum"] - r["reservoir_supply"]) / r["flux_gross"] for r in reps if r["flux_gross"] > 0]
@@ -323,6 +326,9 @@ def dmd(args):
         "gatol": lmvm["gatol"], "sigma_ub": lmvm["sigma_ub"],
         "lmvm_iterations": [r["lmvm_iterations"] for r in reps],
         "lmvm_reasons": sorted(set(r["lmvm_reason"] for r in reps)),
+        "diffusion_hbm": {"bytes": 76.0 * nh, "ms": diff_ms, "GB/s": 76.0 * nh / (diff_ms * 1e-3) / 1e9,
+                          "frac": 76.0 * nh / (diff_ms * 1e-3) / 8e12, "kernels_ms": diff_kernels,
+                          "note": "76 B per H site (x and gamma, 12 HH1 indices, new x, full_h) against 8 TB/s"},
     }
 
 
@@ -384,7 +390,7 @@ def main():
 
     X, x, box, npd, nh = mk(args.cells)
     log("workload: %d cells^3, %d sites" % (args.cells, npd + nh))
-    m = measure(A, args, X, x, box, npd, nh, args.quad, args.steps, args.warmup, calib=args.calib)
+    m = measure(A, args, X, x, box, npd, nh, args.quad, args.steps, args.warmup, calib=args.calib, paths=True)
     N = npd + nh
     result = {
         "metric": "free-entropy+grad atom-evals/s",
@@ -412,6 +418,7 @@ def main():
         "neighbor_skin_rebuild_device_ms": m["neighbor_skin_rebuild_device_ms"],
         "density_pass_flop": m["density_pass_flop"],
         "wall_s_timed": m["wall_s_timed"],
+        "hbm_paths": m["hbm_paths"],
     }
     if args.quad == "r12" and not args.no_gh3:
         # the other rule on the same lattice, reported beside the headline
@@ -431,10 +438,50 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False):
+def diffusion_timing(fe, x, nh, iters=50):
+    """The diffusion step of this lattice's HH1 graph on device arrays, HIP
+    events per kernel: gamma near the equilibrium of the occupancy
+    (ln(x/(1-x)) + N(0, 0.05)), no reservoirs, the DMD workload's rate and
+    time step."""
+    dgam, dfull, dxo = fe.dev_alloc(8 * nh), fe.dev_alloc(4 * nh), fe.dev_alloc(8 * nh)
+    try:
+        xc = np.clip(x, 1e-12, 1.0 - 1e-12)
+        gam = np.log(xc / (1.0 - xc)) + np.random.default_rng(12).normal(0.0, 0.05, nh)
+        fe.upload(dgam, gam)
+        fe.upload(dfull, np.zeros(nh, np.int32))
+        args = (5e-13, 1e13, 0.23, 1e-3, 1.0 - 1e-16, dgam, dfull, dxo)
+        fe.time_diffuse(*args, 3)  # warm-up
+        tm = fe.time_diffuse(*args, iters)
+    finally:
+        for p in (dgam, dfull, dxo):
+            fe.dev_free(p)
+    return tm["kernels_ms"]
+
+
+def hbm_paths(N, nh, P, eval_ms, nbr_ms, diff_kernels):
+    """Algorithmic HBM bytes of the three paths against 8 TB/s: evaluation
+    8 B x P (two passes over 4-byte neighbor indices) + 56 B x N + 4 B x nH;
+    neighbor build 24 B x N + 4 B x P + 4 B x (N + 1); diffusion step 76 B x
+    nH (x and gamma, 12 HH1 indices, new x, full_h).  P = directed pairs."""
+    diff_ms = sum(diff_kernels.values())
+    rows = {"evaluation": (8.0 * P + 56.0 * N + 4.0 * nh, eval_ms, "whole evaluation (HIP events)"),
+            "neighbor_rebuild": (24.0 * N + 4.0 * P + 4.0 * (N + 1), nbr_ms,
+                                 "filtered rebuild from device positions, wall with its one host round trip"),
+            "diffusion": (76.0 * nh, diff_ms, "k_diff_f + k_diff_step + k_diff_balance (HIP events)")}
+    out = {}
+    for k, (b, ms, what) in rows.items():
+        gbs = b / (ms * 1e-3) / 1e9
+        out[k] = {"bytes": b, "ms": ms, "GB/s": gbs, "frac": gbs / 8000.0, "timed": what}
+    out["diffusion"]["kernels_ms"] = diff_kernels
+    return out
+
+
+def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False, paths=False):
     """One rule on one lattice: the FP64 kernel (comparator), then the
     measured precision; returns throughput, per-kernel times, roofline of
-    the gradient owner kernel and the error against the FP64 kernel."""
+    the gradient owner kernel and the error against the FP64 kernel (and,
+    with paths, the HBM fractions of evaluation, neighbor build and
+    diffusion step)."""
     N = npd + nh
     fe = A.FreeEntropy(npd, nh, box, (1, 1, 1), T=500.0, precision="fp64", quad=quad)
     fe.set_occupancy(x)
@@ -474,6 +521,7 @@ def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False):
         fe.upload(dXn, X)
         fe.findNeighbors_device(dXn)  # back to the specimen for the timed evaluations
     fe.dev_free(dXn)
+    diff_kernels = diffusion_timing(fe, x, nh) if paths else None
     nbr_dev_ms = float(np.median(reb))
     nbr_skin_ms = float(np.median(skin_reb))
     log("%s neighbors: %.1f ms first build from host arrays; rebuilds from device: %.2f ms filtered from the "
@@ -531,6 +579,8 @@ def measure(A, args, X, x, box, npd, nh, quad, steps, warmup, calib=False):
         "neighbor_skin_rebuild_device_ms": nbr_skin_ms,
         "density_pass_flop": flop_dens,
         "wall_s_timed": wall,
+        "hbm_paths": hbm_paths(N, nh, float(sum(st["total"])), tm["total_ms"] / steps, nbr_dev_ms, diff_kernels)
+        if paths else None,
     }
 
 

@@ -304,6 +304,15 @@ typedef struct a2c_timing {
 int a2c_fe_time_eval(a2c_fe_ctx* ctx, const double* dX, double* dG, double* df,
                      int32_t iters, a2c_timing* out);
 
+/* No counterpart in the reference (measurement).  Runs `iters` diffusion
+ * steps of a2c_fe_diffuse on device-resident arrays (d_gamma and d_full_h
+ * n_h each; d_x, n_h doubles and not the context occupancy, receives the new
+ * occupancy), each from the context occupancy, which stays unchanged; times
+ * each kernel with HIP events recorded on the context stream. */
+int a2c_fe_time_diffuse(a2c_fe_ctx* ctx, double dt, double nu, double Qm, double xLb, double xUb,
+                        const double* d_gamma, const int32_t* d_full_h, double* d_x, int32_t iters,
+                        a2c_timing* out);
+
 /* Host-only (no GPU): the FP32 EAM tables of the mixed path, 5 tables
  * (f_Pd, f_H, phi_Pd, phi_PdH, phi_H) x 500 intervals x 4 cubic coefficients;
  * interval k covers r in [1 + 0.01k, 1 + 0.01(k+1)) A.  Returns the float count

@@ -103,3 +103,31 @@ def test_diffusion_conserves_total_h_at_size():
     b = fe.diffusion_balance()
     assert b["clamp_lift"] == 0.0 and b["flux_gross"] > 0
     assert abs(b["flux_sum"]) <= 1e-12 * b["flux_gross"]
+
+
+def test_timed_diffusion_equals_a_step_and_keeps_the_occupancy():
+    """a2c_fe_time_diffuse: every timed step starts from the context
+    occupancy, writes what a2c_fe_diffuse returns (bitwise) and leaves the
+    context occupancy as it was."""
+    fe, orc, X, x = setup()
+    nh = fe.n_h
+    rng = np.random.default_rng(2)
+    gam = rng.normal(0.0, 0.8, nh)
+    full = np.zeros(nh, np.int32)
+    full[::7] = 1
+    args = (1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16)
+    print("timed diffusion: %d H sites" % nh, flush=True)
+    dg, df, dx = fe.dev_alloc(8 * nh), fe.dev_alloc(4 * nh), fe.dev_alloc(8 * nh)
+    try:
+        fe.upload(dg, gam)
+        fe.upload(df, full)
+        tm = fe.time_diffuse(*args, dg, df, dx, 4)
+        xt = fe.download(np.zeros(nh), dx)
+    finally:
+        for p in (dg, df, dx):
+            fe.dev_free(p)
+    assert list(tm["kernels_ms"]) == ["diff_f", "diff_step", "diff_balance"]
+    assert all(v > 0 for v in tm["kernels_ms"].values()) and tm["total_ms"] > 0
+    xs = fe.integrateDiffusion(*args, gam, full)  # from the unchanged occupancy x
+    assert np.array_equal(xt, xs)
+    assert np.max(np.abs(xs - orc.diffuse(*args, gam, full, x))) <= 1e-14
