@@ -138,7 +138,8 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   }
   dfree(c->nb_sup_q);
   dfree(c->depair);
-  dfree(c->occ_g); dfree(c->occ_full); dfree(c->occ_corr); dfree(c->occ_f); dfree(c->occ_xn);
+  dfree(c->occ_g); dfree(c->occ_full); dfree(c->occ_corr); dfree(c->occ_uv);
+  dfree(c->occ_xn);
   dfree(c->occ_part); dfree(c->occ_bal);
   dfree(c->nb_cid); dfree(c->nb_order); dfree(c->nb_cc); dfree(c->nb_cursor); dfree(c->nb_misc);
   if (c->hX) hfree(c->hX);

@@ -291,7 +291,7 @@ struct a2c_fe_ctx {
   double* occ_g = nullptr;     // nH gamma
   int32_t* occ_full = nullptr; // nH full_H
   double* occ_corr = nullptr;  // N correction sums
-  double* occ_f = nullptr;     // nH f = gamma - ln(x/(1-x))
+  double* occ_uv = nullptr;    // 2 nH edge-term factors of the diffusion step
   double* occ_xn = nullptr;    // nH new occupancy
   double* occ_part = nullptr;  // 4 per block of the balance
   double* occ_bal = nullptr;   // 4: the balance of the last step (a2c_diffusion_balance)
@@ -395,7 +395,7 @@ int launch_chem(a2c_fe_ctx* c, const double* dX, const double* coe_pd, const dou
 // Scratch of one diffusion step: f (n_h doubles), block partials of the
 // balance (4 per block of 256 H sites) and the balance itself (4 doubles).
 struct DiffuseScratch {
-  double* f;
+  double* uv;   // 2 nH: (x e^{f/2}, (1-x) e^{-f/2}) per site, f = gamma - ln(x/(1-x)); NaN where |f| >= 700
   double* part;
   double* bal;
 };

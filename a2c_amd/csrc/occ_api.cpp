@@ -21,7 +21,7 @@ int occ_scratch(a2c_fe_ctx* c) {
   A2C_HIP(c, dmalloc(&c->occ_g, 8 * nh));
   A2C_HIP(c, dmalloc(&c->occ_full, 4 * nh));
   A2C_HIP(c, dmalloc(&c->occ_corr, 8 * N));
-  A2C_HIP(c, dmalloc(&c->occ_f, 8 * nh));
+  A2C_HIP(c, dmalloc(&c->occ_uv, 16 * nh));
   A2C_HIP(c, dmalloc(&c->occ_xn, 8 * nh));
   A2C_HIP(c, dmalloc(&c->occ_part, 8 * 4 * nblk));
   A2C_HIP(c, dmalloc(&c->occ_bal, 8 * 4));
@@ -73,7 +73,7 @@ extern "C" int a2c_fe_diffuse(a2c_fe_ctx* c, double dt, double nu, double Qm, do
   int32_t* dfull = c->occ_full;
   A2C_HIP(c, hipMemcpyAsync(dg, gamma, 8 * nh, hipMemcpyHostToDevice, c->stream));
   A2C_HIP(c, hipMemcpyAsync(dfull, full_h, 4 * nh, hipMemcpyHostToDevice, c->stream));
-  const DiffuseScratch w{c->occ_f, c->occ_part, c->occ_bal};
+  const DiffuseScratch w{c->occ_uv, c->occ_part, c->occ_bal};
   if (int rc = launch_diffuse(c, dt, pref, xLb, xUb, dg, dfull, w, dxn)) return rc;
   A2C_HIP(c, hipMemcpyAsync(c->dx, dxn, 8 * nh, hipMemcpyDeviceToDevice, c->stream));
   A2C_HIP(c, hipMemcpyAsync(x, dxn, 8 * nh, hipMemcpyDeviceToHost, c->stream));
@@ -105,7 +105,7 @@ extern "C" int a2c_fe_time_diffuse(a2c_fe_ctx* c, double dt, double nu, double Q
   if (nh == 0) return A2C_OK;
   if (int rc = occ_scratch(c)) return rc;
   const double pref = nu * std::exp(-Qm / (KB * c->desc.temperature));  // integrate.cpp:26
-  const DiffuseScratch w{c->occ_f, c->occ_part, c->occ_bal};
+  const DiffuseScratch w{c->occ_uv, c->occ_part, c->occ_bal};
   constexpr int per = 4;  // marks per step: three kernels and the end
   std::vector<hipEvent_t> ev((size_t)per * iters + 2);
   for (auto& e : ev) A2C_HIP(c, hipEventCreate(&e));

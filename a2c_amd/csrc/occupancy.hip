@@ -17,12 +17,16 @@
 // (:721-722, :793-800); full_H sites keep gamma.  The pair sums sum_k w use
 // the context's quadrature rule (12-point, w = 1/12, or Gauss-Hermite).
 //
-// Diffusion: f_a = gamma_a - ln(x_a/(1-x_a)) once per site (k_diff_f), then
+// Diffusion: f_a = gamma_a - ln(x_a/(1-x_a)) (k_diff_f, once per site), then
 // x_a += (rate x_deriv) dt with x_deriv = sum over HH1(a) of
-// x_b(1-x_a)e^{-(f_a-f_b)/2} - x_a(1-x_b)e^{(f_a-f_b)/2} (k_diff_step).  The
-// edge term is evaluated from either end with the same operands, so it is
-// exactly antisymmetric; the total changes only by the rounding of the site
-// sums (plus clamps and full_H reservoirs, as in the reference).
+// x_b(1-x_a)e^{-(f_a-f_b)/2} - x_a(1-x_b)e^{(f_a-f_b)/2} (k_diff_step).  With
+// h = e^{f/2}, U = x h and V = (1-x)/h per site (one exp per site in
+// k_diff_f) the edge term is V_a U_b - U_a V_b: no exp per edge.  A site with
+// |f| >= 700 (h near the end of the double range) has U = V = NaN, and its
+// edges take the reference's form.  Either form is evaluated from both ends
+// with the same operands and without contraction, so the edge term is exactly
+// antisymmetric; the total changes only by the rounding of the site sums (plus
+// clamps and full_H reservoirs, as in the reference).
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -153,11 +157,32 @@ __global__ __launch_bounds__(TB) void k_chem(DevSys S, const double* __restrict_
   gamma[a] = g;
 }
 
+constexpr double F_FACTOR_MAX = 700.0;  // |f| below which e^{+-f/2} and the edge products stay finite
+
+__device__ __forceinline__ double diff_f(double gamma, double x) { return gamma - log(x / (1.0 - x)); }
+
 __global__ void k_diff_f(int nh, const double* __restrict__ gamma, const double* __restrict__ xo,
-                         double* __restrict__ f) {
+                         double2* __restrict__ uv) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= nh) return;
-  f[a] = gamma[a] - log(xo[a] / (1.0 - xo[a]));
+  const double x = xo[a];
+  const double fa = diff_f(gamma[a], x);
+  double2 w = make_double2(NAN, NAN);
+  if (fabs(fa) < F_FACTOR_MAX) {  // false for NaN
+    const double h = exp(0.5 * fa);
+    w = make_double2(x * h, (1.0 - x) / h);
+  }
+  uv[a] = w;
+}
+
+// The reference's edge term (integrate.cpp:40-47) of (a, b), seen from a:
+// for the edges of sites whose factors are NaN (out of line: rare).
+__device__ __attribute__((noinline)) double edge_ref(const double* __restrict__ gamma,
+                                                    const double* __restrict__ xo, int a, int b) {
+#pragma clang fp contract(off)
+  const double xa = xo[a], xb = xo[b];
+  const double df = diff_f(gamma[a], xa) - diff_f(gamma[b], xb);
+  return xb * (1.0 - xa) * exp(-df / 2.0) - xa * (1.0 - xb) * exp(df / 2.0);
 }
 
 // One explicit step per H site, and per block the four balance sums of the
@@ -169,23 +194,36 @@ __global__ void k_diff_f(int nh, const double* __restrict__ gamma, const double*
 // applied increments equal the reservoirs' supply up to the rounding of the
 // site sums.
 __global__ __launch_bounds__(TB) void k_diff_step(Ell hh1, const double* __restrict__ xo,
-                                                  const double* __restrict__ f, const int32_t* __restrict__ full,
-                                                  double pref, double dt, double xLb, double xUb,
-                                                  double* __restrict__ xn, double* __restrict__ part) {
+                                                  const double* __restrict__ gamma, const double2* __restrict__ uv,
+                                                  const int32_t* __restrict__ full, double pref, double dt,
+                                                  double xLb, double xUb, double* __restrict__ xn,
+                                                  double* __restrict__ part) {
+#pragma clang fp contract(off)
   __shared__ double red[4][TB];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double inc = 0.0, gross = 0.0, lift = 0.0, supply = 0.0;
   if (a < hh1.rows) {
     const double xa = xo[a];
-    const double fa = f[a];
+    const double2 wa = uv[a];
     double xd = 0.0, xg = 0.0;
     const int n = hh1.cnt[a];
-    for (int t = 0; t < n; ++t) {
-      const int b = hh1.idx[(size_t)t * hh1.rows + a];
-      const double xb = xo[b], df = fa - f[b];
-      const double e = xb * (1.0 - xa) * exp(-df / 2.0) - xa * (1.0 - xb) * exp(df / 2.0);
-      xd += e;
-      xg += fabs(e);
+    const size_t rows = hh1.rows;
+    constexpr int B = 4;  // edges whose gathers are in flight together (8 or 12: slower at 64^3 cells)
+    for (int t0 = 0; t0 < n; t0 += B) {
+      int b[B];
+      double2 wb[B];
+#pragma unroll
+      for (int k = 0; k < B; ++k) b[k] = t0 + k < n ? hh1.idx[(size_t)(t0 + k) * rows + a] : -1;
+#pragma unroll
+      for (int k = 0; k < B; ++k) wb[k] = b[k] >= 0 ? uv[b[k]] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        if (b[k] < 0) continue;
+        const double e = (isnan(wa.x) || isnan(wb[k].x)) ? edge_ref(gamma, xo, a, b[k])
+                                                          : wa.y * wb[k].x - wa.x * wb[k].y;
+        xd += e;
+        xg += fabs(e);
+      }
     }
     gross = pref * xg * dt;
     if (full[a] == 1) {  // reservoir: keeps x (integrate.cpp:50, :62, :77)
@@ -215,18 +253,33 @@ __global__ __launch_bounds__(TB) void k_diff_step(Ell hh1, const double* __restr
   if (threadIdx.x < 4) part[4 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// The block sums of k_diff_step in block order (one workgroup): bal[0..3].
-__global__ __launch_bounds__(TB) void k_diff_balance(const double* __restrict__ part, int nblk,
+// The block sums of k_diff_step (one workgroup of BT lanes, each summing the
+// blocks b = lane + BT i in order, four blocks' loads in flight; then a fixed
+// tree): bal[0..3].
+constexpr int BT = 1024;
+__global__ __launch_bounds__(BT) void k_diff_balance(const double4* __restrict__ part, int nblk,
                                                      double* __restrict__ bal) {
-  __shared__ double red[4][TB];
+  __shared__ double red[4][BT];
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = threadIdx.x; b < nblk; b += TB)
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += 4 * BT) {
+    double4 p[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] += part[4 * (size_t)b + k];
+    for (int i = 0; i < 4; ++i) {
+      const int b = b0 + i * BT;
+      p[i] = b < nblk ? part[b] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[0] += p[i].x;
+      v[1] += p[i].y;
+      v[2] += p[i].z;
+      v[3] += p[i].w;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = v[k];
   __syncthreads();
-  for (int w = TB / 2; w > 0; w >>= 1) {
+  for (int w = BT / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w)
 #pragma unroll
       for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
@@ -265,14 +318,16 @@ int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, doubl
   if (nh == 0) return A2C_OK;
   const int nblk = diffuse_blocks(nh);
   if (tm) tm->mark(c->stream, "diff_f");
-  hipLaunchKernelGGL(k_diff_f, dim3(nblk), dim3(TB), 0, c->stream, nh, dgam, c->dx, w.f);
+  hipLaunchKernelGGL(k_diff_f, dim3(nblk), dim3(TB), 0, c->stream, nh, dgam, c->dx,
+                     reinterpret_cast<double2*>(w.uv));
   A2C_HIP(c, hipGetLastError());
   if (tm) tm->mark(c->stream, "diff_step");
-  hipLaunchKernelGGL(k_diff_step, dim3(nblk), dim3(TB), 0, c->stream, c->ell[4], c->dx, w.f, dfull, prefactor, dt,
-                     xLb, xUb, dxnew, w.part);
+  hipLaunchKernelGGL(k_diff_step, dim3(nblk), dim3(TB), 0, c->stream, c->ell[4], c->dx, dgam,
+                     reinterpret_cast<const double2*>(w.uv), dfull, prefactor, dt, xLb, xUb, dxnew, w.part);
   A2C_HIP(c, hipGetLastError());
   if (tm) tm->mark(c->stream, "diff_balance");
-  hipLaunchKernelGGL(k_diff_balance, dim3(1), dim3(TB), 0, c->stream, w.part, nblk, w.bal);
+  hipLaunchKernelGGL(k_diff_balance, dim3(1), dim3(BT), 0, c->stream, reinterpret_cast<const double4*>(w.part),
+                     nblk, w.bal);
   A2C_HIP(c, hipGetLastError());
   if (tm) tm->mark(c->stream, nullptr);
   return A2C_OK;
@@ -282,8 +337,9 @@ int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, doubl
 
 // ------------------------------------------------------------------ no-alloc
 // a2c_fe_diffuse_device_noalloc: the diffusion step on device arrays with its
-// scratch in this file-scope array -- f for up to A2C_DIFFUSE_STATIC_MAX_H H
-// sites, then 4 balance partials per block of 256 sites, then the balance.
+// scratch in this file-scope array -- the two edge-term factors for up to
+// A2C_DIFFUSE_STATIC_MAX_H H sites, then 4 balance partials per block of 256
+// sites, then the balance.
 // Each GPU holds its own copy for the life of the process.  Every kernel
 // overwrites what it reads later (f per site, one partial triple per block,
 // the balance), so nothing needs clearing between calls; calls that overlap
@@ -291,7 +347,7 @@ int launch_diffuse(a2c_fe_ctx* c, double dt, double prefactor, double xLb, doubl
 namespace a2c {
 namespace {
 constexpr size_t STATIC_BLK = (A2C_DIFFUSE_STATIC_MAX_H + TB - 1) / TB;
-constexpr size_t STATIC_DOUBLES = A2C_DIFFUSE_STATIC_MAX_H + 4 * STATIC_BLK + 4;
+constexpr size_t STATIC_DOUBLES = 2 * (size_t)A2C_DIFFUSE_STATIC_MAX_H + 4 * STATIC_BLK + 4;
 static __device__ __align__(256) double scratch[STATIC_DOUBLES];
 
 constexpr int MAX_DEVICES = 64;
@@ -339,8 +395,8 @@ extern "C" int a2c_fe_diffuse_device_noalloc(a2c_fe_ctx* c, double dt, double nu
   if (nh == 0) return A2C_OK;
   double* base = nullptr;
   if (int rc = static_scratch(c, &base)) return rc;
-  const DiffuseScratch w{base, base + A2C_DIFFUSE_STATIC_MAX_H,
-                         d_balance ? d_balance : base + A2C_DIFFUSE_STATIC_MAX_H + 4 * STATIC_BLK};
+  const size_t MH = A2C_DIFFUSE_STATIC_MAX_H;
+  const DiffuseScratch w{base, base + 2 * MH, d_balance ? d_balance : base + 2 * MH + 4 * STATIC_BLK};
   const double pref = nu * std::exp(-Qm / (KB * c->desc.temperature));  // integrate.cpp:26
   if (int rc = launch_diffuse(c, dt, pref, xLb, xUb, d_gamma, d_full_h, w, d_x)) return rc;
   A2C_HIP(c, hipMemcpyAsync(c->dx, d_x, 8 * (size_t)nh, hipMemcpyDeviceToDevice, c->stream));

@@ -218,8 +218,8 @@ int a2c_fe_diffusion_balance(const a2c_fe_ctx* ctx, a2c_diffusion_balance* out);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): a2c_fe_diffuse on device-resident arrays that calls no
- * allocator.  Its scratch (f = gamma - ln(x/(1-x)) per H site and the balance
- * partials, ~8.5 MB) is a library-static device array, one per GPU for the
+ * allocator.  Its scratch (two edge-term factors per H site, from
+ * f = gamma - ln(x/(1-x)), and the balance partials, ~17 MB) is a library-static device array, one per GPU for the
  * life of the process, so n_h is limited to A2C_DIFFUSE_STATIC_MAX_H
  * (A2C_EINVAL beyond it; a2c_fe_diffuse has no limit).  d_gamma (n_h
  * doubles) and d_full_h (n_h int32) are inputs; d_x (n_h doubles, not the

@@ -89,6 +89,24 @@ def test_diffusion_parity_and_context_update():
     assert np.max(np.abs(xg2 - xo2)) <= 1e-14
 
 
+def test_diffusion_parity_with_sites_far_from_equilibrium():
+    """Sites whose f = gamma - ln(x/(1-x)) is beyond +-700 take the reference
+    form of the edge term (e^{f/2} would leave the double range); their
+    neighbors' increments are huge and clamped, as in the oracle."""
+    fe, orc, X, x = setup(seed=9)
+    nh = fe.n_h
+    gam = np.random.default_rng(4).normal(0.0, 0.8, nh)
+    gam[::17] = 800.0
+    gam[5::23] = -800.0
+    full = np.zeros(nh, np.int32)
+    args = (1e-13, 1e13, 0.23, 1e-3, 1 - 1e-16)
+    print("diffusion: %d sites at gamma = +-800" % (len(gam[::17]) + len(gam[5::23])), flush=True)
+    xg = fe.integrateDiffusion(*args, gam, full)
+    xo = orc.diffuse(*args, gam, full, x)
+    assert np.all(np.isfinite(xg)) and np.any(xg == 1e-3) and np.any(xg == 1 - 1e-16)
+    assert np.max(np.abs(xg - xo)) <= 1e-14
+
+
 def test_diffusion_conserves_total_h_at_size():
     """24^3 cells (55,296 H sites), no reservoir, no clamp: sum x changes
     only by round-off of the site sums."""
