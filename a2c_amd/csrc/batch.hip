@@ -1941,15 +1941,10 @@ int a2c_batch_create(const a2c_batch_desc* desc, a2c_batch** out) {
             dmalloc(&b->dres, 16 * cap) == hipSuccess && hmalloc(&b->hres, 16 * cap) == hipSuccess &&
             dmalloc(&b->dscal, sizeof(double) * SCAL_PER_SLOT * cap) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
-  std::vector<float4> t;
-  host_tables(t);
-  if (dmalloc(&b->dtab, sizeof(float4) * t.size()) != hipSuccess) return fail(A2C_ENOMEM);
-  if (hipMemcpy(b->dtab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice) != hipSuccess) return fail(A2C_EHIP);
-  std::vector<double> t64;
-  host_tables64(t64);
-  if (dmalloc(&b->dtab64, sizeof(double) * t64.size()) != hipSuccess) return fail(A2C_ENOMEM);
-  if (hipMemcpy(b->dtab64, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(A2C_EHIP);
+  DeviceTables tabs;
+  if (device_tables(desc->device, &tabs) != hipSuccess) return fail(A2C_EHIP);
+  b->dtab = tabs.t32;  // shared, not freed with the batch
+  b->dtab64 = tabs.t64;
   *out = b;
   return A2C_OK;
 }
@@ -1970,8 +1965,6 @@ void a2c_batch_destroy(a2c_batch* b) {
   (void)dfree(b->dscal);
   if (b->hres) (void)hfree(b->hres);
   (void)dfree(b->dpart);
-  (void)dfree(b->dtab);
-  (void)dfree(b->dtab64);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

@@ -18,6 +18,17 @@ int set_err(a2c_fe_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+// Pinned staging of the host-array entry points (X in, G out), allocated by
+// the first call that needs it: a context driven through device arrays, or
+// as a batch's lattice, pins no host memory.
+int host_staging(a2c_fe_ctx* c) {
+  if (c->hX && c->hG) return A2C_OK;
+  const size_t N = std::max(c->N, 1);
+  if ((!c->hX && hmalloc(&c->hX, 32 * N) != hipSuccess) || (!c->hG && hmalloc(&c->hG, 32 * N) != hipSuccess))
+    return set_err(c, A2C_ENOMEM, "pinned host staging");
+  return A2C_OK;
+}
+
 namespace {
 
 int check_ctx(a2c_fe_ctx* c) {
@@ -29,6 +40,7 @@ int check_ctx(a2c_fe_ctx* c) {
 int upload_state(a2c_fe_ctx* c, const double* X) {
   const size_t n = 4 * (size_t)c->N;
   if (n == 0) return A2C_OK;
+  if (int rc = host_staging(c)) return rc;
   std::memcpy(c->hX, X, n * sizeof(double));
   A2C_HIP(c, hipMemcpyAsync(c->dX, c->hX, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   return A2C_OK;
@@ -101,18 +113,14 @@ int a2c::fe_create(const a2c_fe_desc* desc, hipStream_t shared, a2c_fe_ctx** out
             dmalloc(&c->dx, 8 * nh) == hipSuccess && dmalloc(&c->drho, 8 * N) == hipSuccess &&
             dmalloc(&c->dFp, 8 * N) == hipSuccess && dmalloc(&c->demb, 8 * N) == hipSuccess &&
             dmalloc(&c->dpart, 8 * (size_t)c->nblk_red) == hipSuccess && dmalloc(&c->dF, 64) == hipSuccess &&
-            hmalloc(&c->hX, 32 * N) == hipSuccess && hmalloc(&c->hG, 32 * N) == hipSuccess &&
-            hmalloc(&c->hlat, sizeof(LatDev)) == hipSuccess && dmalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
-            dmalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess &&
-            dmalloc(&c->dtab64, sizeof(double) * T64_SIZE) == hipSuccess;
+            dmalloc(&c->dlat, sizeof(LatDev)) == hipSuccess &&
+            dmalloc(&c->db2l, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) == hipSuccess;
   if (!ok) return fail(A2C_ENOMEM);
   {
-    std::vector<double> t64;
-    host_tables64(t64);
-    if (hipMemcpy(c->dtab64, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice) != hipSuccess)
-      return fail(A2C_EHIP);
+    DeviceTables t;
+    if (device_tables(desc->device, &t) != hipSuccess) return fail(A2C_EHIP);
+    c->dtab64 = t.t64;  // shared, not freed with the context
   }
-  new (c->hlat) LatDev();
   // all-zero block -> lattice map, long enough for the pair-owner launches too
   if (hipMemset(c->db2l, 0, 4 * (size_t)c->nblk_red * MAX_LANES_PER_SITE) != hipSuccess) return fail(A2C_EHIP);
   c->lat_dirty = true;
@@ -127,8 +135,7 @@ void a2c_fe_destroy(a2c_fe_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   dfree(c->dX); dfree(c->dG); dfree(c->dx); dfree(c->drho); dfree(c->dFp);
-  dfree(c->demb); dfree(c->dpart); dfree(c->dF); dfree(c->dpdpdnum); dfree(c->dtab); dfree(c->daux);
-  dfree(c->dtab64);
+  dfree(c->demb); dfree(c->dpart); dfree(c->dF); dfree(c->dpdpdnum); dfree(c->daux);  // dtab, dtab64: shared
   gen_free(c);
   for (int w = 0; w < 5; ++w) {
     dfree(c->ell[w].idx); dfree(c->ell[w].cnt);
@@ -176,7 +183,9 @@ int a2c_fe_set_occupancy(a2c_fe_ctx* c, const double* x) {
 int a2c_fe_build_neighbors(a2c_fe_ctx* c, const double* X, a2c_nbr_stats* stats) {
   if (int rc = check_ctx(c)) return rc;
   if (!X && c->N > 0) return set_err(c, A2C_EINVAL, "X is null");
-  if (int rc = upload_state(c, X)) return rc;
+  // once per build: straight from the caller's array (ordered on the stream;
+  // the build synchronises before it returns), no pinned staging
+  if (c->N > 0) A2C_HIP(c, hipMemcpyAsync(c->dX, X, 32 * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
   c->lat_dirty = true;
   if (int rc = nbr_build(c)) return rc;
   if (stats) *stats = c->stats;
@@ -235,7 +244,7 @@ int a2c_fe_eval(a2c_fe_ctx* c, const double* X, double* f, double* G) {
   if (!f) return set_err(c, A2C_EINVAL, "null f");
   if (int rc = upload_state(c, X)) return rc;
   if (int rc = eval_dispatch(c, c->dX, c->dG, c->dF, false, nullptr)) return rc;
-  const size_t n = 4 * (size_t)c->N;
+  const size_t n = 4 * (size_t)c->N;  // upload_state allocated hG with hX
   if (n) A2C_HIP(c, hipMemcpyAsync(c->hG, c->dG, 8 * n, hipMemcpyDeviceToHost, c->stream));
   double fh = 0.0;
   A2C_HIP(c, hipMemcpyAsync(&fh, c->dF, 8, hipMemcpyDeviceToHost, c->stream));

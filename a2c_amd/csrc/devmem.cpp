@@ -170,4 +170,35 @@ hipError_t hfree(const void* ptr) {
   return e;
 }
 
+hipError_t device_tables(int device, DeviceTables* out) {
+  static std::mutex mu;
+  static std::map<int, DeviceTables> made;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = made.find(device);
+  if (it == made.end()) {
+    int cur = 0;
+    if (hipError_t e = hipGetDevice(&cur)) return e;
+    if (cur != device)
+      if (hipError_t e = hipSetDevice(device)) return e;
+    std::vector<float4> t32;
+    std::vector<double> t64;
+    host_tables(t32);
+    host_tables64(t64);
+    DeviceTables d;
+    hipError_t e = hipMalloc(&d.t32, sizeof(float4) * t32.size());
+    if (e == hipSuccess) e = hipMalloc(&d.t64, sizeof(double) * t64.size());
+    if (e == hipSuccess) e = hipMemcpy(d.t32, t32.data(), sizeof(float4) * t32.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.t64, t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice);
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+      if (d.t32) (void)hipFree(d.t32);
+      if (d.t64) (void)hipFree(d.t64);
+      return e;
+    }
+    it = made.emplace(device, d).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
+
 }  // namespace a2c

@@ -924,21 +924,24 @@ void fill_table(float4* out, Fn g) {
 }  // namespace
 
 void host_tables(std::vector<float4>& t) {
-  using LD = long double;
-  t.assign((size_t)NTAB * NT, make_float4(0.f, 0.f, 0.f, 0.f));
-  fill_table(&t[T_FPD * NT], [](LD r) { return eam::f_Pd_vd<LD>(r); });
-  fill_table(&t[T_FH * NT], [](LD r) { return eam::f_H_vd<LD>(r); });
-  fill_table(&t[T_PHPD * NT], [](LD r) { return eam::phi_Pd_vd<LD>(r, eam::f_Pd_vd<LD>(r)); });
-  fill_table(&t[T_PHPDH * NT], [](LD r) { return eam::phi_morse_vd<0, LD>(r); });
-  fill_table(&t[T_PHH * NT], [](LD r) { return eam::phi_morse_vd<1, LD>(r); });
+  static const std::vector<float4> built = [] {  // once per process (~0.4 ms of long-double work)
+    using LD = long double;
+    std::vector<float4> u((size_t)NTAB * NT, make_float4(0.f, 0.f, 0.f, 0.f));
+    fill_table(&u[T_FPD * NT], [](LD r) { return eam::f_Pd_vd<LD>(r); });
+    fill_table(&u[T_FH * NT], [](LD r) { return eam::f_H_vd<LD>(r); });
+    fill_table(&u[T_PHPD * NT], [](LD r) { return eam::phi_Pd_vd<LD>(r, eam::f_Pd_vd<LD>(r)); });
+    fill_table(&u[T_PHPDH * NT], [](LD r) { return eam::phi_morse_vd<0, LD>(r); });
+    fill_table(&u[T_PHH * NT], [](LD r) { return eam::phi_morse_vd<1, LD>(r); });
+    return u;
+  }();
+  t = built;
 }
 
 int build_tables(a2c_fe_ctx* c) {
-  std::vector<float4> t;
-  host_tables(t);
-  if (!c->dtab) A2C_HIP(c, dmalloc(&c->dtab, sizeof(float4) * t.size()));
+  DeviceTables t;
+  A2C_HIP(c, device_tables(c->desc.device, &t));
+  c->dtab = t.t32;
   if (!c->daux) A2C_HIP(c, dmalloc(&c->daux, sizeof(float2) * std::max(c->N, 1)));
-  A2C_HIP(c, hipMemcpy(c->dtab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice));
   return A2C_OK;
 }
 

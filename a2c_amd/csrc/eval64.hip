@@ -968,6 +968,11 @@ int launch_eval32(a2c_fe_ctx* c, const double* dX, double* dG, double* dF, bool 
 // Refresh the device descriptor of the context's lattice when the state,
 // gradient or objective buffers (or the neighbor lists) changed.
 int sync_lattice(a2c_fe_ctx* c, const double* dX, double* dG, double* dF) {
+  if (!c->hlat) {  // pinned, at the first single-lattice launch (a batch's lattices never need it)
+    A2C_HIP(c, hmalloc(&c->hlat, sizeof(LatDev)));
+    new (c->hlat) LatDev();
+    c->lat_dirty = true;
+  }
   LatDev& h = *c->hlat;
   const int kl32 = lanes_per_site(c->N, false), kl64 = lanes_per_site(c->N, true);
   if (!c->lat_dirty && h.X == dX && h.G == dG && h.f == dF && h.kl32 == kl32 && h.kl64 == kl64) return A2C_OK;

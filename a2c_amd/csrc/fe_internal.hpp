@@ -380,6 +380,15 @@ enum { T64Q_FPD = 0, T64Q_PHPD = 1 };
 constexpr int T64_PART1 = T64_NF * T64_N * T64_P;                // doubles
 constexpr int T64_SIZE = T64_PART1 + T64Q_NF * T64_N * T64Q_P;  // doubles
 void host_tables64(std::vector<double>& t);
+// The FP32 and FP64 EAM tables on one device, uploaded at the first call for
+// that device and shared by every context and batch of the process until it
+// ends (they depend on nothing but the compiled-in potential); never freed.
+struct DeviceTables {
+  float4* t32 = nullptr;
+  double* t64 = nullptr;
+};
+hipError_t device_tables(int device, DeviceTables* out);
+int host_staging(a2c_fe_ctx* c);  // pinned hX, hG of the host-array entry points (capi.cpp)
 // General model (evalgen.hip, general.cpp)
 int run_density_gen(const LatDev* dlat, int N, const GenDev& g, int deriv_mode, double eps, hipStream_t st);
 int run_eval_gen(const LatDev* dlat, int N, const GenDev& g, int deriv_mode, double eps, bool fp32, hipStream_t st,

@@ -42,6 +42,7 @@ extern "C" int a2c_fe_chem_potential(a2c_fe_ctx* c, const double* X, const doubl
   double *dg = c->occ_g, *dcorr = c->occ_corr;
   int32_t* dfull = c->occ_full;
   if (N) {
+    if (int rc = host_staging(c)) return rc;
     std::copy(X, X + 4 * N, c->hX);
     A2C_HIP(c, hipMemcpyAsync(c->dX, c->hX, 32 * N, hipMemcpyHostToDevice, c->stream));
   }
