@@ -205,3 +205,49 @@ def test_mixed_path_below_its_tables_falls_back_to_fp64():
     _, rep = fe.minimizeFreeEntropy(X, gatol=1e-6, grtol=0.0, max_it=400)
     assert rep["reason"] != -4  # not stopped on the non-finite mixed value
     assert rep["evals"] - rep["evals_fp64"] <= 1  # only the first evaluation was mixed
+
+
+def test_contexts_and_batches_share_the_tables_and_stage_lazily():
+    """The EAM tables live once per device: destroying a context or a batch
+    leaves every other context's results bitwise unchanged.  A context first
+    driven through device arrays allocates its pinned host staging at its
+    first host-array call, with the same result."""
+    n, per, disp, sig, xr, T, seed = CASES[1]
+    X, x, box, per3, npd, nh = specimen(n, seed=seed, disp=disp, sigma=sig, x=xr, periodic=per)
+    print("shared tables: contexts and a batch created and destroyed around one context", flush=True)
+
+    def ctx(prec):
+        fe = A.FreeEntropy(npd, nh, box, per3, T=T, precision=prec)
+        fe.set_occupancy(x)
+        fe.findNeighbors(X)
+        return fe
+
+    keep = {p: ctx(p) for p in ("mixed", "fp64")}
+    ref = {p: fe.eval(X) for p, fe in keep.items()}
+    for p in ("mixed", "fp64"):
+        ctx(p).close()
+    with A.Batch(precision="mixed", max_lattices=2) as B:
+        sl = B.add(X, x, npd, nh, box, per3, T=T)
+        fb = B.eval([sl])[0]
+    assert abs(fb - ref["mixed"][0]) <= 1e-7 * abs(ref["mixed"][0])
+    for p, fe in keep.items():
+        f, G = fe.eval(X)
+        assert f == ref[p][0] and np.array_equal(G, ref[p][1]), p
+        fe.close()
+    # device arrays first, host arrays afterwards
+    fe = A.FreeEntropy(npd, nh, box, per3, T=T, precision="mixed")
+    fe.set_occupancy(x)
+    dX, dG, df = fe.dev_alloc(X.nbytes), fe.dev_alloc(X.nbytes), fe.dev_alloc(8)
+    try:
+        fe.upload(dX, X)
+        fe.findNeighbors_device(dX)
+        fe.eval_device(dX, dG, df)
+        Gd = fe.download(np.zeros(X.size), dG)
+        fd = fe.download(np.zeros(1), df)[0]
+    finally:
+        for p in (dX, dG, df):
+            fe.dev_free(p)
+    f, G = fe.eval(X)
+    fe.close()
+    assert f == fd and np.array_equal(G, Gd)
+    assert f == ref["mixed"][0] and np.array_equal(G, ref["mixed"][1])
