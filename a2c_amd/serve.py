@@ -36,6 +36,7 @@ id.  The comparator relaxes the same requests one at a time in FP64.
     python -m a2c_amd.serve --rates 5,10,20,40 --requests 200
 """
 import argparse
+import bisect
 import json
 import math
 import sys
@@ -119,6 +120,65 @@ def due_set(policy, running, vt):
     return due
 
 
+class Scheduler:
+    """due_set kept up to date incrementally, for the serving loop: a
+    request's iteration cost and deadline are computed once at admission and
+    the deadline (or arrival) order is a sorted list updated at admission and
+    retirement, so a tick sorts nothing.  due() returns what due_set returns
+    for the same running set (the walk stops once no running lattice can
+    fit the budget any more)."""
+
+    def __init__(self, policy):
+        if policy not in POLICIES:
+            raise ValueError("unknown policy %r (one of %s)" % (policy, ", ".join(POLICIES)))
+        self.policy = policy
+        self.slack = DEADLINE_SLACK if policy == "deadline" else 0.0
+        self.cost = {}  # slot -> iteration cost, in admission order
+        self.vt = {}  # slot -> virtual time ("fair")
+        self.order = []  # sorted (deadline, slot) ("deadline", "fcfs")
+        self.key = {}
+        self.min_part = math.inf  # lower bound of every running lattice's per-site part
+
+    def add(self, slot, req):
+        c = iter_cost(8 * req["n"] ** 3)
+        self.cost[slot] = c
+        self.min_part = min(self.min_part, c - ITER_FIXED_S)
+        if self.policy == "fair":
+            self.vt[slot] = min(self.vt.values()) if self.vt else 0.0
+        elif self.policy != "lockstep":
+            k = (deadline(req, self.slack), slot)
+            self.key[slot] = k
+            bisect.insort(self.order, k)
+
+    def remove(self, slot):
+        del self.cost[slot]
+        self.vt.pop(slot, None)
+        k = self.key.pop(slot, None)
+        if k is not None:
+            del self.order[bisect.bisect_left(self.order, k)]
+
+    def due(self):
+        if self.policy == "lockstep":
+            return list(self.cost)
+        if self.policy == "fair":
+            horizon = min(self.vt.values()) + min(self.cost.values())
+            due = [sl for sl in self.cost if self.vt[sl] <= horizon]
+            for sl in due:
+                self.vt[sl] += self.cost[sl]
+            return due
+        head = self.order[0][1]
+        cap = max((1.0 + RIDE_ALONG) * self.cost[head], TICK_MIN_S)
+        due, acc = [], ITER_FIXED_S
+        for _, sl in self.order:
+            c = self.cost[sl] - ITER_FIXED_S
+            if not due or acc + c <= cap:
+                due.append(sl)
+                acc += c
+            if acc + self.min_part > cap:
+                break
+        return due
+
+
 def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None, log=None, policy="deadline",
           keep_state=False):
     """Serve reqs (sorted by arrival) under one of POLICIES.  keep_state:
@@ -126,7 +186,7 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
     if policy not in POLICIES:
         raise ValueError("unknown policy %r (one of %s)" % (policy, ", ".join(POLICIES)))
     opts = opts or {"gatol": 1e-8, "grtol": 0.0}
-    vt = {}  # slot -> virtual time (seconds of GPU time received; "fair")
+    sched = Scheduler(policy)
     pending = list(reqs)
     running = {}  # slot -> request
     done = []
@@ -143,15 +203,16 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
                 X, x, box, npd, nh = req["state"] if "state" in req else materialize(req)
                 slot = B.add(X, x, npd, nh, box, (1, 1, 1), T=req["T"])
                 B.start(slot, **opts)
-                vt[slot] = min(vt.values()) if vt else 0.0
+                sched.add(slot, req)
                 req["admitted"] = time.perf_counter() - t0
                 running[slot] = req
                 pending.pop(0)
             if not running:
                 time.sleep(max(0.0, pending[0]["arrival"] - (time.perf_counter() - t0)))
                 continue
-            B.step_slots(due_set(policy, running, vt), 1)
-            for slot in list(running):
+            due = sched.due()
+            B.step_slots(due, 1)
+            for slot in due:  # only a stepped relaxation can have finished
                 rep = B.report(slot)
                 if rep["reason"] != 0:
                     req = running.pop(slot)
@@ -162,7 +223,7 @@ def serve(reqs, precision="mixed", budget=200 << 30, max_lattices=256, opts=None
                         req["X"] = B.get_state(slot)[0]
                     done.append(req)
                     B.remove(slot)
-                    vt.pop(slot, None)
+                    sched.remove(slot)
                     if log:
                         log(req)
     wall = time.perf_counter() - t0

@@ -105,3 +105,30 @@ def test_no_request_starves_under_a_stream_of_small_ones(policy):
     if policy == "fair":
         small = sorted(finish[i] - reqs[i]["arrival"] for i in range(1, 2000))
         assert small[-1] < 0.5  # equal GPU time: no small request waits on the big one
+
+
+@pytest.mark.parametrize("policy", S.POLICIES)
+def test_incremental_scheduler_matches_due_set(policy):
+    """The serving loop's Scheduler (costs and order kept from admission)
+    picks what due_set picks from scratch, through admissions and
+    retirements of requests of every size."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    sched, running, vt, nxt, t = S.Scheduler(policy), {}, {}, 0, 0.0
+    for tick in range(3000):
+        if rng.random() < 0.3 or not running:  # an arrival
+            t += rng.exponential(0.02)
+            r = req(nxt, int(round(np.exp(rng.uniform(np.log(6), np.log(48))))), t)
+            sched.add(nxt, r)
+            running[nxt] = r
+            vt[nxt] = min(vt.values()) if vt else 0.0
+            nxt += 1
+        due = sched.due()
+        assert due == S.due_set(policy, running, vt), tick
+        if policy == "fair":
+            assert sched.vt == vt
+        for sl in due:  # some of the stepped ones finish
+            if rng.random() < 0.05:
+                sched.remove(sl)
+                running.pop(sl)
+                vt.pop(sl)
