@@ -214,3 +214,23 @@ def test_host_program_takes_the_potential_file(setfl, tmp_path):
         assert r.returncode == 0, r.stderr
         out.append(float(re.search(r"f = (\S+)", r.stdout).group(1)))
     assert abs(out[1] - out[0]) <= 1e-7 * abs(out[0])
+
+
+def test_general_model_at_12_cubed(setfl):
+    """12^3 cells (13,824 sites, displacements 0.03 A, widths 0.08-0.15 A):
+    the setfl potential in FP64 and mixed precision and the 64-node table,
+    against the oracle at the FP64 bars (mixed: 1e-7 objective)."""
+    path, t = setfl
+    case = (12, True, 0.03, (0.08, 0.15), (0.0, 0.6), 500.0, 12)
+    print("general: 12^3 cells, setfl FP64 / mixed, 64-node table", flush=True)
+    fe, orc, X, x = pair(case, "tab", None, path, t)
+    with fe:
+        f64, _ = check64(fe, orc, X, x)
+        fe.set_precision("mixed")
+        fm, Gm = fe.eval(X)
+        fo, Go = orc.eval(X, x)
+        assert abs(fm - fo) <= 1e-7 * abs(fo)
+        assert np.linalg.norm(Gm - Go) <= 1e-4 * np.linalg.norm(Go)
+    fe, orc, X, x = pair(case, "eam", product_rule_2pt())
+    with fe:
+        check64(fe, orc, X, x)
