@@ -84,15 +84,22 @@ def test_objective_gradient_mixed_gh3(case):
 
 
 def test_mixed_matches_fp64_kernel_gh3_at_12_cubed():
-    """13,824 sites: mixed vs the FP64 kernel."""
+    """13,824 sites: the FP64 kernel against the oracle, mixed against both."""
     X, x, box, per3, npd, nh = specimen(12, seed=2, disp=0.03, sigma=(0.08, 0.15), x=(0.0, 0.6))
+    print("gh3: 12^3 cells, FP64 and mixed against the oracle", flush=True)
     fe = A.FreeEntropy(npd, nh, box, per3, T=500.0, quad="gh3")
     fe.set_occupancy(x)
     fe.findNeighbors(X)
     f64, G64 = fe.eval(X)
+    orc = O.Oracle(npd, nh, box, per3, T=500.0, threads=8, quad="gh3")
+    orc.build_neighbors(X)
+    fo, Go = orc.eval(X, x)
+    assert abs(f64 - fo) <= FP64_REL * abs(fo)
+    assert np.max(np.abs(G64 - Go)) <= FP64_GRAD * np.abs(Go).max()
     fe.set_precision("mixed")
     f32, G32 = fe.eval(X)
     assert abs(f32 - f64) <= MIXED_OBJ_TOL * abs(f64)
+    assert abs(f32 - fo) <= MIXED_OBJ_TOL * abs(fo)
     assert np.linalg.norm(G32 - G64) <= MIXED_GRAD_TOL * np.linalg.norm(G64)
 
 
